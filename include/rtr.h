@@ -1,0 +1,227 @@
+/*
+ * librtr -- MI355X (gfx950) ICP registration and voxel fusion, C ABI.
+ *
+ * Drop-in boundary for the registration path of ICCD/RealTime_Robot.  The
+ * reference reaches it from keyPointICP (RealTimeRobot/function.h:111-148):
+ *
+ *     pcl::IterativeClosestPoint<pcl::PointXYZ, pcl::PointXYZ> icp;  // :112
+ *     icp.setInputCloud(mcloud);                                      // :113
+ *     icp.setInputTarget(cloud);                                      // :114
+ *     icp.align(Final);                                               // :117
+ *     icp.getFinalTransformation();                                   // :119,123
+ *     pcl::transformPointCloud(*mcloud, *transformed, T);             // :123
+ *
+ * and merges transformed clouds into the scene map at
+ * RealTimeRobot.cpp:104-114.  Each entry point below names the call it
+ * replaces.  Conventions:
+ *   - every function returns an int status (RTR_OK = 0); rtr_last_error()
+ *     gives a thread-local message.  No C++ exception crosses the ABI;
+ *   - point arrays are host pointers to float xyz with a byte stride
+ *     (12 = packed xyz, 16 = pcl::PointXYZ with its padding);
+ *   - 4x4 transforms are double[16], column-major (Eigen's layout);
+ *   - a handle is not thread-safe; use one handle per host thread.  Each
+ *     handle owns one HIP stream on its device.
+ */
+#ifndef RTR_H
+#define RTR_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RTR_OK 0
+#define RTR_ERR_INVALID_ARG 1
+#define RTR_ERR_EMPTY_INPUT 2
+#define RTR_ERR_NO_CORRESPONDENCES 3
+#define RTR_ERR_CAPACITY 4
+#define RTR_ERR_DEVICE 5
+#define RTR_ERR_NOT_READY 6
+#define RTR_ERR_IO 7
+#define RTR_ERR_FORMAT 8
+
+/* correspondence metric */
+#define RTR_MODE_POINT_TO_POINT 0
+#define RTR_MODE_POINT_TO_PLANE 1
+
+/* convergence states (PCL DefaultConvergenceCriteria::ConvergenceState) */
+#define RTR_NOT_CONVERGED 0
+#define RTR_CONV_ITERATIONS 1
+#define RTR_CONV_TRANSFORM 2
+#define RTR_CONV_ABS_MSE 3
+#define RTR_CONV_REL_MSE 4
+#define RTR_FAIL_NO_CORRESPONDENCES 5
+
+/* Registration parameters.  rtr_params_default() gives PCL 1.8's defaults:
+ * 10 iterations, max correspondence distance sqrt(DBL_MAX), transformation
+ * epsilon 0, euclidean fitness epsilon -DBL_MAX, 3 minimum correspondences,
+ * point-to-point. */
+typedef struct {
+  int max_iterations;
+  int min_correspondences;
+  int mode;                           /* RTR_MODE_* */
+  int reserved0;
+  double max_correspondence_distance; /* metres */
+  double transformation_epsilon;      /* squared translation threshold; rotation 1-eps */
+  double rotation_epsilon;            /* cos threshold; <= 0 means 1 - transformation_epsilon */
+  double euclidean_fitness_epsilon;   /* relative MSE threshold */
+  float cell_size;                    /* grid cell edge; 0 = autotuned */
+  float cells_per_point;              /* grid cell budget per target point; 0 = autotuned */
+} rtr_params;
+
+/* Outcome of one registration. */
+typedef struct {
+  double T[16];        /* final transform, column-major */
+  double mse;          /* mean squared distance of the last iteration's correspondences */
+  double fitness_rmse; /* sqrt(mse): the run-time quality measure */
+  int64_t n_corr;      /* correspondences of the last iteration */
+  int iterations;
+  int converged;       /* PCL hasConverged(): criteria fired (incl. max iterations) */
+  int state;           /* RTR_CONV_* or RTR_FAIL_* */
+  int reserved0;
+} rtr_result;
+
+/* Phase timings of the last align, from HIP events on the handle's stream
+ * (filled only when profiling is enabled). */
+typedef struct {
+  float ms_total;      /* whole align on the device */
+  float ms_grid;       /* target grid build + source spatial sort */
+  float ms_icp;        /* persistent correspondence + reduction + solve kernel */
+  float ms_finish;     /* output transforms / fitness */
+  int64_t icp_launches;
+  int64_t iterations_total; /* sum of iterations over problems */
+} rtr_timing;
+
+const char* rtr_last_error(void);
+int rtr_version(void);
+int rtr_params_default(rtr_params* p);
+
+/* ------------------------------------------------------------------ */
+/* Single registration: pcl::IterativeClosestPoint<PointXYZ, PointXYZ> */
+/* ------------------------------------------------------------------ */
+typedef struct rtr_icp rtr_icp;
+
+/* `IterativeClosestPoint<PointXYZ,PointXYZ> icp;` (function.h:112) */
+int rtr_icp_create(rtr_icp** out, int device);
+int rtr_icp_destroy(rtr_icp* h);
+/* `icp.setInputCloud(mcloud)` / setInputSource (function.h:113).  Copies the points. */
+int rtr_icp_set_input_source(rtr_icp* h, const float* xyz, size_t n, size_t stride_bytes);
+/* `icp.setInputTarget(cloud)` (function.h:114).  Copies; the grid is rebuilt lazily. */
+int rtr_icp_set_input_target(rtr_icp* h, const float* xyz, size_t n, size_t stride_bytes);
+/* Target normals for point-to-plane (PCL PointNormal target, normal_x/y/z fields). */
+int rtr_icp_set_target_normals(rtr_icp* h, const float* nxyz, size_t n, size_t stride_bytes);
+/* pcl::NormalEstimation with setKSearch(k) and viewpoint vp on the current target. */
+int rtr_icp_compute_target_normals(rtr_icp* h, int k, const float vp[3]);
+/* setMaxCorrespondenceDistance / setMaximumIterations / setTransformationEpsilon /
+ * setEuclideanFitnessEpsilon, and the whole parameter block at once. */
+int rtr_icp_set_max_correspondence_distance(rtr_icp* h, double d);
+int rtr_icp_set_maximum_iterations(rtr_icp* h, int n);
+int rtr_icp_set_transformation_epsilon(rtr_icp* h, double e);
+int rtr_icp_set_euclidean_fitness_epsilon(rtr_icp* h, double e);
+int rtr_icp_set_params(rtr_icp* h, const rtr_params* p);
+int rtr_icp_get_params(rtr_icp* h, rtr_params* p);
+/* `icp.align(Final[, guess])` (function.h:117) followed by getFinalTransformation()
+ * (:119,123) and hasConverged().  guess may be NULL (identity).  out_xyz (nullable,
+ * n_source points at out_stride bytes) receives transformPointCloud(source, T)
+ * (function.h:123).  final_T and converged are nullable. */
+int rtr_icp_align(rtr_icp* h, const double guess[16], float* out_xyz, size_t out_stride,
+                  double final_T[16], int* converged);
+/* Full outcome of the last align (iterations, state, mse, correspondences). */
+int rtr_icp_get_result(rtr_icp* h, rtr_result* out);
+/* Registration::getFitnessScore(max_range): mean NN squared distance of the
+ * transformed source over points with d2 <= max_range; DBL_MAX if none. */
+int rtr_icp_get_fitness_score(rtr_icp* h, double max_range, double* out);
+/* CorrespondenceEstimation::determineCorrespondences for the source under T:
+ * idx[i] = target index (or -1 if d2 > max_dist^2), d2[i] = squared distance
+ * (+inf when rejected).  Ties go to the lowest target index. */
+int rtr_icp_get_correspondences(rtr_icp* h, const double T[16], int32_t* idx, float* d2);
+/* Origin used for the accumulated moments (point-to-plane linearisation point). */
+int rtr_icp_get_anchor(rtr_icp* h, float anchor[3]);
+int rtr_icp_set_profiling(rtr_icp* h, int on);
+int rtr_icp_get_timing(rtr_icp* h, rtr_timing* out);
+
+/* ------------------------------------------------------------------ */
+/* Batched registration: many (source, target, guess) problems in one  */
+/* persistent launch over a work queue, replayed from a hipGraph.       */
+/* No reference counterpart (the reference registers one pair).        */
+/* ------------------------------------------------------------------ */
+typedef struct rtr_batch rtr_batch;
+
+int rtr_batch_create(rtr_batch** out, int device);
+int rtr_batch_destroy(rtr_batch* b);
+/* Upload clouds once; they stay resident in HBM.  src[i]/tgt[j] are packed xyz. */
+int rtr_batch_set_clouds(rtr_batch* b, int n_src, const float* const* src, const size_t* ns,
+                         int n_tgt, const float* const* tgt, const size_t* nt);
+/* Optional per-target normals (packed nxyz, same counts as the targets). */
+int rtr_batch_set_target_normals(rtr_batch* b, const float* const* nrm);
+/* Compute k-NN PCA normals for every target on the device. */
+int rtr_batch_compute_target_normals(rtr_batch* b, int k, const float vp[3]);
+/* Problems: problem p registers source src_id[p] onto target tgt_id[p]. */
+int rtr_batch_set_problems(rtr_batch* b, int n, const int* src_id, const int* tgt_id);
+int rtr_batch_set_params(rtr_batch* b, const rtr_params* p);
+/* Enable/disable hipGraph replay of align (default on). */
+int rtr_batch_set_graph(rtr_batch* b, int on);
+int rtr_batch_set_profiling(rtr_batch* b, int on);
+/* Run every problem from guesses (n*16 doubles, nullable = identity);
+ * results (nullable, n entries). */
+int rtr_batch_align(rtr_batch* b, const double* guesses, rtr_result* results);
+int rtr_batch_get_timing(rtr_batch* b, rtr_timing* out);
+/* Grid parameters chosen for target j: cell edge, dims, anchor. */
+int rtr_batch_get_grid(rtr_batch* b, int tgt, float* cell, int dims[3], float anchor[3]);
+/* Fitness score of every problem at its final transform (see rtr_icp_get_fitness_score). */
+int rtr_batch_fitness(rtr_batch* b, double max_range, double* out);
+
+/* ------------------------------------------------------------------ */
+/* Voxel-hash map: merging transformed clouds into the scene map.       */
+/* Replaces transformPointCloud + savePCDFileASCII + viewer overlay     */
+/* (RealTimeRobot.cpp:104-114).  key = floor(p / voxel) per axis.       */
+/* ------------------------------------------------------------------ */
+typedef struct rtr_map rtr_map;
+
+int rtr_map_create(rtr_map** out, int device, float voxel_size, size_t capacity);
+int rtr_map_destroy(rtr_map* m);
+int rtr_map_clear(rtr_map* m);
+/* Insert T * xyz (T nullable = identity): per voxel FP32 sum of xyz and count. */
+int rtr_map_integrate(rtr_map* m, const float* xyz, size_t n, size_t stride_bytes,
+                      const double T[16]);
+int rtr_map_size(rtr_map* m, size_t* n_voxels);
+/* Copy out occupied voxels sorted by key: keys (packed 21 bits/axis, +2^20
+ * bias), sums (3 per voxel) or centroids when as_centroids != 0, counts. */
+int rtr_map_extract(rtr_map* m, int64_t* keys, float* xyz, uint32_t* counts, size_t cap,
+                    size_t* n_out, int as_centroids);
+/* Points dropped so far (non-finite or key out of range). */
+int rtr_map_dropped(rtr_map* m, uint64_t* n);
+
+/* PCL VoxelGrid downsample on the device: centroids ordered by linear voxel
+ * index over the cloud's bounding box (x fastest). */
+int rtr_voxel_downsample(int device, const float* xyz, size_t n, size_t stride_bytes, float leaf,
+                         float* out_xyz, size_t cap, size_t* n_out);
+
+/* ------------------------------------------------------------------ */
+/* PCD v0.7 I/O (pcl::io::loadPCDFile / savePCDFileASCII,               */
+/* RealTimeRobot.cpp:34-35,108-109; function.h:126-127)                 */
+/* ------------------------------------------------------------------ */
+/* Number of points and whether normal_x/y/z fields exist. */
+int rtr_pcd_info(const char* path, size_t* n_points, int* has_normals);
+/* Read xyz (packed, cap points) and optionally normals (nullable). */
+int rtr_pcd_read(const char* path, float* xyz, float* normals, size_t cap, size_t* n_out);
+/* ASCII writer with 8 significant digits, like savePCDFileASCII. */
+int rtr_pcd_write_ascii(const char* path, const float* xyz, size_t n, size_t stride_bytes);
+
+/* ------------------------------------------------------------------ */
+/* Self-check hooks: run the library's own geometry code on the host.   */
+/* ------------------------------------------------------------------ */
+/* The grid NN search on the CPU over the same cell-sorted layout. */
+int rtr_selfcheck_nn(const float* tgt, size_t nt, const float* q, size_t nq, float cell,
+                     double max_dist, int32_t* idx, float* d2);
+/* Reduce one iteration's moments and solve (FP64) on the CPU:
+ * a[i] (transformed source), b[i] (target), nrm (nullable) -> Tinc. */
+int rtr_selfcheck_solve(int mode, const float* a, const float* b, const float* nrm, size_t n,
+                        const float anchor[3], double Tinc[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+// Device-side view of a registration engine, passed by value to kernels,
+// and the launchers implemented in kernels_icp.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtr_types.hpp"
+
+namespace rtr {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+
+struct DevEngine {
+  // targets: raw packed xyz, descriptors, cell-sorted points and grids
+  const float* tgt_raw;
+  const float* tgt_nrm_raw;  // packed normals aligned with tgt_raw (nullable)
+  TargetDesc* tgts;
+  int n_tgt;
+  float4* tgt_sorted;        // (x, y, z, original index bits)
+  float4* tgt_nrm_sorted;    // normals in sorted order (nullable)
+  uint32_t* cell_count;
+  uint32_t* cell_start;
+  uint32_t* cell_id;         // per raw target point
+  uint32_t n_cells_total;    // sum of (budget + 1)
+  // sources: raw packed xyz, spatially sorted SoA and the permutation
+  const float* src_raw;
+  SourceDesc* srcs;
+  int n_src;
+  float* sx;
+  float* sy;
+  float* sz;
+  uint32_t* perm;            // sorted position -> original index
+  uint32_t* bin_count;
+  uint32_t* bin_start;
+  uint32_t* bin_id;          // per raw source point
+  uint32_t n_bins_total;
+  // problems and the work queue
+  const Problem* probs;
+  int n_prob;
+  ProblemState* st;
+  const double* guesses;     // n_prob * 16 (column-major)
+  const uint2* items;        // (problem, first sorted source point)
+  int n_items;
+  int item_pts;              // points per item
+  float* partials;           // n_items * NSUM_MAX * 2 floats
+  uint32_t* claim;           // one claim counter per pass
+  uint32_t* done;            // per problem: items finished so far
+  uint32_t* problems_done;
+  int* error;                // nonzero: a bounded wait expired
+  uint32_t* scan_tmp;        // block sums for the scans
+  IcpParams prm;
+};
+
+// launchers (stream-ordered, no host synchronisation)
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
+                           hipStream_t s);
+size_t scan_tmp_words(uint32_t n);
+void launch_target_grid(const DevEngine& e, int max_tgt_pts, hipStream_t s);
+void launch_source_sort(const DevEngine& e, int max_src_pts, hipStream_t s);
+void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s);
+int icp_blocks_per_cu(int mode);
+// correspondences / fitness / transformed output for problem set; T per
+// problem from `T` (n_prob*16 column-major, device) or from the state when null.
+void launch_correspondences(const DevEngine& e, int prob, const double* T, float r2f, int32_t* idx,
+                            float* d2, hipStream_t s);
+void launch_transform_out(const DevEngine& e, int prob, float* out, hipStream_t s);
+void launch_fitness_d2(const DevEngine& e, float* d2_all, hipStream_t s);
+void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, float vpz,
+                           int max_tgt_pts, hipStream_t s);
+
+}  // namespace rtr

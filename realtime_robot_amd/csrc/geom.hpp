@@ -1,0 +1,463 @@
+// Geometry core shared by the HIP kernels and the host-side self-check hooks.
+//
+// Everything here is __host__ __device__ so that the exact same source runs
+// inside the gfx950 kernels and, through the rtr_selfcheck_* entry points, on
+// the CPU.  The library is compiled with -ffp-contract=off: every float
+// expression below rounds exactly as written, on both sides.
+//
+// Reference behaviour restated here (PCL 1.8.0 is not part of the reference
+// tree; it is reached from RealTimeRobot/function.h:111-123):
+//   * squared distance: FLANN L2_Simple<float>, ((dx*dx) + dy*dy) + dz*dz
+//   * correspondence rejection: distance > max_dist^2 rejects (strict)
+//   * transformPointCloud: row-wise ((m0*x + m1*y) + m2*z) + m3 in float
+//   * DefaultConvergenceCriteria with max_iterations_similar_transforms_ = 0
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtr_types.hpp"
+
+namespace rtr {
+
+#define RTR_HD __host__ __device__ __forceinline__
+
+RTR_HD float d2_f32(float qx, float qy, float qz, float tx, float ty, float tz) {
+  float dx = qx - tx;
+  float dy = qy - ty;
+  float dz = qz - tz;
+  float s = dx * dx;
+  s = s + dy * dy;
+  s = s + dz * dz;
+  return s;
+}
+
+RTR_HD void apply_f32(const float* m, float x, float y, float z, float& ox, float& oy, float& oz) {
+  float a = m[0] * x;
+  a = a + m[1] * y;
+  a = a + m[2] * z;
+  ox = a + m[3];
+  float b = m[4] * x;
+  b = b + m[5] * y;
+  b = b + m[6] * z;
+  oy = b + m[7];
+  float c = m[8] * x;
+  c = c + m[9] * y;
+  c = c + m[10] * z;
+  oz = c + m[11];
+}
+
+// rows 0..2 of a column-major double 4x4 as float (PCL's Matrix4f)
+RTR_HD void tf_rows_f32(const double* T, float* m) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 4; ++c) m[r * 4 + c] = (float)T[c * 4 + r];
+}
+
+// Lexicographic (d2, original index) order packed in one 64-bit key: d2 >= 0
+// so its IEEE bits order like the value.
+RTR_HD uint64_t nn_key(float d2, int32_t idx) {
+  union {
+    float f;
+    uint32_t u;
+  } v;
+  v.f = d2;
+  return ((uint64_t)v.u << 32) | (uint32_t)idx;
+}
+
+struct NNBest {
+  uint64_t key;  // (d2 bits, original index); ~0 = none
+  float tx, ty, tz;
+  uint32_t slot;  // position in the cell-sorted target array
+};
+
+RTR_HD void scan_range(const float4* __restrict__ pts, uint32_t b, uint32_t e, float qx, float qy,
+                       float qz, NNBest& best) {
+  for (uint32_t j = b; j < e; ++j) {
+    float4 t = pts[j];
+    float d = d2_f32(qx, qy, qz, t.x, t.y, t.z);
+    union {
+      float f;
+      int32_t i;
+    } w;
+    w.f = t.w;
+    uint64_t k = nn_key(d, w.i);
+    if (k < best.key) {
+      best.key = k;
+      best.tx = t.x;
+      best.ty = t.y;
+      best.tz = t.z;
+      best.slot = j;
+    }
+  }
+}
+
+RTR_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Ring walk over a uniform grid of cell-sorted points.
+//
+// Rings of cells at Chebyshev distance k = 0, 1, 2, ... around the query's
+// (clamped) cell are handed to vis.scan(begin, end) as contiguous ranges of
+// the sorted array (an x-row of cells is contiguous), until every unsearched
+// cell is provably farther than vis.bound() or than the acceptance bound r2f.
+// The bound is the distance from q to the faces of the searched box, less
+// g.margin, which absorbs the float rounding of the cell assignment
+// floor((p - o) * inv_h); squared and scaled by (1 - 2e-6) it undercuts the
+// float d2 of any unsearched point (float d2 is within 6 ulp of exact).
+// Rings 0 and 1 are walked together as the nine x-rows of the 3x3x3 block.
+template <class V>
+RTR_HD void ring_walk(const GridView& g, float qx, float qy, float qz, float r2f, V& vis) {
+  float fx = (qx - g.ox) * g.inv_h;
+  float fy = (qy - g.oy) * g.inv_h;
+  float fz = (qz - g.oz) * g.inv_h;
+  if (!(fx == fx) || !(fy == fy) || !(fz == fz)) return;  // NaN query
+  if (g.n_pts == 0) return;
+  // clamp in float before converting (far queries must not overflow int)
+  const float lim = 1.0e9f;
+  fx = fx < -1.f ? -1.f : (fx > lim ? lim : fx);
+  fy = fy < -1.f ? -1.f : (fy > lim ? lim : fy);
+  fz = fz < -1.f ? -1.f : (fz > lim ? lim : fz);
+  const int cx = clampi((int)floorf(fx), 0, g.dx - 1);
+  const int cy = clampi((int)floorf(fy), 0, g.dy - 1);
+  const int cz = clampi((int)floorf(fz), 0, g.dz - 1);
+  const int kmax = (g.dx > g.dy ? (g.dx > g.dz ? g.dx : g.dz) : (g.dy > g.dz ? g.dy : g.dz));
+  for (int k = 1; k <= kmax; ++k) {
+    if (k == 1) {
+      const int x0 = cx > 0 ? cx - 1 : 0;
+      const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.dz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int y = cy + dy;
+          if (y < 0 || y >= g.dy) continue;
+          const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+          vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
+        }
+      }
+    } else {
+      for (int dz = -k; dz <= k; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.dz) continue;
+        const bool zface = (dz == -k || dz == k);
+        for (int dy = -k; dy <= k; ++dy) {
+          const int y = cy + dy;
+          if (y < 0 || y >= g.dy) continue;
+          const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+          if (zface || dy == -k || dy == k) {
+            const int x0 = clampi(cx - k, 0, g.dx - 1);
+            const int x1 = clampi(cx + k, 0, g.dx - 1);
+            vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
+          } else {
+            if (cx - k >= 0) vis.scan(g.cell_start[row + cx - k], g.cell_start[row + cx - k + 1]);
+            if (cx + k < g.dx) vis.scan(g.cell_start[row + cx + k], g.cell_start[row + cx + k + 1]);
+          }
+        }
+      }
+    }
+    // lower bound on the distance from q to any cell outside the searched box
+    float lb = 3.0e38f;
+    bool any = false;
+    if (cx - k > 0) {
+      lb = fminf(lb, qx - (g.ox + (float)(cx - k) * g.h));
+      any = true;
+    }
+    if (cx + k < g.dx - 1) {
+      lb = fminf(lb, (g.ox + (float)(cx + k + 1) * g.h) - qx);
+      any = true;
+    }
+    if (cy - k > 0) {
+      lb = fminf(lb, qy - (g.oy + (float)(cy - k) * g.h));
+      any = true;
+    }
+    if (cy + k < g.dy - 1) {
+      lb = fminf(lb, (g.oy + (float)(cy + k + 1) * g.h) - qy);
+      any = true;
+    }
+    if (cz - k > 0) {
+      lb = fminf(lb, qz - (g.oz + (float)(cz - k) * g.h));
+      any = true;
+    }
+    if (cz + k < g.dz - 1) {
+      lb = fminf(lb, (g.oz + (float)(cz + k + 1) * g.h) - qz);
+      any = true;
+    }
+    if (!any) break;  // the whole grid has been searched
+    lb = lb - g.margin;
+    if (lb > 0.f) {
+      const float lb2 = (lb * lb) * 0.999998f;
+      if (vis.bound() < lb2) break;
+      if (lb2 > r2f) break;
+    }
+  }
+}
+
+RTR_HD float key_d2(uint64_t key) {
+  union {
+    uint32_t u;
+    float f;
+  } bd;
+  bd.u = (uint32_t)(key >> 32);
+  return bd.f;
+}
+
+struct NNVisitor {
+  const float4* pts;
+  float qx, qy, qz;
+  NNBest best;
+  RTR_HD void scan(uint32_t b, uint32_t e) { scan_range(pts, b, e, qx, qy, qz, best); }
+  RTR_HD float bound() const { return best.key == ~0ull ? 3.4e38f : key_d2(best.key); }
+};
+
+// Exact nearest neighbour (lexicographic (d2, index) minimum) accepted iff
+// d2 <= r2f, which equals PCL's (double)d2 <= r_max^2 when r2f is r_max^2
+// rounded down to float.  key == ~0 when none.
+RTR_HD NNBest nn_search(const GridView& g, float qx, float qy, float qz, float r2f) {
+  NNVisitor v;
+  v.pts = g.pts;
+  v.qx = qx;
+  v.qy = qy;
+  v.qz = qz;
+  v.best.key = ~0ull;
+  v.best.tx = v.best.ty = v.best.tz = 0.f;
+  v.best.slot = 0xffffffffu;
+  ring_walk(g, qx, qy, qz, r2f, v);
+  if (v.best.key != ~0ull && !(key_d2(v.best.key) <= r2f)) v.best.key = ~0ull;
+  return v.best;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 small solves
+// ---------------------------------------------------------------------------
+
+// Cyclic Jacobi eigen-decomposition of a symmetric 4x4 (row-major, in place):
+// on return the diagonal holds eigenvalues, columns of V the eigenvectors.
+RTR_HD void jacobi4(double* a, double* V) {
+  for (int i = 0; i < 16; ++i) V[i] = (i % 5 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < 4; ++p) {
+      diag += fabs(a[p * 5]);
+      for (int q = p + 1; q < 4; ++q) off += fabs(a[p * 4 + q]);
+    }
+    if (off == 0.0 || off <= 1e-20 * diag) break;
+    for (int p = 0; p < 3; ++p)
+      for (int q = p + 1; q < 4; ++q) {
+        const double apq = a[p * 4 + q];
+        if (apq == 0.0) continue;
+        const double theta = (a[q * 5] - a[p * 5]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 4; ++k) {
+          const double akp = a[k * 4 + p], akq = a[k * 4 + q];
+          a[k * 4 + p] = c * akp - s * akq;
+          a[k * 4 + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 4; ++k) {
+          const double apk = a[p * 4 + k], aqk = a[q * 4 + k];
+          a[p * 4 + k] = c * apk - s * aqk;
+          a[q * 4 + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 4; ++k) {
+          const double vkp = V[k * 4 + p], vkq = V[k * 4 + q];
+          V[k * 4 + p] = c * vkp - s * vkq;
+          V[k * 4 + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+// Rigid fit from first and second moments (Horn 1987, closed form via unit
+// quaternions): n pairs (a_i -> b_i) with sums Sa, Sb and Sab[r*3+c] =
+// sum a_r b_c, all relative to a common origin.  Produces R, t with
+// b ~ R a + t in that origin's frame.  The quaternion of the largest
+// eigenvalue of Horn's symmetric 4x4 N is a proper rotation, which is the
+// reflection-guarded Kabsch/Umeyama solution.
+RTR_HD void rigid_horn(double n, const double* Sa, const double* Sb, const double* Sab, double* R,
+                       double* t) {
+  double ma[3], mb[3], M[9];
+  for (int i = 0; i < 3; ++i) {
+    ma[i] = Sa[i] / n;
+    mb[i] = Sb[i] / n;
+  }
+  // centred cross-covariance M[r][c] = sum (a_r - ma_r)(b_c - mb_c) / n
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) M[r * 3 + c] = Sab[r * 3 + c] / n - ma[r] * mb[c];
+  const double Sxx = M[0], Sxy = M[1], Sxz = M[2];
+  const double Syx = M[3], Syy = M[4], Syz = M[5];
+  const double Szx = M[6], Szy = M[7], Szz = M[8];
+  double N[16] = {Sxx + Syy + Szz, Syz - Szy,        Szx - Sxz,        Sxy - Syx,
+                  Syz - Szy,       Sxx - Syy - Szz,  Sxy + Syx,        Szx + Sxz,
+                  Szx - Sxz,       Sxy + Syx,        -Sxx + Syy - Szz, Syz + Szy,
+                  Sxy - Syx,       Szx + Sxz,        Syz + Szy,        -Sxx - Syy + Szz};
+  double V[16];
+  jacobi4(N, V);
+  int best = 0;
+  for (int i = 1; i < 4; ++i)
+    if (N[i * 5] > N[best * 5]) best = i;
+  double q0 = V[0 * 4 + best], qx = V[1 * 4 + best], qy = V[2 * 4 + best], qz = V[3 * 4 + best];
+  const double nq = sqrt(q0 * q0 + qx * qx + qy * qy + qz * qz);
+  q0 /= nq;
+  qx /= nq;
+  qy /= nq;
+  qz /= nq;
+  R[0] = q0 * q0 + qx * qx - qy * qy - qz * qz;
+  R[1] = 2.0 * (qx * qy - q0 * qz);
+  R[2] = 2.0 * (qx * qz + q0 * qy);
+  R[3] = 2.0 * (qy * qx + q0 * qz);
+  R[4] = q0 * q0 - qx * qx + qy * qy - qz * qz;
+  R[5] = 2.0 * (qy * qz - q0 * qx);
+  R[6] = 2.0 * (qz * qx - q0 * qy);
+  R[7] = 2.0 * (qz * qy + q0 * qx);
+  R[8] = q0 * q0 - qx * qx - qy * qy + qz * qz;
+  for (int r = 0; r < 3; ++r) t[r] = mb[r] - (R[r * 3] * ma[0] + R[r * 3 + 1] * ma[1] + R[r * 3 + 2] * ma[2]);
+}
+
+RTR_HD int up6(int r, int c) { return r * 6 - r * (r - 1) / 2 + (c - r); }
+
+// 6x6 SPD solve by Cholesky.  A pivot below 1e-15 of the largest diagonal
+// marks a direction the correspondences do not constrain; its unknown is 0.
+RTR_HD int cholesky6(const double* AtA, const double* Atb, double* x) {
+  double L[36], y[6];
+  int active[6];
+  double dmax = 0;
+  for (int i = 0; i < 36; ++i) L[i] = 0.0;
+  for (int i = 0; i < 6; ++i) dmax = fmax(dmax, AtA[up6(i, i)]);
+  int ok = 1;
+  for (int j = 0; j < 6; ++j) {
+    double s = AtA[up6(j, j)];
+    for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
+    if (!(s > 1e-15 * dmax)) {
+      active[j] = 0;
+      ok = 0;
+      L[j * 6 + j] = 1.0;
+      continue;
+    }
+    active[j] = 1;
+    const double ljj = sqrt(s);
+    L[j * 6 + j] = ljj;
+    for (int i = j + 1; i < 6; ++i) {
+      double t = AtA[up6(j, i)];
+      for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
+      L[i * 6 + j] = t / ljj;
+    }
+  }
+  for (int i = 0; i < 6; ++i) {
+    double s = Atb[i];
+    for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * y[k];
+    y[i] = active[i] ? s / L[i * 6 + i] : 0.0;
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < 6; ++k) s -= L[k * 6 + i] * x[k];
+    x[i] = active[i] ? s / L[i * 6 + i] : 0.0;
+  }
+  return ok;
+}
+
+// Low (2004) small-angle parameterisation: R = Rz(g) Ry(b) Rx(a), t in the
+// anchor frame; returns the world-frame 4x4 (column-major).
+RTR_HD void euler_anchor_T(const double* x, const double* anchor, double* T) {
+  const double ca = cos(x[0]), sa = sin(x[0]);
+  const double cb = cos(x[1]), sb = sin(x[1]);
+  const double cg = cos(x[2]), sg = sin(x[2]);
+  const double R[9] = {cg * cb, -sg * ca + cg * sb * sa, sg * sa + cg * sb * ca,
+                       sg * cb, cg * ca + sg * sb * sa,  -cg * sa + sg * sb * ca,
+                       -sb,     cb * sa,                 cb * ca};
+  for (int i = 0; i < 16; ++i) T[i] = 0.0;
+  T[15] = 1.0;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) T[c * 4 + r] = R[r * 3 + c];
+    const double Ra = R[r * 3] * anchor[0] + R[r * 3 + 1] * anchor[1] + R[r * 3 + 2] * anchor[2];
+    T[12 + r] = x[3 + r] + anchor[r] - Ra;
+  }
+}
+
+RTR_HD void mat4_mul(const double* A, const double* B, double* C) {
+  double Rm[16];
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < 4; ++k) s += A[k * 4 + r] * B[c * 4 + k];
+      Rm[c * 4 + r] = s;
+    }
+  for (int i = 0; i < 16; ++i) C[i] = Rm[i];
+}
+
+// Incremental transform from reduced FP64 sums (layout in rtr_types.hpp).
+// Returns the number of correspondences; Tinc is column-major.
+RTR_HD double solve_increment(int mode, const double* v, const float* anchor_f, double* Tinc,
+                              double* mse) {
+  const double n = v[SUM_N];
+  *mse = n > 0 ? v[SUM_D2] / n : 0.0;
+  const double anchor[3] = {(double)anchor_f[0], (double)anchor_f[1], (double)anchor_f[2]};
+  for (int i = 0; i < 16; ++i) Tinc[i] = 0.0;
+  Tinc[0] = Tinc[5] = Tinc[10] = Tinc[15] = 1.0;
+  if (n <= 0) return n;
+  if (mode == RTR_MODE_POINT_TO_POINT) {
+    double R[9], t[3];
+    rigid_horn(n, v + SUM_A, v + SUM_B, v + SUM_AB, R, t);
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) Tinc[c * 4 + r] = R[r * 3 + c];
+      const double Ra = R[r * 3] * anchor[0] + R[r * 3 + 1] * anchor[1] + R[r * 3 + 2] * anchor[2];
+      Tinc[12 + r] = t[r] + anchor[r] - Ra;
+    }
+  } else {
+    double x[6];
+    cholesky6(v + SUM_ATA, v + SUM_ATB, x);
+    euler_anchor_T(x, anchor, Tinc);
+  }
+  return n;
+}
+
+// PCL 1.8 DefaultConvergenceCriteria (max_iterations_similar_transforms_ = 0,
+// mse_threshold_absolute_ = 1e-12), thresholds as IterativeClosestPoint sets
+// them: rotation 1 - transformation_epsilon (or the rotation epsilon when
+// set), translation transformation_epsilon, relative MSE the euclidean fitness
+// epsilon.  prev_mse starts at DBL_MAX.
+RTR_HD int convergence(const IcpParams& p, int iterations, const double* Tinc, double mse,
+                       double* prev_mse) {
+  if (iterations >= p.max_iter) return RTR_CONV_ITERATIONS;
+  const double rot_thr = p.eps_rot > 0 ? p.eps_rot : 1.0 - p.eps_T;
+  const double cos_angle = 0.5 * (Tinc[0] + Tinc[5] + Tinc[10] - 1.0);
+  const double tsq = Tinc[12] * Tinc[12] + Tinc[13] * Tinc[13] + Tinc[14] * Tinc[14];
+  if (cos_angle >= rot_thr && tsq <= p.eps_T) return RTR_CONV_TRANSFORM;
+  const double dm = fabs(mse - *prev_mse);
+  if (dm < 1e-12) return RTR_CONV_ABS_MSE;
+  if (dm / *prev_mse < p.eps_fit) return RTR_CONV_REL_MSE;
+  *prev_mse = mse;
+  return RTR_NOT_CONVERGED;
+}
+
+// Per-correspondence contributions, relative to the anchor (float).
+RTR_HD int point_terms(int mode, float ax, float ay, float az, float bx, float by, float bz, float nx,
+                       float ny, float nz, float d2, float* out) {
+  out[SUM_N] = 1.0f;
+  out[SUM_D2] = d2;
+  if (mode == RTR_MODE_POINT_TO_POINT) {
+    out[SUM_A + 0] = ax;
+    out[SUM_A + 1] = ay;
+    out[SUM_A + 2] = az;
+    out[SUM_B + 0] = bx;
+    out[SUM_B + 1] = by;
+    out[SUM_B + 2] = bz;
+    const float a[3] = {ax, ay, az}, b[3] = {bx, by, bz};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[SUM_AB + r * 3 + c] = a[r] * b[c];
+    return SUM_P2P_COUNT;
+  }
+  const float row[6] = {nz * ay - ny * az, nx * az - nz * ax, ny * ax - nx * ay, nx, ny, nz};
+  float rhs = nx * (bx - ax);
+  rhs = rhs + ny * (by - ay);
+  rhs = rhs + nz * (bz - az);
+  int k = 0;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = r; c < 6; ++c) out[SUM_ATA + k++] = row[r] * row[c];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) out[SUM_ATB + r] = row[r] * rhs;
+  return SUM_P2L_COUNT;
+}
+
+}  // namespace rtr

@@ -1,0 +1,394 @@
+// Grid construction on the device: per-target uniform grids (cell-sorted
+// float4 points + cell start offsets) and per-source spatial (Morton) sort,
+// batched over all clouds of an engine.  Counting sort: count -> flat
+// exclusive scan over every cloud's cell range -> scatter.
+#include <hip/hip_runtime.h>
+
+#include "engine_dev.hpp"
+#include "geom.hpp"
+
+namespace rtr {
+
+// ---------------------------------------------------------------------------
+// fill and scan
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  size_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, v, n);
+}
+
+constexpr int kScanPer = 8;                    // elements per thread
+constexpr int kScanTile = kBlock * kScanPer;   // 2048 per block
+
+size_t scan_tmp_words(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// exclusive block scan of one value per thread; returns prefix, total in *tot
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tot) {
+  __shared__ uint32_t wsum[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t base = 0, all = 0;
+  for (int i = 0; i < kWaves; ++i) {
+    if (i < w) base += wsum[i];
+    all += wsum[i];
+  }
+  __syncthreads();
+  *tot = all;
+  return base + inc - v;
+}
+
+__global__ void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* tile_sums) {
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k)
+    if (base + k < n) s += in[base + k];
+  uint32_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_tiles(uint32_t* tile_sums, uint32_t ntiles) {
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < ntiles; b += kBlock) {
+    uint32_t i = b + threadIdx.x;
+    uint32_t v = i < ntiles ? tile_sums[i] : 0;
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(v, &tot);
+    if (i < ntiles) tile_sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) tile_sums[ntiles] = carry;
+}
+
+__global__ void k_scan_apply(const uint32_t* in, uint32_t* out, uint32_t n,
+                             const uint32_t* tile_sums) {
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  uint32_t v[kScanPer];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t ex = block_excl_scan(s, &tot) + tile_sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+}
+
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
+                           hipStream_t s) {
+  if (n == 0) return;
+  uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kBlock), 0, s, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, tmp, ntiles);
+  hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, s, in, out, n, tmp);
+}
+
+// ---------------------------------------------------------------------------
+// block reductions
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// bbox of finite points of one cloud: lo[3], hi[3], count -> shared result
+__device__ void block_bbox(const float* xyz, uint32_t n, float* lo, float* hi, uint32_t* cnt) {
+  __shared__ float s_lo[kWaves][3], s_hi[kWaves][3];
+  __shared__ uint32_t s_cnt[kWaves];
+  float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, h[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    float x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
+    if (!(isfinite(x) && isfinite(y) && isfinite(z))) continue;
+    l[0] = fminf(l[0], x);
+    l[1] = fminf(l[1], y);
+    l[2] = fminf(l[2], z);
+    h[0] = fmaxf(h[0], x);
+    h[1] = fmaxf(h[1], y);
+    h[2] = fmaxf(h[2], z);
+    ++c;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int a = 0; a < 3; ++a) {
+    l[a] = wave_min(l[a]);
+    h[a] = wave_max(h[a]);
+  }
+  c = wave_sum_u32(c);
+  if (lane == 0) {
+    for (int a = 0; a < 3; ++a) {
+      s_lo[w][a] = l[a];
+      s_hi[w][a] = h[a];
+    }
+    s_cnt[w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = s_lo[0][a];
+      hi[a] = s_hi[0][a];
+    }
+    *cnt = s_cnt[0];
+    for (int i = 1; i < kWaves; ++i) {
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(lo[a], s_lo[i][a]);
+        hi[a] = fmaxf(hi[a], s_hi[i][a]);
+      }
+      *cnt += s_cnt[i];
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// target grids
+// ---------------------------------------------------------------------------
+
+// Choose the cell edge: the smallest h >= h_req (growing by 5%) whose grid
+// over the bbox fits the cell budget.  dims use the same float expression as
+// the cell assignment, so every point lands inside the grid.
+__device__ void choose_grid(TargetDesc& t, const float* lo, const float* hi, uint32_t cnt) {
+  t.n_valid = cnt;
+  if (cnt == 0) {
+    t.dx = t.dy = t.dz = 1;
+    t.ox = t.oy = t.oz = 0.f;
+    t.h = t.h_req > 0.f ? t.h_req : 1.f;
+    t.inv_h = 1.0f / t.h;
+    t.margin = 1e-3f * t.h;
+    t.anchor[0] = t.anchor[1] = t.anchor[2] = 0.f;
+    return;
+  }
+  const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  float h = t.h_req > 0.f ? t.h_req : 1e-3f;
+  const double vol = (double)ex * ey * ez;
+  const double hb = cbrt(vol / (double)t.cell_budget);
+  if ((float)hb > h) h = (float)hb;
+  int dx = 1, dy = 1, dz = 1;
+  float inv = 1.0f / h;
+  bool fits = false;
+  if (isfinite(ex) && isfinite(ey) && isfinite(ez)) {
+    for (int it = 0; it < 400; ++it) {
+      inv = 1.0f / h;
+      const float fx = floorf((hi[0] - lo[0]) * inv), fy = floorf((hi[1] - lo[1]) * inv),
+                  fz = floorf((hi[2] - lo[2]) * inv);
+      if (fx < 1.0e7f && fy < 1.0e7f && fz < 1.0e7f) {
+        dx = (int)fx + 1;
+        dy = (int)fy + 1;
+        dz = (int)fz + 1;
+        if ((long long)dx * dy * dz <= (long long)t.cell_budget) {
+          fits = true;
+          break;
+        }
+      }
+      h *= 1.05f;
+    }
+  }
+  if (!fits) {
+    // a single cell holds everything (cell_of clamps every index to 0): slow, exact
+    dx = dy = dz = 1;
+    h = fmaxf(fmaxf(ex, ey), ez);
+    if (!(h > 0.f) || !isfinite(h)) h = 1.0f;
+    inv = 1.0f / h;
+  }
+  t.dx = dx;
+  t.dy = dy;
+  t.dz = dz;
+  t.ox = lo[0];
+  t.oy = lo[1];
+  t.oz = lo[2];
+  t.h = h;
+  t.inv_h = inv;
+  float mag = fmaxf(fmaxf(fabsf(lo[0]), fabsf(hi[0])), fmaxf(fmaxf(fabsf(lo[1]), fabsf(hi[1])),
+                                                             fmaxf(fabsf(lo[2]), fabsf(hi[2]))));
+  t.margin = 1e-3f * h + 4e-6f * (mag + fmaxf(ex, fmaxf(ey, ez)));
+  t.anchor[0] = lo[0] + 0.5f * ex;
+  t.anchor[1] = lo[1] + 0.5f * ey;
+  t.anchor[2] = lo[2] + 0.5f * ez;
+}
+
+__global__ void k_tgt_bbox(DevEngine e) {
+  TargetDesc& t = e.tgts[blockIdx.x];
+  __shared__ float lo[3], hi[3];
+  __shared__ uint32_t cnt;
+  block_bbox(e.tgt_raw + 3 * (size_t)t.raw_off, t.n, lo, hi, &cnt);
+  if (threadIdx.x == 0) choose_grid(t, lo, hi, cnt);
+}
+
+__device__ __forceinline__ bool cell_of(const TargetDesc& t, float x, float y, float z,
+                                        uint32_t* c) {
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) return false;
+  int ix = clampi((int)floorf((x - t.ox) * t.inv_h), 0, t.dx - 1);
+  int iy = clampi((int)floorf((y - t.oy) * t.inv_h), 0, t.dy - 1);
+  int iz = clampi((int)floorf((z - t.oz) * t.inv_h), 0, t.dz - 1);
+  *c = ((uint32_t)iz * t.dy + iy) * t.dx + ix;
+  return true;
+}
+
+__global__ void k_tgt_count(DevEngine e) {
+  const TargetDesc& t = e.tgts[blockIdx.y];
+  const float* xyz = e.tgt_raw + 3 * (size_t)t.raw_off;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += gridDim.x * blockDim.x) {
+    uint32_t c;
+    if (cell_of(t, xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], &c)) {
+      e.cell_id[t.raw_off + i] = c;
+      atomicAdd(&e.cell_count[t.cell_off + c], 1u);
+    } else {
+      e.cell_id[t.raw_off + i] = 0xffffffffu;
+    }
+  }
+}
+
+__global__ void k_tgt_scatter(DevEngine e) {
+  const TargetDesc& t = e.tgts[blockIdx.y];
+  const float* xyz = e.tgt_raw + 3 * (size_t)t.raw_off;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += gridDim.x * blockDim.x) {
+    const uint32_t c = e.cell_id[t.raw_off + i];
+    if (c == 0xffffffffu) continue;
+    const uint32_t slot = t.cell_off + c;
+    const uint32_t pos = e.cell_start[slot] + atomicSub(&e.cell_count[slot], 1u) - 1u;
+    union {
+      int32_t i;
+      float f;
+    } w;
+    w.i = (int32_t)i;
+    e.tgt_sorted[pos] =
+        make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], w.f);
+    if (e.tgt_nrm_raw && e.tgt_nrm_sorted) {
+      const float* nr = e.tgt_nrm_raw + 3 * ((size_t)t.raw_off + i);
+      e.tgt_nrm_sorted[pos] = make_float4(nr[0], nr[1], nr[2], 0.f);
+    }
+  }
+}
+
+static unsigned blocks_for(int max_pts) {
+  unsigned b = (unsigned)((max_pts + kBlock - 1) / kBlock);
+  if (b < 1) b = 1;
+  if (b > 256) b = 256;
+  return b;
+}
+
+void launch_target_grid(const DevEngine& e, int max_tgt_pts, hipStream_t s) {
+  if (e.n_tgt == 0) return;
+  launch_fill_u32(e.cell_count, 0u, e.n_cells_total, s);
+  hipLaunchKernelGGL(k_tgt_bbox, dim3(e.n_tgt), dim3(kBlock), 0, s, e);
+  dim3 g(blocks_for(max_tgt_pts), e.n_tgt);
+  hipLaunchKernelGGL(k_tgt_count, g, dim3(kBlock), 0, s, e);
+  launch_exclusive_scan(e.cell_count, e.cell_start, e.n_cells_total, e.scan_tmp, s);
+  hipLaunchKernelGGL(k_tgt_scatter, g, dim3(kBlock), 0, s, e);
+}
+
+// ---------------------------------------------------------------------------
+// source spatial sort (Morton order of a coarse grid in the source frame)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  // 5-bit value -> bits at positions 0, 3, 6, 9, 12
+  uint32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 5; ++b) r |= ((v >> b) & 1u) << (3 * b);
+  return r;
+}
+
+__global__ void k_src_bbox(DevEngine e) {
+  SourceDesc& s = e.srcs[blockIdx.x];
+  __shared__ float lo[3], hi[3];
+  __shared__ uint32_t cnt;
+  block_bbox(e.src_raw + 3 * (size_t)s.off, s.n, lo, hi, &cnt);
+  if (threadIdx.x == 0) {
+    // bin_budget = P^3 with P a power of two <= 32 (chosen by the host)
+    int P = 1;
+    while (P * P * P < (int)s.bin_budget) P <<= 1;
+    s.dx = s.dy = s.dz = P;
+    float ext = cnt ? fmaxf(hi[0] - lo[0], fmaxf(hi[1] - lo[1], hi[2] - lo[2])) : 0.f;
+    s.ox = cnt ? lo[0] : 0.f;
+    s.oy = cnt ? lo[1] : 0.f;
+    s.oz = cnt ? lo[2] : 0.f;
+    s.inv_h = ext > 0.f ? (float)P / ext : 0.f;
+  }
+}
+
+__device__ __forceinline__ uint32_t bin_of(const SourceDesc& s, float x, float y, float z) {
+  const uint32_t last = s.bin_budget - 1;
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) return last;
+  const int P = s.dx;
+  int ix = clampi((int)floorf((x - s.ox) * s.inv_h), 0, P - 1);
+  int iy = clampi((int)floorf((y - s.oy) * s.inv_h), 0, P - 1);
+  int iz = clampi((int)floorf((z - s.oz) * s.inv_h), 0, P - 1);
+  uint32_t m = spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
+  return m > last ? last : m;
+}
+
+__global__ void k_src_count(DevEngine e) {
+  const SourceDesc& s = e.srcs[blockIdx.y];
+  const float* xyz = e.src_raw + 3 * (size_t)s.off;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+    uint32_t b = bin_of(s, xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]);
+    e.bin_id[s.off + i] = b;
+    atomicAdd(&e.bin_count[s.bin_off + b], 1u);
+  }
+}
+
+__global__ void k_src_scatter(DevEngine e) {
+  const SourceDesc& s = e.srcs[blockIdx.y];
+  const float* xyz = e.src_raw + 3 * (size_t)s.off;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+    const uint32_t slot = s.bin_off + e.bin_id[s.off + i];
+    // bin_start is global over all sources; every source point is placed, so
+    // source s occupies exactly [s.off, s.off + s.n) of the sorted arrays
+    const uint32_t pos = e.bin_start[slot] + atomicSub(&e.bin_count[slot], 1u) - 1u;
+    e.sx[pos] = xyz[3 * (size_t)i];
+    e.sy[pos] = xyz[3 * (size_t)i + 1];
+    e.sz[pos] = xyz[3 * (size_t)i + 2];
+    e.perm[pos] = i;
+  }
+}
+
+void launch_source_sort(const DevEngine& e, int max_src_pts, hipStream_t s) {
+  if (e.n_src == 0) return;
+  launch_fill_u32(e.bin_count, 0u, e.n_bins_total, s);
+  hipLaunchKernelGGL(k_src_bbox, dim3(e.n_src), dim3(kBlock), 0, s, e);
+  dim3 g(blocks_for(max_src_pts), e.n_src);
+  hipLaunchKernelGGL(k_src_count, g, dim3(kBlock), 0, s, e);
+  launch_exclusive_scan(e.bin_count, e.bin_start, e.n_bins_total, e.scan_tmp, s);
+  hipLaunchKernelGGL(k_src_scatter, g, dim3(kBlock), 0, s, e);
+}
+
+}  // namespace rtr

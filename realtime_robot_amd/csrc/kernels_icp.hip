@@ -1,0 +1,519 @@
+// Persistent batched ICP over a work queue, plus the per-point output
+// kernels (correspondences, transformed cloud, fitness) and k-NN normals.
+//
+// Work queue.  The source points of every problem are split into items of
+// item_pts consecutive (spatially sorted) points.  Workgroups loop over passes
+// (one pass = one ICP iteration of every unfinished problem); in each pass they
+// claim items from that pass's atomic counter.  An item of problem p in pass k
+// waits (bounded) until p has completed k iterations, computes its points'
+// correspondences and FP32 compensated moments, and the workgroup that
+// finishes p's last item of the pass reduces the item moments in FP64, solves
+// the increment, applies the convergence rules and publishes iteration k+1.
+// Problems therefore advance independently; no grid-wide barrier is needed,
+// and every wait is bounded (a timeout sets e.error and the waiters bail out).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "engine_dev.hpp"
+#include "geom.hpp"
+
+namespace rtr {
+
+constexpr uint32_t kFinishedBit = 1u << 30;
+constexpr int kMaxSpins = 1 << 22;
+
+__device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+  const float y = x - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// (s - c) + (s2 - c2): TwoSum of the leading parts, rounding error folded
+// into the compensation.
+__device__ __forceinline__ void comp_combine(float& s, float& c, float s2, float c2) {
+  const float S = s + s2;
+  const float bb = S - s;
+  const float err = (s - (S - bb)) + (s2 - bb);
+  c = (c + c2) - err;
+  s = S;
+}
+
+__device__ __forceinline__ GridView grid_of(const DevEngine& e, const TargetDesc& td) {
+  GridView g;
+  g.pts = e.tgt_sorted;
+  g.cell_start = e.cell_start + td.cell_off;
+  g.dx = td.dx;
+  g.dy = td.dy;
+  g.dz = td.dz;
+  g.n_pts = td.n_valid;
+  g.ox = td.ox;
+  g.oy = td.oy;
+  g.oz = td.oz;
+  g.h = td.h;
+  g.inv_h = td.inv_h;
+  g.margin = td.margin;
+  return g;
+}
+
+__global__ void k_icp_init(DevEngine e) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= e.n_prob) return;
+  ProblemState& S = e.st[p];
+  for (int k = 0; k < 16; ++k) S.T[k] = e.guesses ? e.guesses[16 * (size_t)p + k] : ((k % 5 == 0) ? 1.0 : 0.0);
+  S.prev_mse = DBL_MAX;
+  S.mse = 0.0;
+  S.n_corr = 0.0;
+  S.iter = 0;
+  S.state = RTR_NOT_CONVERGED;
+  S.finished = 0;
+  e.done[p] = 0;
+  uint32_t* word = (uint32_t*)&S.gen;
+  if (e.probs[p].n_items == 0) {
+    S.state = RTR_FAIL_NO_CORRESPONDENCES;
+    S.finished = 1;
+    *word = kFinishedBit;
+    atomicAdd(e.problems_done, 1u);
+  } else {
+    *word = 0u;
+  }
+}
+
+template <int NS>
+struct IcpShared {
+  float m[12];
+  float anchor[3];
+  int item;
+  int run;
+  int stop;
+  int last;
+  float red[kWaves][NS][2];
+  double dred[kBlock / NSUM_MAX][NSUM_MAX];
+};
+
+template <int MODE, int NS>
+__device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
+                              const TargetDesc& td, int pass, IcpShared<NS>& sh) {
+  const int tid = threadIdx.x;
+  const int v = tid % NSUM_MAX, grp = tid / NSUM_MAX;
+  double acc = 0.0;
+  if (v < NS) {
+    for (uint32_t i = grp; i < pr.n_items; i += kBlock / NSUM_MAX) {
+      const float* P = e.partials + (size_t)(pr.item_off + i) * (NSUM_MAX * 2);
+      acc += (double)P[2 * v] - (double)P[2 * v + 1];
+    }
+  }
+  sh.dred[grp][v] = acc;
+  __syncthreads();
+  if (tid < NSUM_MAX) {
+    double t = 0.0;
+    for (int g2 = 0; g2 < kBlock / NSUM_MAX; ++g2) t += sh.dred[g2][tid];
+    sh.dred[0][tid] = t;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double vals[NSUM_MAX];
+    for (int k = 0; k < NSUM_MAX; ++k) vals[k] = sh.dred[0][k];
+    ProblemState& S = e.st[p];
+    const double n = vals[SUM_N];
+    bool fin = false;
+    S.n_corr = n;
+    if (n < (double)e.prm.min_corr) {
+      S.state = RTR_FAIL_NO_CORRESPONDENCES;
+      S.mse = n > 0 ? vals[SUM_D2] / n : 0.0;
+      fin = true;
+    } else {
+      double Tinc[16], mse, Tn[16];
+      solve_increment(MODE, vals, td.anchor, Tinc, &mse);
+      mat4_mul(Tinc, S.T, Tn);
+      for (int k = 0; k < 16; ++k) S.T[k] = Tn[k];
+      S.iter += 1;
+      S.mse = mse;
+      double prev = S.prev_mse;
+      const int stt = convergence(e.prm, S.iter, Tinc, mse, &prev);
+      S.prev_mse = prev;
+      S.state = stt;
+      fin = (stt != RTR_NOT_CONVERGED);
+    }
+    if (fin) S.finished = 1;
+    __threadfence();
+    store_release((uint32_t*)&S.gen, (uint32_t)(pass + 1) | (fin ? kFinishedBit : 0u));
+    if (fin) atomicAdd(e.problems_done, 1u);
+  }
+}
+
+template <int MODE, int NS>
+__device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS>& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint2 itm = e.items[it];
+  const uint32_t p = itm.x;
+  const Problem pr = e.probs[p];
+  const SourceDesc sd = e.srcs[pr.src];
+  const TargetDesc td = e.tgts[pr.tgt];
+  const GridView g = grid_of(e, td);
+  float m[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) m[k] = sh.m[k];
+  const float ax = sh.anchor[0], ay = sh.anchor[1], az = sh.anchor[2];
+  float acc_s[NS], acc_c[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) acc_s[k] = acc_c[k] = 0.f;
+  const uint32_t end = min(itm.y + (uint32_t)e.item_pts, sd.n);
+  for (uint32_t j = itm.y + tid; j < end; j += kBlock) {
+    const uint32_t sj = sd.off + j;
+    float qx, qy, qz;
+    apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+    const NNBest b = nn_search(g, qx, qy, qz, e.prm.r2f);
+    if (b.key != ~0ull) {
+      float nx = 0.f, ny = 0.f, nz = 0.f;
+      if (MODE == RTR_MODE_POINT_TO_PLANE) {
+        const float4 nn = e.tgt_nrm_sorted[b.slot];
+        nx = nn.x;
+        ny = nn.y;
+        nz = nn.z;
+      }
+      float terms[NSUM_MAX];
+      point_terms(MODE, qx - ax, qy - ay, qz - az, b.tx - ax, b.ty - ay, b.tz - az, nx, ny, nz,
+                  key_d2(b.key), terms);
+#pragma unroll
+      for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+    }
+  }
+  // wave reduction with compensation, then across the four waves
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    float s = acc_s[k], c = acc_c[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float s2 = __shfl_xor(s, off, 64);
+      const float c2 = __shfl_xor(c, off, 64);
+      comp_combine(s, c, s2, c2);
+    }
+    if (lane == 0) {
+      sh.red[w][k][0] = s;
+      sh.red[w][k][1] = c;
+    }
+  }
+  __syncthreads();
+  if (tid < NS) {
+    float s = sh.red[0][tid][0], c = sh.red[0][tid][1];
+    for (int ww = 1; ww < kWaves; ++ww) comp_combine(s, c, sh.red[ww][tid][0], sh.red[ww][tid][1]);
+    float* P = e.partials + (size_t)it * (NSUM_MAX * 2);
+    P[2 * tid] = s;
+    P[2 * tid + 1] = c;
+    __threadfence();
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t prev = atomicAdd(&e.done[p], 1u);
+    sh.last = (prev + 1u == (uint32_t)(pass + 1) * pr.n_items);
+  }
+  __syncthreads();
+  if (sh.last) {
+    __threadfence();
+    solve_problem<MODE, NS>(e, p, pr, td, pass, sh);
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_icp(DevEngine e) {
+  constexpr int NS = MODE == RTR_MODE_POINT_TO_POINT ? SUM_P2P_COUNT : SUM_P2L_COUNT;
+  __shared__ IcpShared<NS> sh;
+  const int tid = threadIdx.x;
+  for (int pass = 0; pass < e.prm.max_iter; ++pass) {
+    if (tid == 0)
+      sh.stop = (load_acquire(e.problems_done) >= (uint32_t)e.n_prob) ||
+                (load_acquire((const uint32_t*)e.error) != 0u);
+    __syncthreads();
+    if (sh.stop) break;
+    for (;;) {
+      if (tid == 0) {
+        const int it = (int)atomicAdd(&e.claim[pass], 1u);
+        int run = 0;
+        if (it < e.n_items) {
+          const uint32_t p = e.items[it].x;
+          const uint32_t* word = (const uint32_t*)&e.st[p].gen;
+          uint32_t wv = load_acquire(word);
+          int spins = 0;
+          while (!(wv & kFinishedBit) && (int)wv < pass) {
+            if (++spins > kMaxSpins || load_acquire((const uint32_t*)e.error) != 0u) {
+              atomicOr(e.error, 1);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            wv = load_acquire(word);
+          }
+          if (!(wv & kFinishedBit) && (int)wv == pass) {
+            run = 1;
+            tf_rows_f32(e.st[p].T, sh.m);
+            const TargetDesc& td = e.tgts[e.probs[p].tgt];
+            sh.anchor[0] = td.anchor[0];
+            sh.anchor[1] = td.anchor[1];
+            sh.anchor[2] = td.anchor[2];
+          }
+        }
+        sh.item = it;
+        sh.run = run;
+      }
+      __syncthreads();
+      const int it = sh.item;
+      if (it >= e.n_items) break;
+      if (sh.run) process_item<MODE, NS>(e, it, pass, sh);
+      __syncthreads();
+    }
+  }
+}
+
+int icp_blocks_per_cu(int mode) {
+  int nb = 0;
+  hipError_t err;
+  if (mode == RTR_MODE_POINT_TO_PLANE)
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_PLANE>, kBlock, 0);
+  else
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_POINT>, kBlock, 0);
+  if (err != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_icp_init, dim3((e.n_prob + kBlock - 1) / kBlock), dim3(kBlock), 0, s, e);
+  if (e.prm.mode == RTR_MODE_POINT_TO_PLANE)
+    hipLaunchKernelGGL(k_icp<RTR_MODE_POINT_TO_PLANE>, dim3(grid_blocks), dim3(kBlock), 0, s, e);
+  else
+    hipLaunchKernelGGL(k_icp<RTR_MODE_POINT_TO_POINT>, dim3(grid_blocks), dim3(kBlock), 0, s, e);
+}
+
+// ---------------------------------------------------------------------------
+// per-point outputs
+// ---------------------------------------------------------------------------
+
+__global__ void k_corr(DevEngine e, int prob, const double* T, float r2f, int32_t* idx, float* d2) {
+  const Problem pr = e.probs[prob];
+  const SourceDesc sd = e.srcs[pr.src];
+  const TargetDesc td = e.tgts[pr.tgt];
+  const GridView g = grid_of(e, td);
+  float m[12];
+  tf_rows_f32(T ? T : e.st[prob].T, m);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < sd.n; j += gridDim.x * blockDim.x) {
+    const uint32_t sj = sd.off + j;
+    float qx, qy, qz;
+    apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+    const NNBest b = nn_search(g, qx, qy, qz, r2f);
+    const uint32_t o = e.perm[sj];
+    if (b.key != ~0ull) {
+      idx[o] = (int32_t)(uint32_t)(b.key & 0xffffffffull);
+      d2[o] = key_d2(b.key);
+    } else {
+      idx[o] = -1;
+      d2[o] = __builtin_inff();
+    }
+  }
+}
+
+void launch_correspondences(const DevEngine& e, int prob, const double* T, float r2f, int32_t* idx,
+                            float* d2, hipStream_t s) {
+  hipLaunchKernelGGL(k_corr, dim3(256), dim3(kBlock), 0, s, e, prob, T, r2f, idx, d2);
+}
+
+__global__ void k_transform_out(DevEngine e, int prob, float* out) {
+  const Problem pr = e.probs[prob];
+  const SourceDesc sd = e.srcs[pr.src];
+  float m[12];
+  tf_rows_f32(e.st[prob].T, m);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < sd.n; j += gridDim.x * blockDim.x) {
+    const uint32_t sj = sd.off + j;
+    float qx, qy, qz;
+    apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+    const size_t o = e.perm[sj];
+    out[3 * o] = qx;
+    out[3 * o + 1] = qy;
+    out[3 * o + 2] = qz;
+  }
+}
+
+void launch_transform_out(const DevEngine& e, int prob, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_transform_out, dim3(256), dim3(kBlock), 0, s, e, prob, out);
+}
+
+__global__ void k_fitness_d2(DevEngine e, float* d2_all) {
+  const int prob = blockIdx.y;
+  const Problem pr = e.probs[prob];
+  const SourceDesc sd = e.srcs[pr.src];
+  const TargetDesc td = e.tgts[pr.tgt];
+  const GridView g = grid_of(e, td);
+  float m[12];
+  tf_rows_f32(e.st[prob].T, m);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < sd.n; j += gridDim.x * blockDim.x) {
+    const uint32_t sj = sd.off + j;
+    float qx, qy, qz;
+    apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+    const NNBest b = nn_search(g, qx, qy, qz, FLT_MAX);
+    d2_all[pr.pt_off + j] = b.key != ~0ull ? key_d2(b.key) : __builtin_inff();
+  }
+}
+
+void launch_fitness_d2(const DevEngine& e, float* d2_all, hipStream_t s) {
+  if (e.n_prob == 0) return;
+  hipLaunchKernelGGL(k_fitness_d2, dim3(64, e.n_prob), dim3(kBlock), 0, s, e, d2_all);
+}
+
+// ---------------------------------------------------------------------------
+// k-NN PCA normals of the targets
+// ---------------------------------------------------------------------------
+
+constexpr int kNrmBlock = 128;
+constexpr int kMaxK = 32;
+
+struct KnnVisitor {
+  const float4* pts;
+  float qx, qy, qz;
+  int k, cnt;
+  uint64_t* key;   // LDS, stride kNrmBlock
+  uint32_t* slot;  // LDS, stride kNrmBlock
+  __device__ void scan(uint32_t b, uint32_t e) {
+    for (uint32_t j = b; j < e; ++j) {
+      const float4 t = pts[j];
+      union {
+        float f;
+        int32_t i;
+      } w;
+      w.f = t.w;
+      const uint64_t kk = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), w.i);
+      if (cnt == k && kk >= key[(k - 1) * kNrmBlock]) continue;
+      int pos = cnt < k ? cnt++ : k - 1;
+      while (pos > 0 && kk < key[(pos - 1) * kNrmBlock]) {
+        key[pos * kNrmBlock] = key[(pos - 1) * kNrmBlock];
+        slot[pos * kNrmBlock] = slot[(pos - 1) * kNrmBlock];
+        --pos;
+      }
+      key[pos * kNrmBlock] = kk;
+      slot[pos * kNrmBlock] = j;
+    }
+  }
+  __device__ float bound() const { return cnt < k ? 3.4e38f : key_d2(key[(k - 1) * kNrmBlock]); }
+};
+
+__device__ void jacobi3(double* a, double* V) {
+  for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    const double off = fabs(a[1]) + fabs(a[2]) + fabs(a[5]);
+    const double dg = fabs(a[0]) + fabs(a[4]) + fabs(a[8]);
+    if (off == 0.0 || off <= 1e-18 * dg) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        const double apq = a[p * 3 + q];
+        if (apq == 0.0) continue;
+        const double theta = (a[q * 4] - a[p * 4]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          const double akp = a[k * 3 + p], akq = a[k * 3 + q];
+          a[k * 3 + p] = c * akp - s * akq;
+          a[k * 3 + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double apk = a[p * 3 + k], aqk = a[q * 3 + k];
+          a[p * 3 + k] = c * apk - s * aqk;
+          a[q * 3 + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double vkp = V[k * 3 + p], vkq = V[k * 3 + q];
+          V[k * 3 + p] = c * vkp - s * vkq;
+          V[k * 3 + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+// One thread per cell-sorted target point: exact k nearest (ordered by
+// (d2, index), the point itself included), FP64 two-pass covariance, the
+// eigenvector of the smallest eigenvalue, flipped towards the viewpoint
+// (pcl::NormalEstimation + flipNormalTowardsViewpoint semantics).
+__global__ __launch_bounds__(kNrmBlock) void k_target_normals(DevEngine e, int k, float vpx,
+                                                              float vpy, float vpz,
+                                                              float* nrm_raw) {
+  __shared__ uint64_t s_key[kMaxK * kNrmBlock];
+  __shared__ uint32_t s_slot[kMaxK * kNrmBlock];
+  const TargetDesc td = e.tgts[blockIdx.y];
+  const GridView g = grid_of(e, td);
+  const uint32_t base = e.cell_start[td.cell_off];
+  for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < td.n_valid; j0 += gridDim.x * blockDim.x) {
+    const uint32_t j = j0 + threadIdx.x;
+    if (j >= td.n_valid) continue;
+    const float4 q = e.tgt_sorted[base + j];
+    KnnVisitor v;
+    v.pts = e.tgt_sorted;
+    v.qx = q.x;
+    v.qy = q.y;
+    v.qz = q.z;
+    v.k = k;
+    v.cnt = 0;
+    v.key = s_key + threadIdx.x;
+    v.slot = s_slot + threadIdx.x;
+    ring_walk(g, q.x, q.y, q.z, FLT_MAX, v);
+    double nv[3] = {__builtin_nan(""), __builtin_nan(""), __builtin_nan("")};
+    if (v.cnt >= 3) {
+      double mean[3] = {0, 0, 0};
+      for (int i = 0; i < v.cnt; ++i) {
+        const float4 p = e.tgt_sorted[v.slot[i * kNrmBlock]];
+        mean[0] += p.x;
+        mean[1] += p.y;
+        mean[2] += p.z;
+      }
+      for (int a = 0; a < 3; ++a) mean[a] /= v.cnt;
+      double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int i = 0; i < v.cnt; ++i) {
+        const float4 p = e.tgt_sorted[v.slot[i * kNrmBlock]];
+        const double d[3] = {p.x - mean[0], p.y - mean[1], p.z - mean[2]};
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) C[r * 3 + c] += d[r] * d[c];
+      }
+      for (int i = 0; i < 9; ++i) C[i] /= v.cnt;
+      double V[9];
+      jacobi3(C, V);
+      int mi = 0;
+      if (C[4] < C[mi * 4]) mi = 1;
+      if (C[8] < C[mi * 4]) mi = 2;
+      nv[0] = V[mi];
+      nv[1] = V[3 + mi];
+      nv[2] = V[6 + mi];
+      const double vx = (double)vpx - q.x, vy = (double)vpy - q.y, vz = (double)vpz - q.z;
+      if (vx * nv[0] + vy * nv[1] + vz * nv[2] < 0) {
+        nv[0] = -nv[0];
+        nv[1] = -nv[1];
+        nv[2] = -nv[2];
+      }
+    }
+    union {
+      float f;
+      int32_t i;
+    } w;
+    w.f = q.w;
+    float* o = nrm_raw + 3 * ((size_t)td.raw_off + (uint32_t)w.i);
+    o[0] = (float)nv[0];
+    o[1] = (float)nv[1];
+    o[2] = (float)nv[2];
+    if (e.tgt_nrm_sorted) e.tgt_nrm_sorted[base + j] = make_float4((float)nv[0], (float)nv[1], (float)nv[2], 0.f);
+  }
+}
+
+void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, float vpz,
+                           int max_tgt_pts, hipStream_t s) {
+  if (e.n_tgt == 0) return;
+  if (k > kMaxK) k = kMaxK;
+  unsigned bx = (unsigned)((max_tgt_pts + kNrmBlock - 1) / kNrmBlock);
+  if (bx < 1) bx = 1;
+  if (bx > 1024) bx = 1024;
+  hipLaunchKernelGGL(k_target_normals, dim3(bx, e.n_tgt), dim3(kNrmBlock), 0, s, e, k, vpx, vpy,
+                     vpz, const_cast<float*>(e.tgt_nrm_raw));
+}
+
+}  // namespace rtr

@@ -1,0 +1,422 @@
+// Voxel-hash map: merges transformed clouds into a scene map.
+//
+// The reference "merges" by transforming a cloud, saving it as a PCD and
+// overlaying files in a viewer (RealTimeRobot.cpp:104-114).  Here every
+// point is transformed on the device (PCL transformPointCloud order), keyed
+// by floor(p * inv_voxel) per axis (the truncation convention of
+// key_point.h:143-160 / matching.h:181-184, floored for negative
+// coordinates), and accumulated into an open-addressing hash table in HBM:
+// per voxel a 64-bit key, the FP32 sum of (p - voxel corner) and a count.
+// Offsets from the corner keep the FP32 sums small, so the centroid keeps
+// full float precision however many frames land in the voxel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rtr.h"
+#include "engine.hpp"
+#include "geom.hpp"
+
+namespace rtr {
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kBias = 1 << 20;
+
+struct MapDev {
+  unsigned long long* keys;  // capacity slots, kEmpty when free
+  float4* acc;               // (sum dx, sum dy, sum dz, count bits)
+  uint32_t* counters;        // [0] occupied slots, [1] dropped points, [2] overflow
+  uint64_t mask;             // capacity - 1 (power of two)
+  uint32_t max_voxels;
+  float h, inv_h;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t k) {
+  // SplitMix64 finaliser (Steele, Lea, Flood 2014)
+  k ^= k >> 30;
+  k *= 0xbf58476d1ce4e5b9ull;
+  k ^= k >> 27;
+  k *= 0x94d049bb133111ebull;
+  k ^= k >> 31;
+  return k;
+}
+
+RTR_HD bool voxel_key(float x, float y, float z, float inv_h, uint64_t* key, int* k3) {
+  const float v[3] = {x * inv_h, y * inv_h, z * inv_h};
+  uint64_t kk = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (!(v[a] - v[a] == 0.f)) return false;  // inf or NaN
+    const float f = floorf(v[a]);
+    if (f < -(float)kBias || f >= (float)kBias) return false;
+    const int k = (int)f;
+    k3[a] = k;
+    kk |= (uint64_t)(uint32_t)(k + kBias) << (21 * a);
+  }
+  *key = kk;
+  return true;
+}
+
+__global__ void k_map_integrate(MapDev m, const float* xyz, uint32_t n, const float* T12,
+                                int has_T) {
+  float mt[12];
+  if (has_T)
+    for (int i = 0; i < 12; ++i) mt[i] = T12[i];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float px = xyz[3 * (size_t)i], py = xyz[3 * (size_t)i + 1], pz = xyz[3 * (size_t)i + 2];
+    if (has_T) apply_f32(mt, px, py, pz, px, py, pz);
+    uint64_t key;
+    int k3[3];
+    if (!voxel_key(px, py, pz, m.inv_h, &key, k3)) {
+      atomicAdd(&m.counters[1], 1u);
+      continue;
+    }
+    uint64_t slot = mix64(key) & m.mask;
+    bool placed = false;
+    for (uint64_t probe = 0; probe <= m.mask; ++probe) {
+      unsigned long long cur = m.keys[slot];
+      if (cur == kEmpty) {
+        cur = atomicCAS(&m.keys[slot], (unsigned long long)kEmpty, (unsigned long long)key);
+        if (cur == kEmpty) {
+          const uint32_t occ = atomicAdd(&m.counters[0], 1u);
+          if (occ >= m.max_voxels) atomicOr(&m.counters[2], 1u);
+          cur = key;
+        }
+      }
+      if (cur == key) {
+        const float dx = px - (float)k3[0] * m.h;
+        const float dy = py - (float)k3[1] * m.h;
+        const float dz = pz - (float)k3[2] * m.h;
+        float* a = reinterpret_cast<float*>(&m.acc[slot]);
+        atomicAdd(a + 0, dx);
+        atomicAdd(a + 1, dy);
+        atomicAdd(a + 2, dz);
+        atomicAdd(reinterpret_cast<uint32_t*>(a + 3), 1u);
+        placed = true;
+        break;
+      }
+      slot = (slot + 1) & m.mask;
+    }
+    if (!placed) {
+      atomicAdd(&m.counters[1], 1u);
+      atomicOr(&m.counters[2], 1u);
+    }
+  }
+}
+
+__global__ void k_map_compact(MapDev m, unsigned long long* okeys, float4* oacc, uint32_t* ocount,
+                              uint32_t cap) {
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s <= m.mask;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long k = m.keys[s];
+    if (k == kEmpty) continue;
+    const uint32_t o = atomicAdd(ocount, 1u);
+    if (o < cap) {
+      okeys[o] = k;
+      oacc[o] = m.acc[s];
+    }
+  }
+}
+
+__global__ void k_map_clear(MapDev m) {
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s <= m.mask;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    m.keys[s] = kEmpty;
+    m.acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 4) m.counters[threadIdx.x] = 0u;
+}
+
+struct VoxelMap {
+  int device;
+  hipStream_t stream = nullptr;
+  DevBuf keys, acc, counters, in, T, okeys, oacc, ocount;
+  MapDev dev;
+  size_t capacity = 0;
+  uint64_t dropped_total = 0;
+
+  int init(int dev_id, float voxel, size_t cap) {
+    device = dev_id;
+    int n = 0;
+    RTR_HIP(hipGetDeviceCount(&n));
+    if (dev_id < 0 || dev_id >= n) {
+      set_error("device %d not available", dev_id);
+      return RTR_ERR_DEVICE;
+    }
+    RTR_HIP(hipSetDevice(device));
+    RTR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    uint64_t slots = 1024;
+    while (slots < 2 * (uint64_t)cap) slots <<= 1;
+    capacity = cap;
+    int rc = keys.ensure(slots * 8);
+    if (!rc) rc = acc.ensure(slots * 16);
+    if (!rc) rc = counters.ensure(16);
+    if (!rc) rc = T.ensure(64);
+    if (rc) return rc;
+    dev.keys = keys.as<unsigned long long>();
+    dev.acc = acc.as<float4>();
+    dev.counters = counters.as<uint32_t>();
+    dev.mask = slots - 1;
+    dev.max_voxels = (uint32_t)cap;
+    dev.h = voxel;
+    dev.inv_h = (float)(1.0 / (double)voxel);
+    return clear();
+  }
+  int clear() {
+    RTR_HIP(hipSetDevice(device));
+    hipLaunchKernelGGL(k_map_clear, dim3(1024), dim3(256), 0, stream, dev);
+    RTR_HIP(hipGetLastError());
+    RTR_HIP(hipStreamSynchronize(stream));
+    dropped_total = 0;
+    return RTR_OK;
+  }
+  ~VoxelMap() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    DevBuf* b[] = {&keys, &acc, &counters, &in, &T, &okeys, &oacc, &ocount};
+    for (DevBuf* x : b) x->release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int read_counters(uint32_t c[4]) {
+    RTR_HIP(hipMemcpyAsync(c, counters.p, 16, hipMemcpyDeviceToHost, stream));
+    RTR_HIP(hipStreamSynchronize(stream));
+    return RTR_OK;
+  }
+  int integrate(const float* xyz, size_t n, size_t stride, const double* Tcm) {
+    RTR_HIP(hipSetDevice(device));
+    if (n == 0) return RTR_OK;
+    if (n >= (1ull << 32)) {
+      set_error("too many points in one call");
+      return RTR_ERR_CAPACITY;
+    }
+    std::vector<float> packed;
+    const float* src = xyz;
+    if (stride != 0 && stride != 12) {
+      packed.resize(n * 3);
+      const char* b = reinterpret_cast<const char*>(xyz);
+      for (size_t i = 0; i < n; ++i) memcpy(packed.data() + 3 * i, b + i * stride, 12);
+      src = packed.data();
+    }
+    int rc = in.ensure(n * 12);
+    if (rc) return rc;
+    RTR_HIP(hipMemcpyAsync(in.p, src, n * 12, hipMemcpyHostToDevice, stream));
+    float m[12];
+    if (Tcm) {
+      tf_rows_f32(Tcm, m);
+      RTR_HIP(hipMemcpyAsync(T.p, m, sizeof(m), hipMemcpyHostToDevice, stream));
+    }
+    uint32_t before[4];
+    rc = read_counters(before);
+    if (rc) return rc;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_map_integrate, dim3(blocks), dim3(256), 0, stream, dev, in.as<float>(),
+                       (uint32_t)n, T.as<float>(), Tcm ? 1 : 0);
+    RTR_HIP(hipGetLastError());
+    uint32_t after[4];
+    rc = read_counters(after);
+    if (rc) return rc;
+    dropped_total = after[1];
+    if (after[2]) {
+      set_error("voxel map capacity exceeded (%zu voxels)", capacity);
+      return RTR_ERR_CAPACITY;
+    }
+    return RTR_OK;
+  }
+  int size(size_t* n) {
+    uint32_t c[4];
+    int rc = read_counters(c);
+    if (rc) return rc;
+    *n = c[0];
+    return RTR_OK;
+  }
+  // host copy of occupied voxels sorted by key: keys, absolute sums (double), counts
+  int gather(std::vector<uint64_t>& k, std::vector<double>& sums, std::vector<uint32_t>& cnt) {
+    RTR_HIP(hipSetDevice(device));
+    size_t nv;
+    int rc = size(&nv);
+    if (rc) return rc;
+    if (nv == 0) {
+      k.clear();
+      sums.clear();
+      cnt.clear();
+      return RTR_OK;
+    }
+    rc = okeys.ensure(nv * 8);
+    if (!rc) rc = oacc.ensure(nv * 16);
+    if (!rc) rc = ocount.ensure(4);
+    if (rc) return rc;
+    launch_fill_u32(ocount.as<uint32_t>(), 0u, 1, stream);
+    hipLaunchKernelGGL(k_map_compact, dim3(1024), dim3(256), 0, stream, dev,
+                       okeys.as<unsigned long long>(), oacc.as<float4>(), ocount.as<uint32_t>(),
+                       (uint32_t)nv);
+    RTR_HIP(hipGetLastError());
+    std::vector<uint64_t> hk(nv);
+    std::vector<float4> ha(nv);
+    RTR_HIP(hipMemcpyAsync(hk.data(), okeys.p, nv * 8, hipMemcpyDeviceToHost, stream));
+    RTR_HIP(hipMemcpyAsync(ha.data(), oacc.p, nv * 16, hipMemcpyDeviceToHost, stream));
+    RTR_HIP(hipStreamSynchronize(stream));
+    std::vector<uint32_t> order(nv);
+    for (size_t i = 0; i < nv; ++i) order[i] = (uint32_t)i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return hk[a] < hk[b]; });
+    k.resize(nv);
+    sums.resize(3 * nv);
+    cnt.resize(nv);
+    for (size_t i = 0; i < nv; ++i) {
+      const uint32_t o = order[i];
+      k[i] = hk[o];
+      uint32_t c;
+      memcpy(&c, &ha[o].w, 4);
+      cnt[i] = c;
+      const float s[3] = {ha[o].x, ha[o].y, ha[o].z};
+      for (int a = 0; a < 3; ++a) {
+        const int kk = (int)((hk[o] >> (21 * a)) & 0x1fffffull) - kBias;
+        const float corner = (float)kk * dev.h;
+        sums[3 * i + a] = (double)corner * c + (double)s[a];
+      }
+    }
+    return RTR_OK;
+  }
+};
+
+}  // namespace rtr
+
+using rtr::set_error;
+using rtr::VoxelMap;
+
+struct rtr_map {
+  VoxelMap m;
+};
+
+extern "C" {
+
+int rtr_map_create(rtr_map** out, int device, float voxel_size, size_t capacity) {
+  try {
+    if (!out || !(voxel_size > 0) || capacity == 0 || capacity > (1ull << 31)) {
+      set_error("invalid voxel map arguments (voxel %g, capacity %zu)", voxel_size, capacity);
+      return RTR_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    rtr_map* m = new rtr_map();
+    int rc = m->m.init(device, voxel_size, capacity);
+    if (rc) {
+      delete m;
+      return rc;
+    }
+    *out = m;
+    return RTR_OK;
+  } catch (...) {
+    set_error("host allocation failed");
+    return RTR_ERR_CAPACITY;
+  }
+}
+
+int rtr_map_destroy(rtr_map* m) {
+  delete m;
+  return RTR_OK;
+}
+
+int rtr_map_clear(rtr_map* m) {
+  if (!m) return RTR_ERR_INVALID_ARG;
+  return m->m.clear();
+}
+
+int rtr_map_integrate(rtr_map* m, const float* xyz, size_t n, size_t stride, const double T[16]) {
+  if (!m || (n && !xyz) || (stride != 0 && stride < 12)) return RTR_ERR_INVALID_ARG;
+  try {
+    return m->m.integrate(xyz, n, stride, T);
+  } catch (...) {
+    set_error("host allocation failed");
+    return RTR_ERR_CAPACITY;
+  }
+}
+
+int rtr_map_size(rtr_map* m, size_t* n) {
+  if (!m || !n) return RTR_ERR_INVALID_ARG;
+  return m->m.size(n);
+}
+
+int rtr_map_dropped(rtr_map* m, uint64_t* n) {
+  if (!m || !n) return RTR_ERR_INVALID_ARG;
+  *n = m->m.dropped_total;
+  return RTR_OK;
+}
+
+int rtr_map_extract(rtr_map* m, int64_t* keys, float* xyz, uint32_t* counts, size_t cap,
+                    size_t* n_out, int as_centroids) {
+  if (!m || !n_out) return RTR_ERR_INVALID_ARG;
+  try {
+    std::vector<uint64_t> k;
+    std::vector<double> s;
+    std::vector<uint32_t> c;
+    int rc = m->m.gather(k, s, c);
+    if (rc) return rc;
+    *n_out = k.size();
+    if (k.size() > cap) {
+      set_error("extract capacity %zu < %zu voxels", cap, k.size());
+      return RTR_ERR_CAPACITY;
+    }
+    for (size_t i = 0; i < k.size(); ++i) {
+      if (keys) keys[i] = (int64_t)k[i];
+      if (counts) counts[i] = c[i];
+      if (xyz)
+        for (int a = 0; a < 3; ++a)
+          xyz[3 * i + a] = as_centroids ? (float)(s[3 * i + a] / (double)c[i]) : (float)s[3 * i + a];
+    }
+    return RTR_OK;
+  } catch (...) {
+    set_error("host allocation failed");
+    return RTR_ERR_CAPACITY;
+  }
+}
+
+int rtr_voxel_downsample(int device, const float* xyz, size_t n, size_t stride, float leaf,
+                         float* out, size_t cap, size_t* n_out) {
+  if ((n && !xyz) || !(leaf > 0) || !n_out) return RTR_ERR_INVALID_ARG;
+  try {
+    VoxelMap vm;
+    int rc = vm.init(device, leaf, n > 0 ? n : 1);
+    if (rc) return rc;
+    rc = vm.integrate(xyz, n, stride, nullptr);
+    if (rc) return rc;
+    std::vector<uint64_t> k;
+    std::vector<double> s;
+    std::vector<uint32_t> c;
+    rc = vm.gather(k, s, c);
+    if (rc) return rc;
+    const size_t nv = k.size();
+    *n_out = nv;
+    if (nv > cap) {
+      set_error("downsample capacity %zu < %zu voxels", cap, nv);
+      return RTR_ERR_CAPACITY;
+    }
+    // PCL VoxelGrid order: linear index over the ijk bounding box, x fastest
+    int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+    std::vector<int64_t> ijk(3 * nv);
+    for (size_t i = 0; i < nv; ++i)
+      for (int a = 0; a < 3; ++a) {
+        const int64_t v = (int64_t)((k[i] >> (21 * a)) & 0x1fffffull) - (1 << 20);
+        ijk[3 * i + a] = v;
+        lo[a] = std::min(lo[a], v);
+        hi[a] = std::max(hi[a], v);
+      }
+    const int64_t dx = hi[0] - lo[0] + 1, dy = hi[1] - lo[1] + 1;
+    std::vector<std::pair<int64_t, uint32_t>> lin(nv);
+    for (size_t i = 0; i < nv; ++i)
+      lin[i] = {(ijk[3 * i] - lo[0]) + (ijk[3 * i + 1] - lo[1]) * dx + (ijk[3 * i + 2] - lo[2]) * dx * dy,
+                (uint32_t)i};
+    std::sort(lin.begin(), lin.end());
+    for (size_t o = 0; o < nv; ++o) {
+      const uint32_t i = lin[o].second;
+      for (int a = 0; a < 3; ++a) out[3 * o + a] = (float)(s[3 * i + a] / (double)c[i]);
+    }
+    return RTR_OK;
+  } catch (...) {
+    set_error("host allocation failed");
+    return RTR_ERR_CAPACITY;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+"""GPU parity tests of the registration path against the CPU oracle.
+
+Bars: correspondence indices and squared distances bit-exact against brute
+force (lowest-index ties); transforms within 1e-5 rotation (Frobenius) and
+1e-5 m translation of the FP64 oracle ICP; the transformed output cloud
+bit-exact against the oracle's float transform of the same T.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import realtime_robot_amd as R
+from helpers import golden, rot_err, trans_err
+from pcd_np import read_xyz
+from realtime_robot_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL_R = 1e-5
+TOL_T = 1e-5
+
+
+def _icp(src, tgt, max_dist=None, iters=None, eps=None):
+    icp = R.IterativeClosestPoint()
+    icp.setInputSource(src)
+    icp.setInputTarget(tgt)
+    if max_dist is not None:
+        icp.setMaxCorrespondenceDistance(max_dist)
+    if iters is not None:
+        icp.setMaximumIterations(iters)
+    if eps is not None:
+        icp.setTransformationEpsilon(eps)
+    return icp
+
+
+def _perturb(T, deg, t, seed):
+    rng = np.random.default_rng(seed)
+    D = synth.random_rigid(rng, deg, t)
+    return D @ T
+
+
+@pytest.mark.parametrize("max_dist", [0.05, 0.02, None])
+def test_correspondences_bitexact_config1(max_dist):
+    src, tgt, T_gt = synth.config1_pair(0)
+    icp = _icp(src, tgt, max_dist=max_dist)
+    for k, T in enumerate([np.eye(4), T_gt, _perturb(T_gt, 3.0, 0.01, 7)]):
+        idx, d2 = icp.getCorrespondences(T)
+        q = oracle.transform(T, src)
+        i_ref, d_ref = oracle.nn_brute(q, tgt, max_dist)
+        assert np.array_equal(idx, i_ref), f"T{k}: {np.sum(idx != i_ref)} index mismatches"
+        assert np.array_equal(d2, d_ref), f"T{k}: squared distances differ"
+
+
+def test_correspondences_ties_and_nans():
+    rng = np.random.default_rng(5)
+    base = np.round(rng.random((4000, 3)) * 20) / 20  # grid-aligned: many exact ties
+    tgt = np.concatenate([base, base[:1000]]).astype(np.float32)  # duplicates
+    tgt[17] = np.nan
+    tgt[123] = [np.inf, 0, 0]
+    src = (np.round(rng.random((3000, 3)) * 40) / 40).astype(np.float32)
+    src[5] = np.nan
+    icp = _icp(src, tgt, max_dist=0.06)
+    idx, d2 = icp.getCorrespondences(np.eye(4))
+    i_ref, d_ref = oracle.nn_brute(src, tgt, 0.06)
+    assert np.array_equal(idx, i_ref)
+    assert np.array_equal(d2, d_ref)
+    assert idx[5] == -1
+
+
+def test_known_answer_pure_translation():
+    """5.pcd -> mcloud.pcd is a pure translation (0.8, 0.8, 0) (index-aligned
+    output of an earlier reference run); plain ICP with PCL's unbounded
+    correspondence distance recovers it."""
+    a = read_xyz(golden("5.pcd"))
+    b = read_xyz(golden("mcloud.pcd"))
+    icp = _icp(a, b, iters=50)
+    out = icp.align()
+    T = icp.getFinalTransformation()
+    assert abs(T[0, 3] - 0.8) < 1e-5 and abs(T[1, 3] - 0.8) < 1e-5 and abs(T[2, 3]) < 1e-5
+    assert rot_err(T, np.eye(4)) < 1e-5
+    ref = oracle.icp(a, b, oracle.params(max_iter=50))
+    assert rot_err(T, ref["T"]) < TOL_R and trans_err(T, ref["T"]) < TOL_T
+    assert np.array_equal(out, oracle.transform(T, a))
+
+
+def test_align_config1_matches_oracle():
+    src, tgt, T_gt = synth.config1_pair(0)
+    icp = _icp(src, tgt, max_dist=0.05, iters=50, eps=1e-8)
+    out = icp.align()
+    T = icp.getFinalTransformation()
+    res = icp.getResult()
+    ref = oracle.icp(src, tgt, oracle.params(max_iter=50, r_max=0.05, eps_T=1e-8))
+    print("gpu", res["iterations"], res["state"], "oracle", ref["iterations"], ref["state"],
+          "rot", rot_err(T, ref["T"]), "t", trans_err(T, ref["T"]))
+    assert rot_err(T, ref["T"]) < TOL_R and trans_err(T, ref["T"]) < TOL_T
+    assert res["iterations"] == ref["iterations"]
+    assert res["state"] == ref["state"]
+    assert res["n_corr"] == ref["n_corr"]
+    assert abs(res["mse"] - ref["mse"]) <= 1e-6 * ref["mse"]
+    assert np.array_equal(out, oracle.transform(T, src))
+    # the chair is recovered: within 1 mm of ground truth
+    assert synth.rmse_between(T, T_gt, src) < 1e-3
+    f = icp.getFitnessScore()
+    f_ref = oracle.fitness(src, tgt, T)
+    assert abs(f - f_ref) <= 1e-9 + 1e-6 * f_ref
+
+
+def test_align_with_guess_and_max_range_fitness():
+    src, tgt, T_gt = synth.config1_pair(1)
+    guess = _perturb(T_gt, 5.0, 0.02, 3)
+    icp = _icp(src, tgt, max_dist=0.05, iters=30, eps=1e-8)
+    icp.align(guess)
+    T = icp.getFinalTransformation()
+    ref = oracle.icp(src, tgt, oracle.params(max_iter=30, r_max=0.05, eps_T=1e-8), guess=guess)
+    assert rot_err(T, ref["T"]) < TOL_R and trans_err(T, ref["T"]) < TOL_T
+    for mr in (1e-6, 1e-5, 4e-6):
+        f = icp.getFitnessScore(mr)
+        f_ref = oracle.fitness(src, tgt, T, mr)
+        assert (f == f_ref) if f_ref == np.finfo(np.float64).max else abs(f - f_ref) <= 1e-6 * f_ref
+
+
+def test_no_correspondences_is_reported():
+    src, tgt, _ = synth.config1_pair(2)
+    icp = _icp(src + np.float32(10.0), tgt, max_dist=0.01, iters=10)
+    icp.align()
+    r = icp.getResult()
+    assert r["state"] == R._lib.FAIL_NO_CORRESPONDENCES and not icp.hasConverged()
+    assert np.allclose(icp.getFinalTransformation(), np.eye(4))
+    ref = oracle.icp(src + np.float32(10.0), tgt, oracle.params(max_iter=10, r_max=0.01))
+    assert ref["state"] == oracle.FAIL_NO_CORR
+
+
+def test_empty_inputs_raise():
+    icp = R.IterativeClosestPoint()
+    icp.setInputSource(np.zeros((0, 3), np.float32))
+    icp.setInputTarget(np.random.rand(100, 3).astype(np.float32))
+    with pytest.raises(R.RtrError) as e:
+        icp.align()
+    assert e.value.code == R._lib.ERR_EMPTY_INPUT
+
+
+def test_batch_matches_oracle_and_graph_replay_is_stable():
+    srcs, tgts, gts = synth.config3_batch(6, seed_offset=500, ns_range=(2000, 6000), nt_range=(8000, 20000),
+                                          max_deg=10.0, max_t=0.05)
+    b = R.BatchRegistration()
+    b.set_clouds(srcs, tgts)
+    b.set_problems(np.arange(6), np.arange(6))
+    b.set_params(R.make_params(max_iterations=40, max_correspondence_distance=0.05, transformation_epsilon=1e-8))
+    r1 = b.align()
+    r2 = b.align()
+    b.set_graph(False)
+    r3 = b.align()
+    for p in range(6):
+        for r in (r2, r3):
+            assert np.array_equal(r[p]["T"], r1[p]["T"]), "replay is not deterministic"
+        ref = oracle.icp(srcs[p], tgts[p], oracle.params(max_iter=40, r_max=0.05, eps_T=1e-8))
+        T = r1[p]["T"]
+        print(p, r1[p]["iterations"], ref["iterations"], rot_err(T, ref["T"]), trans_err(T, ref["T"]))
+        assert rot_err(T, ref["T"]) < TOL_R and trans_err(T, ref["T"]) < TOL_T
+        assert r1[p]["iterations"] == ref["iterations"]
+
+
+def test_batch_shared_clouds_multistart():
+    src, tgt, T_gt = synth.config1_pair(3)
+    b = R.BatchRegistration()
+    b.set_clouds([src], [tgt])
+    b.set_problems([0, 0, 0], [0, 0, 0])
+    b.set_params(R.make_params(max_iterations=25, max_correspondence_distance=0.05, transformation_epsilon=1e-8))
+    guesses = [np.eye(4), _perturb(T_gt, 4, 0.01, 1), T_gt]
+    res = b.align(guesses)
+    for g, r in zip(guesses, res):
+        ref = oracle.icp(src, tgt, oracle.params(max_iter=25, r_max=0.05, eps_T=1e-8), guess=g)
+        assert rot_err(r["T"], ref["T"]) < TOL_R and trans_err(r["T"], ref["T"]) < TOL_T
