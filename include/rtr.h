@@ -161,6 +161,12 @@ int rtr_batch_compute_target_normals(rtr_batch* b, int k, const float vp[3]);
 /* Problems: problem p registers source src_id[p] onto target tgt_id[p]. */
 int rtr_batch_set_problems(rtr_batch* b, int n, const int* src_id, const int* tgt_id);
 int rtr_batch_set_params(rtr_batch* b, const rtr_params* p);
+/* Launch/layout tuning (the autotuner's knobs): requested cell edge as a
+ * fraction of the max correspondence distance, cell budget per target point,
+ * source points per work item (multiple of 256), persistent workgroups per CU
+ * (0 = occupancy limit).  Values <= 0 keep the current setting. */
+int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point, int item_pts,
+                         int blocks_per_cu);
 /* Enable/disable hipGraph replay of align (default on). */
 int rtr_batch_set_graph(rtr_batch* b, int on);
 int rtr_batch_set_profiling(rtr_batch* b, int on);
@@ -216,6 +222,11 @@ int rtr_pcd_write_ascii(const char* path, const float* xyz, size_t n, size_t str
 /* The grid NN search on the CPU over the same cell-sorted layout. */
 int rtr_selfcheck_nn(const float* tgt, size_t nt, const float* q, size_t nq, float cell,
                      double max_dist, int32_t* idx, float* d2);
+/* Same, also returning per query the number of cell ranges walked and of
+ * candidates examined (the search work; input of the cell-size cost model). */
+int rtr_selfcheck_nn_stats(const float* tgt, size_t nt, const float* q, size_t nq, float cell,
+                           double max_dist, int32_t* idx, float* d2, uint32_t* ranges,
+                           uint32_t* cands);
 /* Reduce one iteration's moments and solve (FP64) on the CPU:
  * a[i] (transformed source), b[i] (target), nrm (nullable) -> Tinc. */
 int rtr_selfcheck_solve(int mode, const float* a, const float* b, const float* nrm, size_t n,

@@ -90,6 +90,7 @@ _SIGS = {
     "rtr_batch_compute_target_normals": (_i, [_vp, _i, _vp]),
     "rtr_batch_set_problems": (_i, [_vp, _i, _vp, _vp]),
     "rtr_batch_set_params": (_i, [_vp, C.POINTER(Params)]),
+    "rtr_batch_set_tuning": (_i, [_vp, _f, _f, _i, _i]),
     "rtr_batch_set_graph": (_i, [_vp, _i]),
     "rtr_batch_set_profiling": (_i, [_vp, _i]),
     "rtr_batch_align": (_i, [_vp, _vp, _vp]),
@@ -108,6 +109,7 @@ _SIGS = {
     "rtr_pcd_read": (_i, [C.c_char_p, _vp, _vp, _sz, C.POINTER(_sz)]),
     "rtr_pcd_write_ascii": (_i, [C.c_char_p, _vp, _sz, _sz]),
     "rtr_selfcheck_nn": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp]),
+    "rtr_selfcheck_nn_stats": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp, _vp, _vp]),
     "rtr_selfcheck_solve": (_i, [_i, _vp, _vp, _vp, _sz, _vp, _vp]),
 }
 

@@ -383,6 +383,23 @@ int rtr_batch_set_params(rtr_batch* b, const rtr_params* p) {
   RTR_GUARD_END
 }
 
+int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point, int item_pts,
+                         int blocks_per_cu) {
+  RTR_GUARD_BEGIN
+  if (!b) return RTR_ERR_INVALID_ARG;
+  if (item_pts > 0 && (item_pts % 256 != 0 || item_pts > (1 << 20))) {
+    set_error("item_pts must be a positive multiple of 256 (got %d)", item_pts);
+    return RTR_ERR_INVALID_ARG;
+  }
+  rtr::TuneParams& t = b->eng.tune();
+  if (cell_factor > 0) t.cell_factor = cell_factor;
+  if (cells_per_point > 0) t.cells_per_point = cells_per_point;
+  if (item_pts > 0) t.item_pts = item_pts;
+  if (blocks_per_cu >= 0) t.blocks_per_cu = blocks_per_cu;
+  return b->eng.set_params(b->eng.params());
+  RTR_GUARD_END
+}
+
 int rtr_batch_set_graph(rtr_batch* b, int on) {
   if (!b) return RTR_ERR_INVALID_ARG;
   b->eng.set_graph(on != 0);
@@ -426,8 +443,9 @@ int rtr_batch_fitness(rtr_batch* b, double max_range, double* out) {
 // self-check hooks: the same geometry code on the host
 // ---------------------------------------------------------------------------
 
-int rtr_selfcheck_nn(const float* tgt, size_t nt, const float* q, size_t nq, float cell,
-                     double max_dist, int32_t* idx, float* d2) {
+int rtr_selfcheck_nn_stats(const float* tgt, size_t nt, const float* q, size_t nq, float cell,
+                           double max_dist, int32_t* idx, float* d2, uint32_t* ranges,
+                           uint32_t* cands) {
   RTR_GUARD_BEGIN
   using namespace rtr;
   if ((nt && !tgt) || (nq && (!q || !idx || !d2)) || !(cell > 0)) return RTR_ERR_INVALID_ARG;
@@ -499,8 +517,43 @@ int rtr_selfcheck_nn(const float* tgt, size_t nt, const float* q, size_t nq, flo
     r2f = (float)r2;
     if ((double)r2f > r2) r2f = std::nextafter(r2f, 0.0f);
   }
+  // distance field exactly as the device grid build makes it
+  int skip_ring, dt_cap;
+  dist_caps(max_dist < 3.0e38 ? (float)max_dist : 3.0e38f, g.h, g.margin, &skip_ring, &dt_cap);
+  std::vector<uint8_t> da(ncell), db(ncell);
+  const int dims[3] = {g.dx, g.dy, g.dz};
+  const int strides[3] = {1, g.dx, g.dx * g.dy};
+  for (int axis = 0; axis < 3; ++axis) {
+    const std::vector<uint8_t>& src = axis == 1 ? da : db;
+    std::vector<uint8_t>& dst = axis == 1 ? db : da;
+    for (size_t c = 0; c < ncell; ++c) {
+      const int pos = axis == 0 ? (int)(c % g.dx) : (axis == 1 ? (int)((c / g.dx) % g.dy) : (int)(c / ((size_t)g.dx * g.dy)));
+      int best = dt_cap + 1;
+      for (int d = -dt_cap; d <= dt_cap; ++d) {
+        if (pos + d < 0 || pos + d >= dims[axis]) continue;
+        const size_t cc = (size_t)((long long)c + (long long)d * strides[axis]);
+        int v = axis == 0 ? (start[cc + 1] > start[cc] ? 0 : dt_cap + 1) : src[cc];
+        const int ad = d < 0 ? -d : d;
+        v = v > ad ? v : ad;
+        best = v < best ? v : best;
+      }
+      dst[c] = (uint8_t)best;
+    }
+  }
   for (size_t i = 0; i < nq; ++i) {
-    NNBest b = nn_search(g, q[3 * i], q[3 * i + 1], q[3 * i + 2], r2f);
+    NNCountVisitor v;
+    v.pts = g.pts;
+    v.qx = q[3 * i];
+    v.qy = q[3 * i + 1];
+    v.qz = q[3 * i + 2];
+    v.best.key = ~0ull;
+    v.best.tx = v.best.ty = v.best.tz = 0.f;
+    v.best.slot = 0xffffffffu;
+    ring_walk_dt(g, da.data(), skip_ring, v.qx, v.qy, v.qz, r2f, v);
+    NNBest b = v.best;
+    if (b.key != ~0ull && !(key_d2(b.key) <= r2f)) b.key = ~0ull;
+    if (ranges) ranges[i] = v.ranges;
+    if (cands) cands[i] = v.cands;
     if (b.key != ~0ull) {
       idx[i] = (int32_t)(uint32_t)(b.key & 0xffffffffull);
       d2[i] = key_d2(b.key);
@@ -511,6 +564,11 @@ int rtr_selfcheck_nn(const float* tgt, size_t nt, const float* q, size_t nq, flo
   }
   return RTR_OK;
   RTR_GUARD_END
+}
+
+int rtr_selfcheck_nn(const float* tgt, size_t nt, const float* q, size_t nq, float cell,
+                     double max_dist, int32_t* idx, float* d2) {
+  return rtr_selfcheck_nn_stats(tgt, nt, q, nq, cell, max_dist, idx, d2, nullptr, nullptr);
 }
 
 int rtr_selfcheck_solve(int mode, const float* a, const float* b, const float* nrm, size_t n,
@@ -529,8 +587,8 @@ int rtr_selfcheck_solve(int mode, const float* a, const float* b, const float* n
                 0.f, t);
     for (int k = 0; k < ns; ++k) v[k] += t[k];
   }
-  double mse;
-  solve_increment(mode, v, anchor, Tinc, &mse);
+  double mse, ws[64];
+  solve_increment(mode, v, anchor, Tinc, &mse, ws);
   return RTR_OK;
   RTR_GUARD_END
 }

@@ -72,7 +72,7 @@ Engine::~Engine() {
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&tgt_raw_, &tgt_nrm_raw_, &tgt_sorted_, &tgt_nrm_sorted_, &cell_count_,
-                    &cell_start_, &cell_id_, &tgt_desc_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
+                    &cell_start_, &cell_id_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
                     &bin_count_, &bin_start_, &bin_id_, &src_desc_, &prob_desc_, &state_,
                     &guesses_, &items_buf_, &partials_, &done_, &counters_, &scan_tmp_,
                     &out_i32_, &out_f32_, &out_T_};
@@ -102,7 +102,7 @@ static void pack_xyz(const float* src, size_t n, size_t stride, float* dst) {
 
 static uint32_t source_bins(size_t n) {
   int P = 1;
-  double want = std::cbrt((double)n / 16.0);
+  double want = std::cbrt((double)n / 4.0);
   while (P < want && P < 32) P <<= 1;
   return (uint32_t)(P * P * P);
 }
@@ -240,6 +240,7 @@ int Engine::upload_problems() {
   if (h_req <= 0 && rmax < 1e6) h_req = (float)(tune_.cell_factor * rmax);
   const float cpp = prm_.cells_per_point > 0 ? prm_.cells_per_point : tune_.cells_per_point;
   uint32_t coff = 0;
+  max_cells_ = 0;
   for (auto& d : tgts_) {
     d.cell_off = coff;
     double b = std::ceil((double)cpp * d.n) + 64.0;
@@ -247,10 +248,13 @@ int Engine::upload_problems() {
     d.cell_budget = (uint32_t)b;
     d.h_req = h_req;
     coff += d.cell_budget + 1;
+    if ((int)d.cell_budget > max_cells_) max_cells_ = (int)d.cell_budget;
   }
   n_cells_ = coff;
   RTR_TRY(cell_count_.ensure((size_t)n_cells_ * 4));
   RTR_TRY(cell_start_.ensure((size_t)n_cells_ * 4));
+  RTR_TRY(cell_dist_.ensure((size_t)n_cells_));
+  RTR_TRY(cell_dist_tmp_.ensure((size_t)n_cells_));
   size_t scan_words = scan_tmp_words(n_cells_ > n_bins_ ? n_cells_ : n_bins_) + 16;
   RTR_TRY(scan_tmp_.ensure(scan_words * 4));
   RTR_TRY(tgt_desc_.ensure(tgts_.size() * sizeof(TargetDesc)));
@@ -301,6 +305,9 @@ DevEngine Engine::dev_view() const {
   e.cell_count = cell_count_.as<uint32_t>();
   e.cell_start = cell_start_.as<uint32_t>();
   e.cell_id = cell_id_.as<uint32_t>();
+  e.cell_dist = cell_dist_.as<uint8_t>();
+  e.cell_dist_tmp = cell_dist_tmp_.as<uint8_t>();
+  e.r_max = prm_.max_correspondence_distance < 3.0e38 ? (float)prm_.max_correspondence_distance : 3.0e38f;
   e.n_cells_total = n_cells_;
   e.src_raw = src_raw_.as<float>();
   e.srcs = src_desc_.as<SourceDesc>();
@@ -358,8 +365,8 @@ int Engine::grid_blocks() const {
 
 int Engine::enqueue_build() {
   DevEngine e = dev_view();
-  launch_target_grid(e, max_tgt_, stream_);
-  launch_source_sort(e, max_src_, stream_);
+  launch_target_grid(e, max_tgt_, max_cells_, stream_);
+  launch_source_sort(e, max_src_, (uint32_t)src_total_, stream_);
   RTR_HIP(hipGetLastError());
   return RTR_OK;
 }

@@ -105,9 +105,11 @@ class Engine {
   size_t tgt_total_ = 0, src_total_ = 0, pt_total_ = 0;
   int max_tgt_ = 0, max_src_ = 0;
   uint32_t n_cells_ = 0, n_bins_ = 0;
+  int max_cells_ = 0;
 
   // device arena
   DevBuf tgt_raw_, tgt_nrm_raw_, tgt_sorted_, tgt_nrm_sorted_, cell_count_, cell_start_, cell_id_;
+  DevBuf cell_dist_, cell_dist_tmp_;
   DevBuf tgt_desc_;
   DevBuf src_raw_, sx_, sy_, sz_, perm_, bin_count_, bin_start_, bin_id_, src_desc_;
   DevBuf prob_desc_, state_, guesses_, items_buf_, partials_, done_, counters_, scan_tmp_;

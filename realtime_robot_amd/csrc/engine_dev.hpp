@@ -22,6 +22,9 @@ struct DevEngine {
   uint32_t* cell_count;
   uint32_t* cell_start;
   uint32_t* cell_id;         // per raw target point
+  uint8_t* cell_dist;        // per cell: Chebyshev distance (in cells) to the nearest occupied cell
+  uint8_t* cell_dist_tmp;    // ping-pong buffer of the separable distance transform
+  float r_max;               // max correspondence distance (inf when unbounded)
   uint32_t n_cells_total;    // sum of (budget + 1)
   // sources: raw packed xyz, spatially sorted SoA and the permutation
   const float* src_raw;
@@ -57,8 +60,8 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
                            hipStream_t s);
 size_t scan_tmp_words(uint32_t n);
-void launch_target_grid(const DevEngine& e, int max_tgt_pts, hipStream_t s);
-void launch_source_sort(const DevEngine& e, int max_src_pts, hipStream_t s);
+void launch_target_grid(const DevEngine& e, int max_tgt_pts, int max_cells, hipStream_t s);
+void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s);
 void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s);
 int icp_blocks_per_cu(int mode);
 // correspondences / fitness / transformed output for problem set; T per

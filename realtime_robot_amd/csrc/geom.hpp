@@ -92,102 +92,167 @@ RTR_HD void scan_range(const float4* __restrict__ pts, uint32_t b, uint32_t e, f
 
 RTR_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Ring walk over a uniform grid of cell-sorted points.
-//
-// Rings of cells at Chebyshev distance k = 0, 1, 2, ... around the query's
-// (clamped) cell are handed to vis.scan(begin, end) as contiguous ranges of
-// the sorted array (an x-row of cells is contiguous), until every unsearched
-// cell is provably farther than vis.bound() or than the acceptance bound r2f.
-// The bound is the distance from q to the faces of the searched box, less
-// g.margin, which absorbs the float rounding of the cell assignment
-// floor((p - o) * inv_h); squared and scaled by (1 - 2e-6) it undercuts the
-// float d2 of any unsearched point (float d2 is within 6 ulp of exact).
-// Rings 0 and 1 are walked together as the nine x-rows of the 3x3x3 block.
-template <class V>
-RTR_HD void ring_walk(const GridView& g, float qx, float qy, float qz, float r2f, V& vis) {
+// Cell of a query, clamped to the grid; false for a NaN query or an empty grid.
+RTR_HD bool query_cell(const GridView& g, float qx, float qy, float qz, int& cx, int& cy, int& cz) {
   float fx = (qx - g.ox) * g.inv_h;
   float fy = (qy - g.oy) * g.inv_h;
   float fz = (qz - g.oz) * g.inv_h;
-  if (!(fx == fx) || !(fy == fy) || !(fz == fz)) return;  // NaN query
-  if (g.n_pts == 0) return;
+  if (!(fx == fx) || !(fy == fy) || !(fz == fz)) return false;  // NaN query
+  if (g.n_pts == 0) return false;
   // clamp in float before converting (far queries must not overflow int)
   const float lim = 1.0e9f;
   fx = fx < -1.f ? -1.f : (fx > lim ? lim : fx);
   fy = fy < -1.f ? -1.f : (fy > lim ? lim : fy);
   fz = fz < -1.f ? -1.f : (fz > lim ? lim : fz);
-  const int cx = clampi((int)floorf(fx), 0, g.dx - 1);
-  const int cy = clampi((int)floorf(fy), 0, g.dy - 1);
-  const int cz = clampi((int)floorf(fz), 0, g.dz - 1);
-  const int kmax = (g.dx > g.dy ? (g.dx > g.dz ? g.dx : g.dz) : (g.dy > g.dz ? g.dy : g.dz));
-  for (int k = 1; k <= kmax; ++k) {
-    if (k == 1) {
-      const int x0 = cx > 0 ? cx - 1 : 0;
-      const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
-      for (int dz = -1; dz <= 1; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= g.dz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-          const int y = cy + dy;
-          if (y < 0 || y >= g.dy) continue;
-          const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
-          vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
-        }
-      }
-    } else {
-      for (int dz = -k; dz <= k; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= g.dz) continue;
-        const bool zface = (dz == -k || dz == k);
-        for (int dy = -k; dy <= k; ++dy) {
-          const int y = cy + dy;
-          if (y < 0 || y >= g.dy) continue;
-          const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
-          if (zface || dy == -k || dy == k) {
-            const int x0 = clampi(cx - k, 0, g.dx - 1);
-            const int x1 = clampi(cx + k, 0, g.dx - 1);
-            vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
-          } else {
-            if (cx - k >= 0) vis.scan(g.cell_start[row + cx - k], g.cell_start[row + cx - k + 1]);
-            if (cx + k < g.dx) vis.scan(g.cell_start[row + cx + k], g.cell_start[row + cx + k + 1]);
-          }
-        }
-      }
-    }
-    // lower bound on the distance from q to any cell outside the searched box
-    float lb = 3.0e38f;
-    bool any = false;
-    if (cx - k > 0) {
-      lb = fminf(lb, qx - (g.ox + (float)(cx - k) * g.h));
-      any = true;
-    }
-    if (cx + k < g.dx - 1) {
-      lb = fminf(lb, (g.ox + (float)(cx + k + 1) * g.h) - qx);
-      any = true;
-    }
-    if (cy - k > 0) {
-      lb = fminf(lb, qy - (g.oy + (float)(cy - k) * g.h));
-      any = true;
-    }
-    if (cy + k < g.dy - 1) {
-      lb = fminf(lb, (g.oy + (float)(cy + k + 1) * g.h) - qy);
-      any = true;
-    }
-    if (cz - k > 0) {
-      lb = fminf(lb, qz - (g.oz + (float)(cz - k) * g.h));
-      any = true;
-    }
-    if (cz + k < g.dz - 1) {
-      lb = fminf(lb, (g.oz + (float)(cz + k + 1) * g.h) - qz);
-      any = true;
-    }
-    if (!any) break;  // the whole grid has been searched
-    lb = lb - g.margin;
-    if (lb > 0.f) {
-      const float lb2 = (lb * lb) * 0.999998f;
-      if (vis.bound() < lb2) break;
-      if (lb2 > r2f) break;
+  cx = clampi((int)floorf(fx), 0, g.dx - 1);
+  cy = clampi((int)floorf(fy), 0, g.dy - 1);
+  cz = clampi((int)floorf(fz), 0, g.dz - 1);
+  return true;
+}
+
+// After the box of cells within Chebyshev distance k of (cx, cy, cz) has been
+// searched: true when every unsearched cell is provably farther than
+// vis.bound() or than the acceptance bound r2f (or none is left).  The bound
+// is the distance from q to the faces of the searched box, less g.margin,
+// which absorbs the float rounding of the cell assignment
+// floor((p - o) * inv_h); squared and scaled by (1 - 2e-6) it undercuts the
+// float d2 of any unsearched point (float d2 is within 6 ulp of exact).
+template <class V>
+RTR_HD bool ring_done(const GridView& g, float qx, float qy, float qz, int cx, int cy, int cz, int k,
+                      float r2f, const V& vis) {
+  float lb = 3.0e38f;
+  bool any = false;
+  if (cx - k > 0) {
+    lb = fminf(lb, qx - (g.ox + (float)(cx - k) * g.h));
+    any = true;
+  }
+  if (cx + k < g.dx - 1) {
+    lb = fminf(lb, (g.ox + (float)(cx + k + 1) * g.h) - qx);
+    any = true;
+  }
+  if (cy - k > 0) {
+    lb = fminf(lb, qy - (g.oy + (float)(cy - k) * g.h));
+    any = true;
+  }
+  if (cy + k < g.dy - 1) {
+    lb = fminf(lb, (g.oy + (float)(cy + k + 1) * g.h) - qy);
+    any = true;
+  }
+  if (cz - k > 0) {
+    lb = fminf(lb, qz - (g.oz + (float)(cz - k) * g.h));
+    any = true;
+  }
+  if (cz + k < g.dz - 1) {
+    lb = fminf(lb, (g.oz + (float)(cz + k + 1) * g.h) - qz);
+    any = true;
+  }
+  if (!any) return true;  // the whole grid has been searched
+  lb = lb - g.margin;
+  if (lb > 0.f) {
+    const float lb2 = (lb * lb) * 0.999998f;
+    if (vis.bound() < lb2) return true;
+    if (lb2 > r2f) return true;
+  }
+  return false;
+}
+
+// The 3x3x3 block around (cx, cy, cz) (rings 0 and 1) as nine x-rows; each
+// x-row of cells is one contiguous range of the cell-sorted array.
+template <class V>
+RTR_HD void walk_block(const GridView& g, int cx, int cy, int cz, V& vis) {
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= g.dz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.dy) continue;
+      const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+      vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
     }
   }
+}
+
+// Cells at Chebyshev distance exactly k (k >= 2): full x-rows on the y/z
+// faces of the shell, the two end cells elsewhere.
+template <class V>
+RTR_HD void walk_shell(const GridView& g, int cx, int cy, int cz, int k, V& vis) {
+  for (int dz = -k; dz <= k; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= g.dz) continue;
+    const bool zface = (dz == -k || dz == k);
+    for (int dy = -k; dy <= k; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.dy) continue;
+      const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+      if (zface || dy == -k || dy == k) {
+        const int x0 = clampi(cx - k, 0, g.dx - 1);
+        const int x1 = clampi(cx + k, 0, g.dx - 1);
+        vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
+      } else {
+        if (cx - k >= 0) vis.scan(g.cell_start[row + cx - k], g.cell_start[row + cx - k + 1]);
+        if (cx + k < g.dx) vis.scan(g.cell_start[row + cx + k], g.cell_start[row + cx + k + 1]);
+      }
+    }
+  }
+}
+
+// Shells k0 + 1, k0 + 2, ... once the box of radius k0 has been searched
+// (or is known to be empty), each preceded by the stopping test.
+template <class V>
+RTR_HD void walk_outer(const GridView& g, float qx, float qy, float qz, int cx, int cy, int cz,
+                       float r2f, V& vis, int k0 = 1) {
+  const int kmax = (g.dx > g.dy ? (g.dx > g.dz ? g.dx : g.dz) : (g.dy > g.dz ? g.dy : g.dz));
+  for (int k = k0; k <= kmax; ++k) {
+    if (ring_done(g, qx, qy, qz, cx, cy, cz, k, r2f, vis)) return;
+    walk_shell(g, cx, cy, cz, k + 1, vis);
+  }
+}
+
+// Distance-field parameters for a grid of cell edge h: a query whose cell is
+// at Chebyshev distance >= R from every occupied cell is at least
+// (R - 1) * h - margin from every target point; skip_ring is the smallest R
+// for which that exceeds r_max (255: never), dt_cap the largest stored value
+// below the cap marker dt_cap + 1.
+RTR_HD void dist_caps(float r_max, float h, float margin, int* skip_ring, int* dt_cap) {
+  const double rr = ((double)r_max + (double)margin) / (double)h;
+  if (!(rr < 12.0)) {
+    *skip_ring = 255;
+    *dt_cap = 14;
+  } else {
+    *skip_ring = (int)floor(rr) + 2;
+    *dt_cap = *skip_ring < 14 ? *skip_ring : 14;
+  }
+}
+
+// Ring walk that first consults the cell's distance D (a lower bound on the
+// Chebyshev distance to the nearest occupied cell): D >= skip_ring means no
+// point within r_max; D >= 2 means the 3x3x3 block is empty and the walk
+// starts at shell D.
+template <class V>
+RTR_HD void ring_walk_dt(const GridView& g, const uint8_t* dist, int skip_ring, float qx, float qy,
+                         float qz, float r2f, V& vis) {
+  int cx, cy, cz;
+  if (!query_cell(g, qx, qy, qz, cx, cy, cz)) return;
+  const int D = dist[((uint32_t)cz * g.dy + cy) * g.dx + cx];
+  if (D >= skip_ring) return;
+  int k0 = 1;
+  if (D >= 2)
+    k0 = D - 1;
+  else
+    walk_block(g, cx, cy, cz, vis);
+  walk_outer(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
+}
+
+// Exact search by rings of cells at growing Chebyshev distance around the
+// query's (clamped) cell, stopping as soon as ring_done() holds.
+template <class V>
+RTR_HD void ring_walk(const GridView& g, float qx, float qy, float qz, float r2f, V& vis) {
+  int cx, cy, cz;
+  if (!query_cell(g, qx, qy, qz, cx, cy, cz)) return;
+  walk_block(g, cx, cy, cz, vis);
+  walk_outer(g, qx, qy, qz, cx, cy, cz, r2f, vis);
 }
 
 RTR_HD float key_d2(uint64_t key) {
@@ -205,6 +270,17 @@ struct NNVisitor {
   NNBest best;
   RTR_HD void scan(uint32_t b, uint32_t e) { scan_range(pts, b, e, qx, qy, qz, best); }
   RTR_HD float bound() const { return best.key == ~0ull ? 3.4e38f : key_d2(best.key); }
+};
+
+// NNVisitor that also counts the ranges and candidates it was handed (the
+// search work of one query; used by the host-side cost model).
+struct NNCountVisitor : NNVisitor {
+  uint32_t ranges = 0, cands = 0;
+  RTR_HD void scan(uint32_t b, uint32_t e) {
+    ++ranges;
+    cands += e - b;
+    NNVisitor::scan(b, e);
+  }
 };
 
 // Exact nearest neighbour (lexicographic (d2, index) minimum) accepted iff
@@ -239,6 +315,7 @@ RTR_HD void jacobi4(double* a, double* V) {
       for (int q = p + 1; q < 4; ++q) off += fabs(a[p * 4 + q]);
     }
     if (off == 0.0 || off <= 1e-20 * diag) break;
+#pragma unroll 1
     for (int p = 0; p < 3; ++p)
       for (int q = p + 1; q < 4; ++q) {
         const double apq = a[p * 4 + q];
@@ -270,9 +347,10 @@ RTR_HD void jacobi4(double* a, double* V) {
 // sum a_r b_c, all relative to a common origin.  Produces R, t with
 // b ~ R a + t in that origin's frame.  The quaternion of the largest
 // eigenvalue of Horn's symmetric 4x4 N is a proper rotation, which is the
-// reflection-guarded Kabsch/Umeyama solution.
+// reflection-guarded Kabsch/Umeyama solution.  ws: 32 doubles of scratch
+// (N and its eigenvectors; LDS inside the kernels, keeping VGPRs free).
 RTR_HD void rigid_horn(double n, const double* Sa, const double* Sb, const double* Sab, double* R,
-                       double* t) {
+                       double* t, double* ws) {
   double ma[3], mb[3], M[9];
   for (int i = 0; i < 3; ++i) {
     ma[i] = Sa[i] / n;
@@ -284,11 +362,18 @@ RTR_HD void rigid_horn(double n, const double* Sa, const double* Sb, const doubl
   const double Sxx = M[0], Sxy = M[1], Sxz = M[2];
   const double Syx = M[3], Syy = M[4], Syz = M[5];
   const double Szx = M[6], Szy = M[7], Szz = M[8];
-  double N[16] = {Sxx + Syy + Szz, Syz - Szy,        Szx - Sxz,        Sxy - Syx,
-                  Syz - Szy,       Sxx - Syy - Szz,  Sxy + Syx,        Szx + Sxz,
-                  Szx - Sxz,       Sxy + Syx,        -Sxx + Syy - Szz, Syz + Szy,
-                  Sxy - Syx,       Szx + Sxz,        Syz + Szy,        -Sxx - Syy + Szz};
-  double V[16];
+  double* N = ws;
+  double* V = ws + 16;
+  N[0] = Sxx + Syy + Szz;
+  N[1] = N[4] = Syz - Szy;
+  N[2] = N[8] = Szx - Sxz;
+  N[3] = N[12] = Sxy - Syx;
+  N[5] = Sxx - Syy - Szz;
+  N[6] = N[9] = Sxy + Syx;
+  N[7] = N[13] = Szx + Sxz;
+  N[10] = -Sxx + Syy - Szz;
+  N[11] = N[14] = Syz + Szy;
+  N[15] = -Sxx - Syy + Szz;
   jacobi4(N, V);
   int best = 0;
   for (int i = 1; i < 4; ++i)
@@ -315,13 +400,16 @@ RTR_HD int up6(int r, int c) { return r * 6 - r * (r - 1) / 2 + (c - r); }
 
 // 6x6 SPD solve by Cholesky.  A pivot below 1e-15 of the largest diagonal
 // marks a direction the correspondences do not constrain; its unknown is 0.
-RTR_HD int cholesky6(const double* AtA, const double* Atb, double* x) {
-  double L[36], y[6];
+// ws: 36 doubles of scratch for L.
+RTR_HD int cholesky6(const double* AtA, const double* Atb, double* x, double* ws) {
+  double* L = ws;
+  double y[6];
   int active[6];
   double dmax = 0;
   for (int i = 0; i < 36; ++i) L[i] = 0.0;
   for (int i = 0; i < 6; ++i) dmax = fmax(dmax, AtA[up6(i, i)]);
   int ok = 1;
+#pragma unroll 1
   for (int j = 0; j < 6; ++j) {
     double s = AtA[up6(j, j)];
     for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
@@ -371,6 +459,17 @@ RTR_HD void euler_anchor_T(const double* x, const double* anchor, double* T) {
   }
 }
 
+// C = A B for a C that aliases neither input (one column at a time).
+RTR_HD void mat4_mul_to(const double* A, const double* B, double* C) {
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < 4; ++k) s += A[k * 4 + r] * B[c * 4 + k];
+      C[c * 4 + r] = s;
+    }
+}
+
 RTR_HD void mat4_mul(const double* A, const double* B, double* C) {
   double Rm[16];
   for (int c = 0; c < 4; ++c)
@@ -383,9 +482,10 @@ RTR_HD void mat4_mul(const double* A, const double* B, double* C) {
 }
 
 // Incremental transform from reduced FP64 sums (layout in rtr_types.hpp).
-// Returns the number of correspondences; Tinc is column-major.
+// Returns the number of correspondences; Tinc is column-major.  ws: 64
+// doubles of scratch.
 RTR_HD double solve_increment(int mode, const double* v, const float* anchor_f, double* Tinc,
-                              double* mse) {
+                              double* mse, double* ws) {
   const double n = v[SUM_N];
   *mse = n > 0 ? v[SUM_D2] / n : 0.0;
   const double anchor[3] = {(double)anchor_f[0], (double)anchor_f[1], (double)anchor_f[2]};
@@ -394,7 +494,7 @@ RTR_HD double solve_increment(int mode, const double* v, const float* anchor_f, 
   if (n <= 0) return n;
   if (mode == RTR_MODE_POINT_TO_POINT) {
     double R[9], t[3];
-    rigid_horn(n, v + SUM_A, v + SUM_B, v + SUM_AB, R, t);
+    rigid_horn(n, v + SUM_A, v + SUM_B, v + SUM_AB, R, t, ws);
     for (int r = 0; r < 3; ++r) {
       for (int c = 0; c < 3; ++c) Tinc[c * 4 + r] = R[r * 3 + c];
       const double Ra = R[r * 3] * anchor[0] + R[r * 3 + 1] * anchor[1] + R[r * 3 + 2] * anchor[2];
@@ -402,7 +502,7 @@ RTR_HD double solve_increment(int mode, const double* v, const float* anchor_f, 
     }
   } else {
     double x[6];
-    cholesky6(v + SUM_ATA, v + SUM_ATB, x);
+    cholesky6(v + SUM_ATA, v + SUM_ATB, x, ws);
     euler_anchor_T(x, anchor, Tinc);
   }
   return n;

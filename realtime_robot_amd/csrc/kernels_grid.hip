@@ -244,12 +244,22 @@ __device__ void choose_grid(TargetDesc& t, const float* lo, const float* hi, uin
   t.anchor[2] = lo[2] + 0.5f * ez;
 }
 
+// Distance-field parameters: a query whose cell is at Chebyshev distance >= R
+// from every occupied cell is at least (R - 1) * h - margin from every target
+// point; skip_ring is the smallest R for which that exceeds r_max.
+__device__ void choose_dist_cap(TargetDesc& t, float r_max) {
+  dist_caps(r_max, t.h, t.margin, &t.skip_ring, &t.dt_cap);
+}
+
 __global__ void k_tgt_bbox(DevEngine e) {
   TargetDesc& t = e.tgts[blockIdx.x];
   __shared__ float lo[3], hi[3];
   __shared__ uint32_t cnt;
   block_bbox(e.tgt_raw + 3 * (size_t)t.raw_off, t.n, lo, hi, &cnt);
-  if (threadIdx.x == 0) choose_grid(t, lo, hi, cnt);
+  if (threadIdx.x == 0) {
+    choose_grid(t, lo, hi, cnt);
+    choose_dist_cap(t, e.r_max);
+  }
 }
 
 __device__ __forceinline__ bool cell_of(const TargetDesc& t, float x, float y, float z,
@@ -298,6 +308,36 @@ __global__ void k_tgt_scatter(DevEngine e) {
   }
 }
 
+// Separable Chebyshev distance transform over each target's cells, capped at
+// dt_cap + 1: pass x from the occupancy (cell_start differences), then y, then
+// z, each a min over |d| <= cap of max(|d|, previous pass at offset d).
+template <int AXIS>
+__global__ void k_tgt_dist(DevEngine e, const uint8_t* src, uint8_t* dst) {
+  const TargetDesc& t = e.tgts[blockIdx.y];
+  const uint32_t ncell = (uint32_t)t.dx * t.dy * t.dz;
+  const int K = t.dt_cap;
+  const int len = AXIS == 0 ? t.dx : (AXIS == 1 ? t.dy : t.dz);
+  const int stride = AXIS == 0 ? 1 : (AXIS == 1 ? t.dx : t.dx * t.dy);
+  const uint32_t* cs = e.cell_start + t.cell_off;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+    const int pos = AXIS == 0 ? (int)(c % t.dx) : (AXIS == 1 ? (int)((c / t.dx) % t.dy) : (int)(c / ((uint32_t)t.dx * t.dy)));
+    int best = K + 1;
+    const int dlo = -min(K, pos), dhi = min(K, len - 1 - pos);
+    for (int d = dlo; d <= dhi; ++d) {
+      const uint32_t cc = (uint32_t)((int)c + d * stride);
+      int v;
+      if (AXIS == 0)
+        v = cs[cc + 1] > cs[cc] ? 0 : K + 1;
+      else
+        v = src[t.cell_off + cc];
+      const int ad = d < 0 ? -d : d;
+      v = v > ad ? v : ad;
+      best = v < best ? v : best;
+    }
+    dst[t.cell_off + c] = (uint8_t)best;
+  }
+}
+
 static unsigned blocks_for(int max_pts) {
   unsigned b = (unsigned)((max_pts + kBlock - 1) / kBlock);
   if (b < 1) b = 1;
@@ -305,7 +345,7 @@ static unsigned blocks_for(int max_pts) {
   return b;
 }
 
-void launch_target_grid(const DevEngine& e, int max_tgt_pts, hipStream_t s) {
+void launch_target_grid(const DevEngine& e, int max_tgt_pts, int max_cells, hipStream_t s) {
   if (e.n_tgt == 0) return;
   launch_fill_u32(e.cell_count, 0u, e.n_cells_total, s);
   hipLaunchKernelGGL(k_tgt_bbox, dim3(e.n_tgt), dim3(kBlock), 0, s, e);
@@ -313,6 +353,12 @@ void launch_target_grid(const DevEngine& e, int max_tgt_pts, hipStream_t s) {
   hipLaunchKernelGGL(k_tgt_count, g, dim3(kBlock), 0, s, e);
   launch_exclusive_scan(e.cell_count, e.cell_start, e.n_cells_total, e.scan_tmp, s);
   hipLaunchKernelGGL(k_tgt_scatter, g, dim3(kBlock), 0, s, e);
+  if (e.cell_dist) {
+    dim3 gd(blocks_for(max_cells), e.n_tgt);
+    hipLaunchKernelGGL(k_tgt_dist<0>, gd, dim3(kBlock), 0, s, e, (const uint8_t*)nullptr, e.cell_dist);
+    hipLaunchKernelGGL(k_tgt_dist<1>, gd, dim3(kBlock), 0, s, e, (const uint8_t*)e.cell_dist, e.cell_dist_tmp);
+    hipLaunchKernelGGL(k_tgt_dist<2>, gd, dim3(kBlock), 0, s, e, (const uint8_t*)e.cell_dist_tmp, e.cell_dist);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -381,7 +427,94 @@ __global__ void k_src_scatter(DevEngine e) {
   }
 }
 
-void launch_source_sort(const DevEngine& e, int max_src_pts, hipStream_t s) {
+// The scatter fills each bin in atomic order.  One wave per bin then orders
+// the bin's points by (Morton code of their position inside the bin at 16
+// sub-cells per axis, original index): the layout, and with it the FP32
+// summation order of the moments, is identical from run to run, and
+// consecutive points stay spatially compact below the bin size.  Ranks are
+// found by counting (O(n^2 / 64) per bin, n ~ tens); a bin larger than the
+// LDS tile is ordered by original index alone by one lane.
+constexpr int kBinCap = 256;
+
+__device__ __forceinline__ uint32_t spread3_4(uint32_t v) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) r |= ((v >> b) & 1u) << (3 * b);
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_src_order_bins(DevEngine e, uint32_t total) {
+  __shared__ uint64_t s_key[kWaves][kBinCap];
+  __shared__ float4 s_pt[kWaves][kBinCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t b = blockIdx.x * kWaves + w;
+  if (b >= e.n_bins_total) return;
+  const uint32_t beg = e.bin_start[b];
+  const uint32_t end = (b + 1 < e.n_bins_total) ? e.bin_start[b + 1] : total;
+  const uint32_t n = end - beg;
+  if (n <= 1) return;
+  if (n > (uint32_t)kBinCap) {
+    if (lane == 0) {
+      for (uint32_t i = beg + 1; i < end; ++i) {
+        const uint32_t key = e.perm[i];
+        const float x = e.sx[i], y = e.sy[i], z = e.sz[i];
+        uint32_t j = i;
+        while (j > beg && e.perm[j - 1] > key) {
+          e.perm[j] = e.perm[j - 1];
+          e.sx[j] = e.sx[j - 1];
+          e.sy[j] = e.sy[j - 1];
+          e.sz[j] = e.sz[j - 1];
+          --j;
+        }
+        e.perm[j] = key;
+        e.sx[j] = x;
+        e.sy[j] = y;
+        e.sz[j] = z;
+      }
+    }
+    return;
+  }
+  // the source owning bin b (bins of consecutive sources are consecutive)
+  int lo = 0, hi = e.n_src - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (e.srcs[mid].bin_off <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SourceDesc sd = e.srcs[lo];
+  const int P = sd.dx;
+  for (uint32_t i = lane; i < n; i += 64) {
+    const float x = e.sx[beg + i], y = e.sy[beg + i], z = e.sz[beg + i];
+    const uint32_t id = e.perm[beg + i];
+    uint32_t fine = 0xfffu;
+    if (isfinite(x) && isfinite(y) && isfinite(z)) {
+      const float f[3] = {(x - sd.ox) * sd.inv_h, (y - sd.oy) * sd.inv_h, (z - sd.oz) * sd.inv_h};
+      uint32_t sub[3];
+      for (int a = 0; a < 3; ++a) {
+        const int c = clampi((int)floorf(f[a]), 0, P - 1);
+        sub[a] = (uint32_t)clampi((int)floorf((f[a] - (float)c) * 16.0f), 0, 15);
+      }
+      fine = spread3_4(sub[0]) | (spread3_4(sub[1]) << 1) | (spread3_4(sub[2]) << 2);
+    }
+    s_key[w][i] = ((uint64_t)fine << 32) | id;
+    s_pt[w][i] = make_float4(x, y, z, __int_as_float((int)id));
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t i = lane; i < n; i += 64) {
+    const uint64_t k = s_key[w][i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < n; ++j) r += (s_key[w][j] < k) ? 1u : 0u;
+    const float4 pt = s_pt[w][i];
+    e.sx[beg + r] = pt.x;
+    e.sy[beg + r] = pt.y;
+    e.sz[beg + r] = pt.z;
+    e.perm[beg + r] = (uint32_t)__float_as_int(pt.w);
+  }
+}
+
+void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s) {
   if (e.n_src == 0) return;
   launch_fill_u32(e.bin_count, 0u, e.n_bins_total, s);
   hipLaunchKernelGGL(k_src_bbox, dim3(e.n_src), dim3(kBlock), 0, s, e);
@@ -389,6 +522,8 @@ void launch_source_sort(const DevEngine& e, int max_src_pts, hipStream_t s) {
   hipLaunchKernelGGL(k_src_count, g, dim3(kBlock), 0, s, e);
   launch_exclusive_scan(e.bin_count, e.bin_start, e.n_bins_total, e.scan_tmp, s);
   hipLaunchKernelGGL(k_src_scatter, g, dim3(kBlock), 0, s, e);
+  hipLaunchKernelGGL(k_src_order_bins, dim3((e.n_bins_total + kWaves - 1) / kWaves), dim3(kBlock), 0, s,
+                     e, src_total);
 }
 
 }  // namespace rtr

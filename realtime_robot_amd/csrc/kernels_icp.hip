@@ -87,17 +87,99 @@ __global__ void k_icp_init(DevEngine e) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged candidate tiles
+//
+// Every wave takes 64 consecutive (spatially sorted) source points.  The
+// union of their 3x3x3 cell blocks is a small box of cells; its x-rows are
+// contiguous ranges of the cell-sorted target array, so the wave copies them
+// into LDS with coalesced 16-byte loads, together with the LDS offset of every
+// cell of the box.  Each lane then scans its own nine rows from LDS.  Lanes
+// whose answer is not proven inside the block continue over outer rings
+// straight from global memory; a wave whose box does not fit the tile walks
+// its blocks from global memory too.  Either way the candidate set, and so
+// the result, is exactly that of the plain ring walk.
+// ---------------------------------------------------------------------------
+constexpr int kTilePts = 512;    // staged points per wave (8 KB)
+constexpr int kTileCells = 640;  // staged cell offsets per wave
+constexpr int kTileRows = 64;    // staged x-rows per wave (one per lane)
+
+struct WaveTile {
+  float4 pts[kTilePts];
+  uint16_t cell[kTileCells];     // LDS offset of each box cell, plus each row's end
+  uint32_t row_a[kTileRows];     // first cell-sorted position of each row
+  uint32_t row_off[kTileRows];   // LDS offset of each row
+};
+
 template <int NS>
 struct IcpShared {
+  WaveTile tile[kWaves];
+  float red[kWaves][NS][2];
+  double dred[kBlock / NSUM_MAX][NSUM_MAX];
+  double ws[64];
+  double Tinc[16];
+  double Tn[16];
   float m[12];
   float anchor[3];
   int item;
   int run;
   int stop;
   int last;
-  float red[kWaves][NS][2];
-  double dred[kBlock / NSUM_MAX][NSUM_MAX];
 };
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
+  const int lane = threadIdx.x & 63;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  *total = __shfl(inc, 63, 64);
+  return inc - v;
+}
+
+// Nearest-neighbour state: packed (d2, index) key and where the winner lives
+// (an LDS tile index, or a cell-sorted slot with the top bit set).
+struct GlobalVis {
+  const float4* pts;
+  float qx, qy, qz;
+  uint64_t key;
+  uint32_t where;
+  __device__ __forceinline__ void scan(uint32_t b, uint32_t e) {
+    for (uint32_t j = b; j < e; ++j) {
+      const float4 t = pts[j];
+      const uint64_t k = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), __float_as_int(t.w));
+      if (k < key) {
+        key = k;
+        where = j | 0x80000000u;
+      }
+    }
+  }
+  __device__ __forceinline__ float bound() const { return key == ~0ull ? 3.4e38f : key_d2(key); }
+};
+
+__device__ __forceinline__ int tile_row_of(const WaveTile& tl, int nrows, uint32_t t) {
+  int lo = 0, hi = nrows - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tl.row_off[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
 
 template <int MODE, int NS>
 __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
@@ -120,8 +202,9 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
   }
   __syncthreads();
   if (tid == 0) {
-    double vals[NSUM_MAX];
-    for (int k = 0; k < NSUM_MAX; ++k) vals[k] = sh.dred[0][k];
+    // all FP64 work of the solve reads and writes LDS (sh.dred, sh.ws, sh.Tinc),
+    // so it adds no VGPR pressure to the search loop of this kernel
+    const double* vals = sh.dred[0];
     ProblemState& S = e.st[p];
     const double n = vals[SUM_N];
     bool fin = false;
@@ -131,14 +214,14 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
       S.mse = n > 0 ? vals[SUM_D2] / n : 0.0;
       fin = true;
     } else {
-      double Tinc[16], mse, Tn[16];
-      solve_increment(MODE, vals, td.anchor, Tinc, &mse);
-      mat4_mul(Tinc, S.T, Tn);
-      for (int k = 0; k < 16; ++k) S.T[k] = Tn[k];
+      double mse;
+      solve_increment(MODE, vals, td.anchor, sh.Tinc, &mse, sh.ws);
+      mat4_mul_to(sh.Tinc, S.T, sh.Tn);
+      for (int k = 0; k < 16; ++k) S.T[k] = sh.Tn[k];
       S.iter += 1;
       S.mse = mse;
       double prev = S.prev_mse;
-      const int stt = convergence(e.prm, S.iter, Tinc, mse, &prev);
+      const int stt = convergence(e.prm, S.iter, sh.Tinc, mse, &prev);
       S.prev_mse = prev;
       S.state = stt;
       fin = (stt != RTR_NOT_CONVERGED);
@@ -153,12 +236,14 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
 template <int MODE, int NS>
 __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS>& sh) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  WaveTile& tl = sh.tile[w];
   const uint2 itm = e.items[it];
   const uint32_t p = itm.x;
   const Problem pr = e.probs[p];
   const SourceDesc sd = e.srcs[pr.src];
   const TargetDesc td = e.tgts[pr.tgt];
   const GridView g = grid_of(e, td);
+  const float r2f = e.prm.r2f;
   float m[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) m[k] = sh.m[k];
@@ -167,25 +252,151 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS>
 #pragma unroll
   for (int k = 0; k < NS; ++k) acc_s[k] = acc_c[k] = 0.f;
   const uint32_t end = min(itm.y + (uint32_t)e.item_pts, sd.n);
-  for (uint32_t j = itm.y + tid; j < end; j += kBlock) {
-    const uint32_t sj = sd.off + j;
-    float qx, qy, qz;
-    apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
-    const NNBest b = nn_search(g, qx, qy, qz, e.prm.r2f);
-    if (b.key != ~0ull) {
-      float nx = 0.f, ny = 0.f, nz = 0.f;
-      if (MODE == RTR_MODE_POINT_TO_PLANE) {
-        const float4 nn = e.tgt_nrm_sorted[b.slot];
-        nx = nn.x;
-        ny = nn.y;
-        nz = nn.z;
-      }
-      float terms[NSUM_MAX];
-      point_terms(MODE, qx - ax, qy - ay, qz - az, b.tx - ax, b.ty - ay, b.tz - az, nx, ny, nz,
-                  key_d2(b.key), terms);
-#pragma unroll
-      for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+  for (uint32_t base = itm.y + (uint32_t)w * 64u; base < end; base += kBlock) {
+    const uint32_t j = base + lane;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    int cx = 0, cy = 0, cz = 0;
+    bool act = false;
+    if (j < end) {
+      const uint32_t sj = sd.off + j;
+      apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+      act = query_cell(g, qx, qy, qz, cx, cy, cz);
     }
+    // distance field: cells within D - 1 rings of the query's cell are empty
+    bool blk = act;  // the lane walks its 3x3x3 block
+    int k0 = 1;      // box radius searched (or known empty) before the outer walk
+    if (act && e.cell_dist) {
+      const int D = e.cell_dist[td.cell_off + ((uint32_t)cz * g.dy + cy) * g.dx + cx];
+      if (D >= td.skip_ring) {
+        act = false;  // every target point is farther than r_max
+        blk = false;
+      } else if (D >= 2) {
+        blk = false;
+        k0 = D - 1;
+      }
+    }
+    GlobalVis vis;
+    vis.pts = g.pts;
+    vis.qx = qx;
+    vis.qy = qy;
+    vis.qz = qz;
+    vis.key = ~0ull;
+    vis.where = 0u;
+    bool staged = false;
+    int X0 = 0, Y0 = 0, Z0 = 0, NY = 1, W1 = 1, nrows = 0;
+    if (__ballot(blk) != 0ull) {
+      X0 = max(wave_min_i(blk ? cx : 0x7fffffff) - 1, 0);
+      Y0 = max(wave_min_i(blk ? cy : 0x7fffffff) - 1, 0);
+      Z0 = max(wave_min_i(blk ? cz : 0x7fffffff) - 1, 0);
+      const int X1 = min(wave_max_i(blk ? cx : -1) + 1, g.dx - 1);
+      const int Y1 = min(wave_max_i(blk ? cy : -1) + 1, g.dy - 1);
+      const int Z1 = min(wave_max_i(blk ? cz : -1) + 1, g.dz - 1);
+      NY = Y1 - Y0 + 1;
+      W1 = X1 - X0 + 2;
+      nrows = NY * (Z1 - Z0 + 1);
+      if (nrows <= kTileRows && nrows * W1 <= kTileCells) {
+        uint32_t a = 0u, len = 0u;
+        if (lane < nrows) {
+          const int y = Y0 + lane % NY, z = Z0 + lane / NY;
+          const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+          a = g.cell_start[row + X0];
+          len = g.cell_start[row + X1 + 1] - a;
+        }
+        uint32_t total;
+        const uint32_t off = wave_excl_scan(len, &total);
+        if (total <= (uint32_t)kTilePts) {
+          staged = true;
+          if (lane < nrows) {
+            tl.row_a[lane] = a;
+            tl.row_off[lane] = off;
+          }
+          __builtin_amdgcn_wave_barrier();
+          for (int t = lane; t < nrows * W1; t += 64) {
+            const int r = t / W1, i = t - r * W1;
+            const int y = Y0 + r % NY, z = Z0 + r / NY;
+            const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+            tl.cell[t] = (uint16_t)(g.cell_start[row + X0 + i] - tl.row_a[r] + tl.row_off[r]);
+          }
+          for (uint32_t t = lane; t < total; t += 64) {
+            const int r = tile_row_of(tl, nrows, t);
+            tl.pts[t] = g.pts[tl.row_a[r] + (t - tl.row_off[r])];
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    }
+    if (act) {
+      if (!blk) {
+        // block known empty: only the outer rings
+      } else if (staged) {
+        const int xlo = max(cx - 1, 0) - X0;
+        const int xhi = min(cx + 1, g.dx - 1) - X0 + 1;
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int z = cz + dz;
+          if (z < 0 || z >= g.dz) continue;
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= g.dy) continue;
+            const int r = (z - Z0) * NY + (y - Y0);
+            const uint32_t b0 = tl.cell[r * W1 + xlo], b1 = tl.cell[r * W1 + xhi];
+            uint32_t t = b0;
+            for (; t + 2 <= b1; t += 2) {
+              const float4 c0 = tl.pts[t];
+              const float4 c1 = tl.pts[t + 1];
+              const uint64_t k0v = nn_key(d2_f32(qx, qy, qz, c0.x, c0.y, c0.z), __float_as_int(c0.w));
+              const uint64_t k1v = nn_key(d2_f32(qx, qy, qz, c1.x, c1.y, c1.z), __float_as_int(c1.w));
+              if (k0v < vis.key) {
+                vis.key = k0v;
+                vis.where = t;
+              }
+              if (k1v < vis.key) {
+                vis.key = k1v;
+                vis.where = t + 1;
+              }
+            }
+            if (t < b1) {
+              const float4 c = tl.pts[t];
+              const uint64_t kv = nn_key(d2_f32(qx, qy, qz, c.x, c.y, c.z), __float_as_int(c.w));
+              if (kv < vis.key) {
+                vis.key = kv;
+                vis.where = t;
+              }
+            }
+          }
+        }
+      } else {
+        walk_block(g, cx, cy, cz, vis);
+      }
+      walk_outer(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
+      if (vis.key != ~0ull && key_d2(vis.key) <= r2f) {
+        float4 c;
+        uint32_t slot = 0u;
+        if (vis.where & 0x80000000u) {
+          slot = vis.where & 0x7fffffffu;
+          c = g.pts[slot];
+        } else {
+          c = tl.pts[vis.where];
+          if (MODE == RTR_MODE_POINT_TO_PLANE) {
+            const int r = tile_row_of(tl, nrows, vis.where);
+            slot = tl.row_a[r] + (vis.where - tl.row_off[r]);
+          }
+        }
+        float nx = 0.f, ny = 0.f, nz = 0.f;
+        if (MODE == RTR_MODE_POINT_TO_PLANE) {
+          const float4 nn = e.tgt_nrm_sorted[slot];
+          nx = nn.x;
+          ny = nn.y;
+          nz = nn.z;
+        }
+        float terms[NSUM_MAX];
+        point_terms(MODE, qx - ax, qy - ay, qz - az, c.x - ax, c.y - ay, c.z - az, nx, ny, nz,
+                    key_d2(vis.key), terms);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+      }
+    }
+    // the next round rewrites this wave's tile
+    __builtin_amdgcn_wave_barrier();
   }
   // wave reduction with compensation, then across the four waves
 #pragma unroll

@@ -55,6 +55,8 @@ struct TargetDesc {
   float h, inv_h, margin;
   float anchor[3];       // bbox centre, origin of the accumulated moments
   float h_req;           // requested minimum cell edge
+  int dt_cap;            // distance-field cap: values > dt_cap are stored as dt_cap + 1
+  int skip_ring;         // a query whose cell has distance >= skip_ring has no match
 };
 
 // Per source cloud (device).
