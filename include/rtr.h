@@ -164,9 +164,10 @@ int rtr_batch_set_params(rtr_batch* b, const rtr_params* p);
 /* Launch/layout tuning (the autotuner's knobs): requested cell edge as a
  * fraction of the max correspondence distance, cell budget per target point,
  * source points per work item (multiple of 256), persistent workgroups per CU
- * (0 = occupancy limit).  Values <= 0 keep the current setting. */
+ * (0 = occupancy limit), LDS candidate tiles on (1) or off (0).  Values < 0
+ * (and <= 0 for the first three) keep the current setting. */
 int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point, int item_pts,
-                         int blocks_per_cu);
+                         int blocks_per_cu, int use_tile);
 /* Enable/disable hipGraph replay of align (default on). */
 int rtr_batch_set_graph(rtr_batch* b, int on);
 int rtr_batch_set_profiling(rtr_batch* b, int on);

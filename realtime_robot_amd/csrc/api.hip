@@ -384,7 +384,7 @@ int rtr_batch_set_params(rtr_batch* b, const rtr_params* p) {
 }
 
 int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point, int item_pts,
-                         int blocks_per_cu) {
+                         int blocks_per_cu, int use_tile) {
   RTR_GUARD_BEGIN
   if (!b) return RTR_ERR_INVALID_ARG;
   if (item_pts > 0 && (item_pts % 256 != 0 || item_pts > (1 << 20))) {
@@ -396,6 +396,7 @@ int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point,
   if (cells_per_point > 0) t.cells_per_point = cells_per_point;
   if (item_pts > 0) t.item_pts = item_pts;
   if (blocks_per_cu >= 0) t.blocks_per_cu = blocks_per_cu;
+  if (use_tile >= 0) t.use_tile = use_tile != 0;
   return b->eng.set_params(b->eng.params());
   RTR_GUARD_END
 }

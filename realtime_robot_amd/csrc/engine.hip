@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 
 namespace rtr {
 
@@ -263,19 +264,44 @@ int Engine::upload_problems() {
     RTR_HIP(hipMemcpy(tgt_desc_.p, tgts_.data(), tgts_.size() * sizeof(TargetDesc), hipMemcpyHostToDevice));
   if (!srcs_.empty())
     RTR_HIP(hipMemcpy(src_desc_.p, srcs_.data(), srcs_.size() * sizeof(SourceDesc), hipMemcpyHostToDevice));
-  // items
+  // Items, grouped into one queue per XCD.  All problems of one target go to
+  // the same XCD (its L2 then holds that target's grid); targets are placed
+  // longest-first on the least-loaded XCD.
   items_.clear();
   pt_total_ = 0;
   const uint32_t ip = (uint32_t)tune_.item_pts;
+  std::vector<uint64_t> tload(tgts_.size(), 0);
+  for (const Problem& pr : probs_) tload[pr.tgt] += (srcs_[pr.src].n + ip - 1) / ip;
+  std::vector<int> order(tgts_.size());
+  for (size_t j = 0; j < order.size(); ++j) order[j] = (int)j;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return tload[a] > tload[b] || (tload[a] == tload[b] && a < b); });
+  std::vector<int> txcd(tgts_.size(), 0);
+  uint64_t xload[kXcds] = {0};
+  for (int j : order) {
+    int best = 0;
+    for (int x = 1; x < kXcds; ++x)
+      if (xload[x] < xload[best]) best = x;
+    txcd[j] = best;
+    xload[best] += tload[j];
+  }
   for (size_t p = 0; p < probs_.size(); ++p) {
     Problem& pr = probs_[p];
     const uint32_t n = srcs_[pr.src].n;
-    pr.item_off = (uint32_t)items_.size();
     pr.n_items = (n + ip - 1) / ip;
     pr.pt_off = (uint32_t)pt_total_;
     pt_total_ += n;
-    for (uint32_t b = 0; b < n; b += ip) items_.push_back(make_uint2((uint32_t)p, b));
   }
+  for (int x = 0; x < kXcds; ++x) {
+    xcd_off_[x] = (uint32_t)items_.size();
+    for (size_t p = 0; p < probs_.size(); ++p) {
+      Problem& pr = probs_[p];
+      if (txcd[pr.tgt] != x) continue;
+      pr.item_off = (uint32_t)items_.size();
+      const uint32_t n = srcs_[pr.src].n;
+      for (uint32_t b = 0; b < n; b += ip) items_.push_back(make_uint2((uint32_t)p, b));
+    }
+  }
+  xcd_off_[kXcds] = (uint32_t)items_.size();
   const size_t np = probs_.size();
   RTR_TRY(prob_desc_.ensure(np * sizeof(Problem)));
   RTR_TRY(state_.ensure(np * sizeof(ProblemState)));
@@ -283,7 +309,7 @@ int Engine::upload_problems() {
   RTR_TRY(done_.ensure(np * 4));
   RTR_TRY(items_buf_.ensure(items_.size() * sizeof(uint2)));
   RTR_TRY(partials_.ensure(items_.size() * NSUM_MAX * 2 * sizeof(float)));
-  RTR_TRY(counters_.ensure((size_t)(prm_.max_iterations + 8) * 4));
+  RTR_TRY(counters_.ensure(counter_words() * 4));
   RTR_TRY(out_T_.ensure(16 * sizeof(double)));
   if (np) RTR_HIP(hipMemcpy(prob_desc_.p, probs_.data(), np * sizeof(Problem), hipMemcpyHostToDevice));
   if (!items_.empty())
@@ -327,11 +353,13 @@ DevEngine Engine::dev_view() const {
   e.items = items_buf_.as<uint2>();
   e.n_items = (int)items_.size();
   e.item_pts = tune_.item_pts;
+  e.use_tile = tune_.use_tile;
   e.partials = partials_.as<float>();
   uint32_t* ctr = counters_.as<uint32_t>();
   e.problems_done = ctr;
   e.error = reinterpret_cast<int*>(ctr + 1);
   e.claim = ctr + 8;
+  for (int x = 0; x <= kXcds; ++x) e.xcd_off[x] = xcd_off_[x];
   e.done = done_.as<uint32_t>();
   e.scan_tmp = scan_tmp_.as<uint32_t>();
   e.prm.max_iter = prm_.max_iterations;
@@ -355,7 +383,7 @@ DevEngine Engine::dev_view() const {
 }
 
 int Engine::grid_blocks() const {
-  int bpc = icp_blocks_per_cu(prm_.mode);
+  int bpc = icp_blocks_per_cu(prm_.mode, tune_.use_tile);
   if (tune_.blocks_per_cu > 0 && tune_.blocks_per_cu < bpc) bpc = tune_.blocks_per_cu;
   long long b = (long long)cu_count_ * bpc;
   if (b > (long long)items_.size()) b = (long long)items_.size();
@@ -373,7 +401,7 @@ int Engine::enqueue_build() {
 
 int Engine::enqueue_align_body() {
   DevEngine e = dev_view();
-  launch_fill_u32(counters_.as<uint32_t>(), 0u, (size_t)prm_.max_iterations + 8, stream_);
+  launch_fill_u32(counters_.as<uint32_t>(), 0u, counter_words(), stream_);
   launch_icp(e, grid_blocks(), stream_);
   RTR_HIP(hipGetLastError());
   return RTR_OK;

@@ -41,6 +41,7 @@ struct TuneParams {
   float cells_per_point = 2.0f;   // grid cell budget per target point
   int item_pts = 1024;            // source points per work item
   int blocks_per_cu = 0;          // persistent workgroups per CU (0 = occupancy limit)
+  int use_tile = 1;               // LDS candidate tiles on/off
 };
 
 class Engine {
@@ -106,6 +107,9 @@ class Engine {
   int max_tgt_ = 0, max_src_ = 0;
   uint32_t n_cells_ = 0, n_bins_ = 0;
   int max_cells_ = 0;
+  uint32_t xcd_off_[kXcds + 1] = {0};
+  // [0] problems done, [1] error, [8 + pass * kXcds + x] claim counters
+  size_t counter_words() const { return 8 + (size_t)prm_.max_iterations * kXcds; }
 
   // device arena
   DevBuf tgt_raw_, tgt_nrm_raw_, tgt_sorted_, tgt_nrm_sorted_, cell_count_, cell_start_, cell_id_;

@@ -10,6 +10,7 @@ namespace rtr {
 
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin by blockIdx
 
 struct DevEngine {
   // targets: raw packed xyz, descriptors, cell-sorted points and grids
@@ -46,8 +47,10 @@ struct DevEngine {
   const uint2* items;        // (problem, first sorted source point)
   int n_items;
   int item_pts;              // points per item
+  int use_tile;              // stage candidate tiles in LDS (else global gathers only)
   float* partials;           // n_items * NSUM_MAX * 2 floats
-  uint32_t* claim;           // one claim counter per pass
+  uint32_t* claim;           // claim counter per (pass, XCD queue)
+  uint32_t xcd_off[kXcds + 1];  // item range of each XCD queue
   uint32_t* done;            // per problem: items finished so far
   uint32_t* problems_done;
   int* error;                // nonzero: a bounded wait expired
@@ -63,7 +66,7 @@ size_t scan_tmp_words(uint32_t n);
 void launch_target_grid(const DevEngine& e, int max_tgt_pts, int max_cells, hipStream_t s);
 void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s);
 void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s);
-int icp_blocks_per_cu(int mode);
+int icp_blocks_per_cu(int mode, int use_tile);
 // correspondences / fitness / transformed output for problem set; T per
 // problem from `T` (n_prob*16 column-major, device) or from the state when null.
 void launch_correspondences(const DevEngine& e, int prob, const double* T, float r2f, int32_t* idx,

@@ -198,6 +198,76 @@ RTR_HD void walk_shell(const GridView& g, int cx, int cy, int cz, int k, V& vis)
   }
 }
 
+// Thick shell: every cell within Chebyshev distance kn of (cx, cy, cz) that is
+// not within distance k (already searched).  Rows crossing the inner box
+// contribute their two outer x-segments, the others their full x-range.  Row
+// bounds are read four rows at a time so their loads overlap.
+template <class V>
+RTR_HD void walk_thick_shell(const GridView& g, int cx, int cy, int cz, int k, int kn, V& vis) {
+  const int x0 = cx - kn > 0 ? cx - kn : 0;
+  const int x1 = cx + kn < g.dx - 1 ? cx + kn : g.dx - 1;
+  const int ix0 = cx - k > 0 ? cx - k : 0;
+  const int ix1 = cx + k < g.dx - 1 ? cx + k : g.dx - 1;
+  const int y0 = cy - kn > 0 ? cy - kn : 0;
+  const int y1 = cy + kn < g.dy - 1 ? cy + kn : g.dy - 1;
+  const int z0 = cz - kn > 0 ? cz - kn : 0;
+  const int z1 = cz + kn < g.dz - 1 ? cz + kn : g.dz - 1;
+  const int ny = y1 - y0 + 1;
+  const int nrows = ny * (z1 - z0 + 1);
+  for (int r0 = 0; r0 < nrows; r0 += 4) {
+    uint32_t b[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      b[u][0] = b[u][1] = b[u][2] = b[u][3] = 0u;
+      const int r = r0 + u;
+      if (r < nrows) {
+        const int y = y0 + r % ny, z = z0 + r / ny;
+        const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+        const int ay = y > cy ? y - cy : cy - y, az = z > cz ? z - cz : cz - z;
+        if (ay > k || az > k) {
+          b[u][0] = g.cell_start[row + x0];
+          b[u][1] = g.cell_start[row + x1 + 1];
+        } else {
+          if (x0 < ix0) {
+            b[u][0] = g.cell_start[row + x0];
+            b[u][1] = g.cell_start[row + ix0];
+          }
+          if (ix1 < x1) {
+            b[u][2] = g.cell_start[row + ix1 + 1];
+            b[u][3] = g.cell_start[row + x1 + 1];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      vis.scan(b[u][0], b[u][1]);
+      vis.scan(b[u][2], b[u][3]);
+    }
+  }
+}
+
+// Search beyond the box of radius k0 (searched, or known empty): while the
+// stopping test fails, scan the next shell (row bounds batched).  Growing one
+// ring at a time stops as early as the plain ring walk; while nothing at all
+// has been found and the acceptance radius is unbounded, the radius doubles.
+template <class V>
+RTR_HD void walk_far(const GridView& g, float qx, float qy, float qz, int cx, int cy, int cz,
+                     float r2f, V& vis, int k0) {
+  const int kmax = (g.dx > g.dy ? (g.dx > g.dz ? g.dx : g.dz) : (g.dy > g.dz ? g.dy : g.dz));
+  int k = k0;
+  while (k < kmax && !ring_done(g, qx, qy, qz, cx, cy, cz, k, r2f, vis)) {
+    if (vis.bound() > 1.0e30f && r2f > 1.0e30f) {
+      const int kn = 2 * k + 1 < kmax ? 2 * k + 1 : kmax;
+      walk_thick_shell(g, cx, cy, cz, k, kn, vis);
+      k = kn;
+    } else {
+      walk_shell(g, cx, cy, cz, k + 1, vis);
+      k = k + 1;
+    }
+  }
+}
+
 // Shells k0 + 1, k0 + 2, ... once the box of radius k0 has been searched
 // (or is known to be empty), each preceded by the stopping test.
 template <class V>
@@ -242,7 +312,7 @@ RTR_HD void ring_walk_dt(const GridView& g, const uint8_t* dist, int skip_ring, 
     k0 = D - 1;
   else
     walk_block(g, cx, cy, cz, vis);
-  walk_outer(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
+  walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
 }
 
 // Exact search by rings of cells at growing Chebyshev distance around the
@@ -252,7 +322,7 @@ RTR_HD void ring_walk(const GridView& g, float qx, float qy, float qz, float r2f
   int cx, cy, cz;
   if (!query_cell(g, qx, qy, qz, cx, cy, cz)) return;
   walk_block(g, cx, cy, cz, vis);
-  walk_outer(g, qx, qy, qz, cx, cy, cz, r2f, vis);
+  walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, 1);
 }
 
 RTR_HD float key_d2(uint64_t key) {

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <type_traits>
 
 #include "engine_dev.hpp"
 #include "geom.hpp"
@@ -111,9 +112,18 @@ struct WaveTile {
   uint32_t row_off[kTileRows];   // LDS offset of each row
 };
 
-template <int NS>
+// Placeholder with the same members for the kernel variant without tiles
+// (its LDS then only holds the reductions, which raises occupancy).
+struct WaveTileNone {
+  float4 pts[1];
+  uint16_t cell[2];
+  uint32_t row_a[1];
+  uint32_t row_off[1];
+};
+
+template <int NS, class TileT>
 struct IcpShared {
-  WaveTile tile[kWaves];
+  TileT tile[kWaves];
   float red[kWaves][NS][2];
   double dred[kBlock / NSUM_MAX][NSUM_MAX];
   double ws[64];
@@ -156,20 +166,52 @@ struct GlobalVis {
   float qx, qy, qz;
   uint64_t key;
   uint32_t where;
-  __device__ __forceinline__ void scan(uint32_t b, uint32_t e) {
-    for (uint32_t j = b; j < e; ++j) {
-      const float4 t = pts[j];
-      const uint64_t k = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), __float_as_int(t.w));
-      if (k < key) {
-        key = k;
-        where = j | 0x80000000u;
-      }
+  __device__ __forceinline__ void take(const float4& t, uint32_t j) {
+    const uint64_t k = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), __float_as_int(t.w));
+    if (k < key) {
+      key = k;
+      where = j | 0x80000000u;
     }
+  }
+  // four independent loads in flight per step (gathers are latency-bound)
+  __device__ __forceinline__ void scan(uint32_t b, uint32_t e) {
+    uint32_t j = b;
+    for (; j + 4 <= e; j += 4) {
+      const float4 t0 = pts[j], t1 = pts[j + 1], t2 = pts[j + 2], t3 = pts[j + 3];
+      take(t0, j);
+      take(t1, j + 1);
+      take(t2, j + 2);
+      take(t3, j + 3);
+    }
+    for (; j < e; ++j) take(pts[j], j);
   }
   __device__ __forceinline__ float bound() const { return key == ~0ull ? 3.4e38f : key_d2(key); }
 };
 
-__device__ __forceinline__ int tile_row_of(const WaveTile& tl, int nrows, uint32_t t) {
+// The 3x3x3 block straight from global memory: the 18 row bounds are loaded
+// together before any row is scanned.
+__device__ __forceinline__ void walk_block_global(const GridView& g, int cx, int cy, int cz,
+                                                  GlobalVis& vis) {
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
+  uint32_t rb[9], re[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int z = cz + k / 3 - 1, y = cy + k % 3 - 1;
+    if (z >= 0 && z < g.dz && y >= 0 && y < g.dy) {
+      const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+      rb[k] = g.cell_start[row + x0];
+      re[k] = g.cell_start[row + x1 + 1];
+    } else {
+      rb[k] = re[k] = 0u;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vis.scan(rb[k], re[k]);
+}
+
+template <class TileT>
+__device__ __forceinline__ int tile_row_of(const TileT& tl, int nrows, uint32_t t) {
   int lo = 0, hi = nrows - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -181,9 +223,9 @@ __device__ __forceinline__ int tile_row_of(const WaveTile& tl, int nrows, uint32
   return lo;
 }
 
-template <int MODE, int NS>
+template <int MODE, int NS, class TileT>
 __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
-                              const TargetDesc& td, int pass, IcpShared<NS>& sh) {
+                              const TargetDesc& td, int pass, IcpShared<NS, TileT>& sh) {
   const int tid = threadIdx.x;
   const int v = tid % NSUM_MAX, grp = tid / NSUM_MAX;
   double acc = 0.0;
@@ -233,10 +275,10 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
   }
 }
 
-template <int MODE, int NS>
-__device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS>& sh) {
+template <int MODE, int NS, bool TILE, class TileT>
+__device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS, TileT>& sh) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  WaveTile& tl = sh.tile[w];
+  TileT& tl = sh.tile[w];
   const uint2 itm = e.items[it];
   const uint32_t p = itm.x;
   const Problem pr = e.probs[p];
@@ -294,7 +336,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS>
       NY = Y1 - Y0 + 1;
       W1 = X1 - X0 + 2;
       nrows = NY * (Z1 - Z0 + 1);
-      if (nrows <= kTileRows && nrows * W1 <= kTileCells) {
+      if (TILE && e.use_tile && nrows <= kTileRows && nrows * W1 <= kTileCells) {
         uint32_t a = 0u, len = 0u;
         if (lane < nrows) {
           const int y = Y0 + lane % NY, z = Z0 + lane / NY;
@@ -365,9 +407,9 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS>
           }
         }
       } else {
-        walk_block(g, cx, cy, cz, vis);
+        walk_block_global(g, cx, cy, cz, vis);
       }
-      walk_outer(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
+      walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
       if (vis.key != ~0ull && key_d2(vis.key) <= r2f) {
         float4 c;
         uint32_t slot = 0u;
@@ -430,14 +472,15 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS>
   __syncthreads();
   if (sh.last) {
     __threadfence();
-    solve_problem<MODE, NS>(e, p, pr, td, pass, sh);
+    solve_problem<MODE, NS, TileT>(e, p, pr, td, pass, sh);
   }
 }
 
-template <int MODE>
+template <int MODE, bool TILE>
 __global__ __launch_bounds__(kBlock) void k_icp(DevEngine e) {
   constexpr int NS = MODE == RTR_MODE_POINT_TO_POINT ? SUM_P2P_COUNT : SUM_P2L_COUNT;
-  __shared__ IcpShared<NS> sh;
+  using TileT = typename std::conditional<TILE, WaveTile, WaveTileNone>::type;
+  __shared__ IcpShared<NS, TileT> sh;
   const int tid = threadIdx.x;
   for (int pass = 0; pass < e.prm.max_iter; ++pass) {
     if (tid == 0)
@@ -447,7 +490,20 @@ __global__ __launch_bounds__(kBlock) void k_icp(DevEngine e) {
     if (sh.stop) break;
     for (;;) {
       if (tid == 0) {
-        const int it = (int)atomicAdd(&e.claim[pass], 1u);
+        // own XCD's queue first (L2 locality), then steal from the others
+        int it = e.n_items;
+        const int x0 = (int)(blockIdx.x % kXcds);
+        for (int t = 0; t < kXcds; ++t) {
+          const int x = (x0 + t) % kXcds;
+          const uint32_t cnt = e.xcd_off[x + 1] - e.xcd_off[x];
+          uint32_t* ctr = &e.claim[pass * kXcds + x];
+          if (cnt == 0u || __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
+          const uint32_t c = atomicAdd(ctr, 1u);
+          if (c < cnt) {
+            it = (int)(e.xcd_off[x] + c);
+            break;
+          }
+        }
         int run = 0;
         if (it < e.n_items) {
           const uint32_t p = e.items[it].x;
@@ -477,29 +533,39 @@ __global__ __launch_bounds__(kBlock) void k_icp(DevEngine e) {
       __syncthreads();
       const int it = sh.item;
       if (it >= e.n_items) break;
-      if (sh.run) process_item<MODE, NS>(e, it, pass, sh);
+      if (sh.run) process_item<MODE, NS, TILE, TileT>(e, it, pass, sh);
       __syncthreads();
     }
   }
 }
 
-int icp_blocks_per_cu(int mode) {
+int icp_blocks_per_cu(int mode, int use_tile) {
   int nb = 0;
   hipError_t err;
   if (mode == RTR_MODE_POINT_TO_PLANE)
-    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_PLANE>, kBlock, 0);
+    err = use_tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_PLANE, true>, kBlock, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_PLANE, false>, kBlock, 0);
   else
-    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_POINT>, kBlock, 0);
+    err = use_tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_POINT, true>, kBlock, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_POINT, false>, kBlock, 0);
   if (err != hipSuccess || nb < 1) nb = 1;
   return nb;
 }
 
 void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_icp_init, dim3((e.n_prob + kBlock - 1) / kBlock), dim3(kBlock), 0, s, e);
-  if (e.prm.mode == RTR_MODE_POINT_TO_PLANE)
-    hipLaunchKernelGGL(k_icp<RTR_MODE_POINT_TO_PLANE>, dim3(grid_blocks), dim3(kBlock), 0, s, e);
-  else
-    hipLaunchKernelGGL(k_icp<RTR_MODE_POINT_TO_POINT>, dim3(grid_blocks), dim3(kBlock), 0, s, e);
+  const dim3 g(grid_blocks), b(kBlock);
+  if (e.prm.mode == RTR_MODE_POINT_TO_PLANE) {
+    if (e.use_tile)
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, true>), g, b, 0, s, e);
+    else
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, false>), g, b, 0, s, e);
+  } else {
+    if (e.use_tile)
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true>), g, b, 0, s, e);
+    else
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false>), g, b, 0, s, e);
+  }
 }
 
 // ---------------------------------------------------------------------------
