@@ -7,9 +7,10 @@ correspondence distance 0.05 m, <= 50 iterations, transformation epsilon
 1e-8, 8 starts per pair, all starts of all pairs in one persistent launch
 replayed from a hipGraph.  A step = target grid builds + source sorts +
 every ICP iteration of every start + best-start selection, with the clouds
-already resident in HBM.
+already resident in HBM.  The engine's launch/layout parameters are
+autotuned during warmup (untimed).
 
-Prints one JSON line (see the field list at the end of main()).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -27,11 +28,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "registrations/sec @≤1mm RMSE vs FP64: learned-init+ICP vs plain ICP, 1 GPU"
+R_MAX, MAX_ITER, EPS = 0.05, 50, 1e-8
 
 
-def build_workload(n_pairs):
-    from realtime_robot_amd import synth
-    return synth.config3_batch(n_pairs)
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def start_guesses(srcs, tgts, n_starts):
@@ -42,46 +43,42 @@ def start_guesses(srcs, tgts, n_starts):
     return np.stack(guesses)
 
 
-def cpu_baseline(srcs, tgts, guesses, n_starts, gpu_best, budget_s=20.0):
+def cpu_baseline(srcs, tgts, guesses, n_starts, gpu_best, budget_s):
     """PCL-semantics CPU ICP (float kd-tree, leaf 15) on a bounded sample of
-    the same pairs and starts, on the host cores; the FP64 oracle on the same
-    sample is the accuracy checker of the GPU results."""
+    the same pairs and starts on the host cores; the FP64 oracle on part of
+    the sample is the accuracy checker of the GPU results."""
     import oracle
     from realtime_robot_amd import multistart, synth
     cores = min(16, os.cpu_count() or 1)
-    p = oracle.params(max_iter=50, r_max=0.05, eps_T=1e-8)
-    # grow the sample until the CPU run takes about budget_s
-    done, t_cpu, k = 0, 0.0, 0
-    order = list(range(len(srcs)))
-    while done < len(order) and t_cpu < budget_s:
-        batch = order[done:done + max(cores // 2, 1)]
-        S, Tt, G = [], [], []
+    p = oracle.params(max_iter=MAX_ITER, r_max=R_MAX, eps_T=EPS)
+    done, t_cpu = 0, 0.0
+    while done < len(srcs) and t_cpu < budget_s:
+        batch = list(range(done, min(done + max(cores // 2, 1), len(srcs))))
+        S, T, G = [], [], []
         for i in batch:
             for s in range(n_starts):
                 S.append(srcs[i])
-                Tt.append(tgts[i])
+                T.append(tgts[i])
                 G.append(guesses[i * n_starts + s])
         t0 = time.perf_counter()
-        oracle.pcl_icp_batch(S, Tt, p, G, nthreads=cores)
+        oracle.pcl_icp_batch(S, T, p, G, nthreads=cores)
         t_cpu += time.perf_counter() - t0
         done += len(batch)
-    cpu_rate = done / t_cpu
-    # accuracy of the GPU results vs the FP64 oracle on (part of) the sample
     n_chk = min(done, 8)
 
     def oracle_best(i):
         res = [oracle.icp(srcs[i], tgts[i], p, guess=guesses[i * n_starts + s]) for s in range(n_starts)]
-        b = multistart.select_best([dict(r, state=r["state"]) for r in res])
-        return i, res[b]["T"]
+        return i, res[multistart.select_best(res)]["T"]
 
     with ThreadPoolExecutor(cores) as ex:
         refs = list(ex.map(oracle_best, range(n_chk)))
     rmse = [synth.rmse_between(gpu_best[i], T, srcs[i]) for i, T in refs]
-    return ({"value": cpu_rate, "unit": "registrations/s", "cores": cores, "kind": "port",
-             "sample": f"{done} pairs x {n_starts} starts of the same workload, PCL-semantics float kd-tree ICP "
-                       f"(oracle/oracle.c orc_pcl_icp), one start per thread, {t_cpu:.1f} s"},
-            {"checked_pairs": n_chk, "max_rmse_vs_fp64_m": float(max(rmse)),
-             "within_1mm": int(sum(r <= 1e-3 for r in rmse))})
+    base = {"value": done / t_cpu, "unit": "registrations/s", "cores": cores, "kind": "port",
+            "sample": f"{done} pairs x {n_starts} starts of the same workload, PCL-semantics float kd-tree ICP "
+                      f"(oracle/oracle.c orc_pcl_icp_batch), one start per thread, {t_cpu:.1f} s"}
+    acc = {"checked_pairs": n_chk, "max_rmse_vs_fp64_m": float(max(rmse)),
+           "within_1mm": int(sum(r <= 1e-3 for r in rmse))}
+    return base, acc
 
 
 def main():
@@ -91,33 +88,42 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--starts", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--account-sample", type=int, default=24)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30,
+                    help="bytes of the HBM copy probe (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): registrations are independent and "
-              f"would shard across GPUs with no communication; multi-GPU runs are not part of this build.",
-              file=sys.stderr)
+              f"shard across GPUs with no communication; multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
 
     import realtime_robot_amd as R
     from realtime_robot_amd import _lib as L
-    from realtime_robot_amd import multistart
+    from realtime_robot_amd import accounting, autotune, multistart, synth
 
     t0 = time.perf_counter()
-    srcs, tgts, gts = build_workload(args.pairs)
+    srcs, tgts, gts = synth.config3_batch(args.pairs)
     S = args.starts
     guesses = start_guesses(srcs, tgts, S)
-    print(f"[bench] workload built in {time.perf_counter() - t0:.1f}s: {args.pairs} pairs, "
-          f"{sum(len(s) for s in srcs)} source / {sum(len(t) for t in tgts)} target points", file=sys.stderr,
-          flush=True)
+    log(f"workload built in {time.perf_counter() - t0:.1f}s: {args.pairs} pairs, "
+        f"{sum(len(s) for s in srcs)} source / {sum(len(t) for t in tgts)} target points")
 
     b = R.BatchRegistration(0)
     b.set_clouds(srcs, tgts)
     ids = np.repeat(np.arange(args.pairs), S)
     b.set_problems(ids, ids)
-    b.set_params(R.make_params(max_iterations=50, max_correspondence_distance=0.05, transformation_epsilon=1e-8))
-    res = (L.Result * (args.pairs * S))()
+    b.set_params(R.make_params(max_iterations=MAX_ITER, max_correspondence_distance=R_MAX,
+                               transformation_epsilon=EPS))
+    n_prob = args.pairs * S
+    tuned = None
+    if not args.no_autotune:
+        key = autotune.bucket_key(n_prob, np.mean([len(s) for s in srcs]), np.mean([len(t) for t in tgts]), 0)
+        tuned, tuned_ms, table = autotune.tune(b, guesses, key=None, use_cache=False)
+        log(f"autotune: {tuned} ({tuned_ms:.1f} ms); table {[(c, round(m, 1)) for c, m in table]}")
+    res = (L.Result * n_prob)()
     g = np.ascontiguousarray(np.transpose(guesses, (0, 2, 1))).reshape(-1)
 
     def step():
@@ -137,26 +143,43 @@ def main():
     t_el = time.perf_counter() - t_start
     ms_step = 1000.0 * t_el / args.steps
     value = args.pairs / (t_el / args.steps)
-    print(f"[bench] {args.steps} steps, {ms_step:.2f} ms/step, {value:.1f} registrations/s", file=sys.stderr,
-          flush=True)
+    log(f"{args.steps} steps, {ms_step:.2f} ms/step, {value:.1f} registrations/s")
 
-    # dominant kernel: persistent ICP (correspondence search + moments + solve), timed with HIP events
+    # dominant kernel: the persistent ICP kernel, timed with HIP events on the engine's stream
     b.set_profiling(True)
-    icp_ms = []
-    grid_ms = []
+    icp_ms, grid_ms = [], []
     for _ in range(3):
         L.check(L.lib.rtr_batch_align(b._h, g.ctypes.data, res))
         t = b.timing()
         icp_ms.append(t["ms_icp"])
         grid_ms.append(t["ms_grid"])
     b.set_profiling(False)
-    iters = np.array([res[i].iterations for i in range(args.pairs * S)], np.int64)
+    iters = np.array([res[i].iterations for i in range(n_prob)], np.int64)
     ns = np.array([len(srcs[i]) for i in ids], np.int64)
     nt = np.array([len(tgts[i]) for i in ids], np.int64)
-    # compulsory bytes per problem-iteration (query xyz + target xyz), fused reduction
-    alg_bytes = float(np.sum(iters * (12 * ns + 12 * nt)))
+    b_min_total = float(np.sum(iters * (12 * ns + 12 * nt)))
+    # B_iter on a seeded sample of problems from the transform of every iteration
+    rng = np.random.default_rng(0)
+    sample = rng.choice(n_prob, min(args.account_sample, n_prob), replace=False)
+    s_iter = s_min = 0
+    counts = {}
+    for p in sample:
+        pair = int(ids[p])
+        if pair not in counts:
+            counts[pair] = accounting.TargetCounts(tgts[pair], R_MAX)
+        hist = b.history(int(p), MAX_ITER)
+        for it in range(res[p].iterations):
+            bi, bm = accounting.iteration_bytes(srcs[pair], counts[pair], hist[it])
+            s_iter += bi
+            s_min += bm
+    ratio = s_iter / s_min if s_min else 1.0
+    b_iter_total = b_min_total * ratio
     t_icp = float(np.median(icp_ms)) / 1e3
-    achieved = alg_bytes / t_icp / 1e9
+    achieved = b_iter_total / t_icp / 1e9
+    import ctypes
+    copy_ms = ctypes.c_float()
+    L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 5, ctypes.byref(copy_ms)))
+    copy_gbs = 2.0 * args.copy_bytes / (copy_ms.value / 1e3) / 1e9
 
     best_T = [L.rowmajor(res[k].T) for k in best]
     out = {
@@ -174,14 +197,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched plain multi-start point-to-point ICP, {args.pairs} box-composite pairs "
                                f"(Ns logU[2k,50k], Nt logU[10k,200k], init rot U[0,30deg]), {S} starts, "
-                               f"r_max 0.05 m, <=50 iterations, eps 1e-8",
-                   "pairs": args.pairs, "starts": S, "problems": args.pairs * S,
-                   "iterations_total": int(iters.sum())},
+                               f"r_max {R_MAX} m, <={MAX_ITER} iterations, eps {EPS}",
+                   "pairs": args.pairs, "starts": S, "problems": n_prob, "iterations_total": int(iters.sum()),
+                   "tuning": list(tuned) if tuned else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_icp (persistent correspondence search + moments + FP64 solve)",
                      "kernel_ms": t_icp * 1e3, "grid_build_ms": float(np.median(grid_ms)),
-                     "algorithmic_bytes": alg_bytes},
+                     "algorithmic_bytes": b_iter_total, "compulsory_bytes": b_min_total,
+                     "achieved_compulsory_gbs": b_min_total / t_icp / 1e9,
+                     "b_iter_over_b_min": ratio, "b_iter_sample_problems": int(len(sample)),
+                     "hbm_copy_gbs": copy_gbs, "copy_probe_bytes": args.copy_bytes},
     }
     if not args.no_cpu_baseline:
         cb, acc = cpu_baseline(srcs, tgts, guesses, S, best_T, args.cpu_budget)

@@ -177,6 +177,10 @@ int rtr_batch_align(rtr_batch* b, const double* guesses, rtr_result* results);
 int rtr_batch_get_timing(rtr_batch* b, rtr_timing* out);
 /* Grid parameters chosen for target j: cell edge, dims, anchor. */
 int rtr_batch_get_grid(rtr_batch* b, int tgt, float* cell, int dims[3], float anchor[3]);
+/* Transforms each iteration of problem `prob` queried with, as float 3x4 rows
+ * (12 floats per entry): entry i for iteration i, entry `iterations` the final
+ * one.  *n_out = iterations + 1. */
+int rtr_batch_get_history(rtr_batch* b, int prob, float* rows12, int cap, int* n_out);
 /* Fitness score of every problem at its final transform (see rtr_icp_get_fitness_score). */
 int rtr_batch_fitness(rtr_batch* b, double max_range, double* out);
 
@@ -216,6 +220,11 @@ int rtr_pcd_info(const char* path, size_t* n_points, int* has_normals);
 int rtr_pcd_read(const char* path, float* xyz, float* normals, size_t cap, size_t* n_out);
 /* ASCII writer with 8 significant digits, like savePCDFileASCII. */
 int rtr_pcd_write_ascii(const char* path, const float* xyz, size_t n, size_t stride_bytes);
+
+/* Streaming copy of `bytes` (multiple of 16) repeated `reps` times; average
+ * milliseconds per copy.  Calibrates profiler byte counters and measures the
+ * HBM bandwidth the device sustains.  No reference counterpart. */
+int rtr_copy_probe(int device, size_t bytes, int reps, float* ms_avg);
 
 /* ------------------------------------------------------------------ */
 /* Self-check hooks: run the library's own geometry code on the host.   */

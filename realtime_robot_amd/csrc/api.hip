@@ -433,6 +433,13 @@ int rtr_batch_get_grid(rtr_batch* b, int tgt, float* cell, int dims[3], float an
   RTR_GUARD_END
 }
 
+int rtr_batch_get_history(rtr_batch* b, int prob, float* rows12, int cap, int* n_out) {
+  RTR_GUARD_BEGIN
+  if (!b || !rows12 || !n_out) return RTR_ERR_INVALID_ARG;
+  return b->eng.history(prob, rows12, cap, n_out);
+  RTR_GUARD_END
+}
+
 int rtr_batch_fitness(rtr_batch* b, double max_range, double* out) {
   RTR_GUARD_BEGIN
   if (!b || !out) return RTR_ERR_INVALID_ARG;

@@ -76,7 +76,7 @@ Engine::~Engine() {
                     &cell_start_, &cell_id_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
                     &bin_count_, &bin_start_, &bin_id_, &src_desc_, &prob_desc_, &state_,
                     &guesses_, &items_buf_, &partials_, &done_, &counters_, &scan_tmp_,
-                    &out_i32_, &out_f32_, &out_T_};
+                    &out_i32_, &out_f32_, &out_T_, &hist_};
   for (DevBuf* b : bufs) b->release();
   if (pin_) (void)hipHostFree(pin_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -311,6 +311,7 @@ int Engine::upload_problems() {
   RTR_TRY(partials_.ensure(items_.size() * NSUM_MAX * 2 * sizeof(float)));
   RTR_TRY(counters_.ensure(counter_words() * 4));
   RTR_TRY(out_T_.ensure(16 * sizeof(double)));
+  RTR_TRY(hist_.ensure(np * (size_t)(prm_.max_iterations + 1) * 12 * sizeof(float)));
   if (np) RTR_HIP(hipMemcpy(prob_desc_.p, probs_.data(), np * sizeof(Problem), hipMemcpyHostToDevice));
   if (!items_.empty())
     RTR_HIP(hipMemcpy(items_buf_.p, items_.data(), items_.size() * sizeof(uint2), hipMemcpyHostToDevice));
@@ -350,6 +351,7 @@ DevEngine Engine::dev_view() const {
   e.n_prob = (int)probs_.size();
   e.st = state_.as<ProblemState>();
   e.guesses = guesses_.as<double>();
+  e.hist = hist_.as<float>();
   e.items = items_buf_.as<uint2>();
   e.n_items = (int)items_.size();
   e.item_pts = tune_.item_pts;
@@ -603,6 +605,30 @@ int Engine::compute_target_normals(int k, const float vp[3]) {
   normals_valid_ = true;
   grid_dirty_ = true;
   graph_dirty_ = true;
+  return RTR_OK;
+}
+
+// Float rows (3x4) of the transform each iteration of problem `prob` queried
+// with: entry i for iteration i, entry `iterations` the final transform.
+int Engine::history(int prob, float* rows12, int cap, int* n_out) {
+  RTR_HIP(hipSetDevice(device_));
+  if (prob < 0 || prob >= (int)probs_.size()) {
+    set_error("problem %d out of range", prob);
+    return RTR_ERR_INVALID_ARG;
+  }
+  ProblemState st;
+  RTR_HIP(hipMemcpyAsync(&st, state_.as<ProblemState>() + prob, sizeof(st), hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  const int n = st.iter + 1;
+  *n_out = n;
+  if (n > cap) {
+    set_error("history of %d entries, buffer holds %d", n, cap);
+    return RTR_ERR_CAPACITY;
+  }
+  const size_t off = (size_t)prob * (prm_.max_iterations + 1) * 12;
+  RTR_HIP(hipMemcpyAsync(rows12, hist_.as<float>() + off, (size_t)n * 12 * sizeof(float), hipMemcpyDeviceToHost,
+                         stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
   return RTR_OK;
 }
 

@@ -41,7 +41,7 @@ struct TuneParams {
   float cells_per_point = 2.0f;   // grid cell budget per target point
   int item_pts = 1024;            // source points per work item
   int blocks_per_cu = 0;          // persistent workgroups per CU (0 = occupancy limit)
-  int use_tile = 1;               // LDS candidate tiles on/off
+  int use_tile = 0;               // LDS candidate tiles on/off
 };
 
 class Engine {
@@ -68,6 +68,7 @@ class Engine {
   int transformed_source(int prob, float* out, size_t stride);
   int fitness(double max_range, double* out);
   int grid_info(int tgt, float* cell, int dims[3], float anchor[3]);
+  int history(int prob, float* rows12, int cap, int* n_out);
   const rtr_timing& timing() const { return timing_; }
   int n_problems() const { return (int)probs_.size(); }
   size_t source_size(int prob) const;
@@ -117,7 +118,7 @@ class Engine {
   DevBuf tgt_desc_;
   DevBuf src_raw_, sx_, sy_, sz_, perm_, bin_count_, bin_start_, bin_id_, src_desc_;
   DevBuf prob_desc_, state_, guesses_, items_buf_, partials_, done_, counters_, scan_tmp_;
-  DevBuf out_i32_, out_f32_, out_T_;
+  DevBuf out_i32_, out_f32_, out_T_, hist_;
   // pinned host staging
   void* pin_ = nullptr;
   size_t pin_bytes_ = 0;

@@ -44,6 +44,7 @@ struct DevEngine {
   int n_prob;
   ProblemState* st;
   const double* guesses;     // n_prob * 16 (column-major)
+  float* hist;               // n_prob * (max_iter + 1) * 12: float rows of T used by each iteration
   const uint2* items;        // (problem, first sorted source point)
   int n_items;
   int item_pts;              // points per item

@@ -77,6 +77,7 @@ __global__ void k_icp_init(DevEngine e) {
   S.state = RTR_NOT_CONVERGED;
   S.finished = 0;
   e.done[p] = 0;
+  if (e.hist) tf_rows_f32(S.T, e.hist + (size_t)p * (e.prm.max_iter + 1) * 12);
   uint32_t* word = (uint32_t*)&S.gen;
   if (e.probs[p].n_items == 0) {
     S.state = RTR_FAIL_NO_CORRESPONDENCES;
@@ -261,6 +262,7 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
       mat4_mul_to(sh.Tinc, S.T, sh.Tn);
       for (int k = 0; k < 16; ++k) S.T[k] = sh.Tn[k];
       S.iter += 1;
+      if (e.hist) tf_rows_f32(sh.Tn, e.hist + ((size_t)p * (e.prm.max_iter + 1) + S.iter) * 12);
       S.mse = mse;
       double prev = S.prev_mse;
       const int stt = convergence(e.prm, S.iter, sh.Tinc, mse, &prev);

@@ -252,6 +252,13 @@ class BatchRegistration:
         L.check(L.lib.rtr_batch_get_grid(self._h, tgt, C.byref(cell), L.ptr(dims), L.ptr(anchor)))
         return cell.value, dims, anchor
 
+    def history(self, prob: int, max_iterations: int = 256):
+        """(iterations + 1, 3, 4) float32 rows of the transform each iteration queried with."""
+        buf = np.empty((max_iterations + 1, 12), np.float32)
+        n = C.c_int()
+        L.check(L.lib.rtr_batch_get_history(self._h, int(prob), L.ptr(buf), len(buf), C.byref(n)))
+        return buf[:n.value].reshape(-1, 3, 4).copy()
+
     def fitness(self, max_range=DBL_MAX):
         out = np.empty(self.n_problems)
         L.check(L.lib.rtr_batch_fitness(self._h, float(max_range), L.ptr(out)))
