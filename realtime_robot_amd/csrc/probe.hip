@@ -12,7 +12,7 @@ namespace rtr {
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
                                                     size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = __builtin_nontemporal_load(src + i);
+    dst[i] = src[i];
 }
 
 }  // namespace rtr
