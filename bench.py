@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--starts", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", type=str, default="",
+                    help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile' "
+                         "(skips the autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--account-sample", type=int, default=24)
@@ -119,7 +122,11 @@ def main():
                                transformation_epsilon=EPS))
     n_prob = args.pairs * S
     tuned = None
-    if not args.no_autotune:
+    if args.tuning:
+        f = [float(v) for v in args.tuning.split(",")]
+        tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4]))
+        autotune.apply(b, tuned)
+    elif not args.no_autotune:
         key = autotune.bucket_key(n_prob, np.mean([len(s) for s in srcs]), np.mean([len(t) for t in tgts]), 0)
         tuned, tuned_ms, table = autotune.tune(b, guesses, key=None, use_cache=False)
         log(f"autotune: {tuned} ({tuned_ms:.1f} ms); table {[(c, round(m, 1)) for c, m in table]}")
