@@ -188,6 +188,16 @@ def main():
     L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 5, ctypes.byref(copy_ms)))
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms.value / 1e3) / 1e9
 
+    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this
+    # workload (copy-probe calibrated); reported only when the tuning matches the profiled one
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", "k_icp_multistart_256x8.json")
+    if os.path.exists(prof) and tuned is not None and args.pairs == 256 and S == 8:
+        with open(prof) as f:
+            pj = json.load(f)
+        if [round(v, 4) for v in pj["tuning"]] == [round(float(v), 4) for v in tuned]:
+            traffic, traffic_src = float(pj["traffic_bytes"]), "profiles/k_icp_multistart_256x8.md"
+
     best_T = [L.rowmajor(res[k].T) for k in best]
     out = {
         "metric": METRIC,
@@ -208,7 +218,7 @@ def main():
                    "pairs": args.pairs, "starts": S, "problems": n_prob, "iterations_total": int(iters.sum()),
                    "tuning": list(tuned) if tuned else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_icp (persistent correspondence search + moments + FP64 solve)",
                      "kernel_ms": t_icp * 1e3, "grid_build_ms": float(np.median(grid_ms)),
                      "algorithmic_bytes": b_iter_total, "compulsory_bytes": b_min_total,

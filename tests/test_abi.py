@@ -1,0 +1,180 @@
+"""C ABI and host-side checks that need no GPU: exported symbols, PCD I/O,
+the library's own geometry code run on the host (rtr_selfcheck_*), the C++
+header, and the error path when no device is present."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import realtime_robot_amd as R
+from helpers import GOLDEN, golden
+from pcd_np import read_pcd, read_xyz
+from realtime_robot_amd import _lib as L
+from realtime_robot_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _gpu_present():
+    try:
+        import torch  # noqa: F401  (device count only; initialises nothing)
+        return torch.cuda.device_count() > 0
+    except Exception:  # noqa: BLE001
+        return False
+
+
+def test_every_declared_symbol_is_exported():
+    hdr = open(os.path.join(ROOT, "include", "rtr.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(rtr_\w+)\(", hdr, re.M))
+    assert len(declared) > 40
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (rtr_\w+)", out))
+    assert declared <= exported, sorted(declared - exported)
+    assert set(L.EXPORTED) <= exported
+
+
+def test_default_params_are_pcl_defaults():
+    p = L.default_params()
+    assert p.max_iterations == 10 and p.min_correspondences == 3 and p.mode == 0
+    assert p.max_correspondence_distance == np.sqrt(np.finfo(np.float64).max)
+    assert p.transformation_epsilon == 0.0 and p.euclidean_fitness_epsilon == -np.finfo(np.float64).max
+
+
+@pytest.mark.parametrize("name", sorted(f for f in os.listdir(GOLDEN) if f.endswith(".pcd")))
+def test_pcd_reader_matches_numpy_reader(name):
+    xyz, nrm = R.load_pcd(golden(name))
+    d = read_pcd(golden(name))
+    assert len(xyz) == d["points"]
+    assert np.array_equal(xyz, read_xyz(golden(name)))
+    if "normal_x" in d["fields"]:
+        ref = np.stack([d["data"]["normal_x"], d["data"]["normal_y"], d["data"]["normal_z"]], 1)
+        assert np.array_equal(nrm, ref.astype(np.float32), equal_nan=True)  # the file holds NaN normals
+    else:
+        assert nrm is None
+
+
+def test_pcd_ascii_writer_reproduces_reference_output_byte_for_byte(tmp_path):
+    """mcloud.pcd was written by pcl::io::savePCDFileASCII (function.h:126)."""
+    xyz, _ = R.load_pcd(golden("mcloud.pcd"))
+    out = tmp_path / "m.pcd"
+    R.save_pcd_ascii(str(out), xyz)
+    assert out.read_bytes() == open(golden("mcloud.pcd"), "rb").read()
+
+
+def test_pcd_errors():
+    with pytest.raises(R.RtrError) as e:
+        R.load_pcd("/nonexistent/file.pcd")
+    assert e.value.code == L.ERR_IO
+
+
+@pytest.mark.parametrize("cell,rmax", [(0.05, 0.08), (0.02, None), (0.3, 0.05), (0.011, 0.2), (0.004, 0.05),
+                                       (0.004, None)])
+def test_host_grid_search_matches_brute_force(cell, rmax):
+    rng = np.random.default_rng(1)
+    t = rng.random((3000, 3)).astype(np.float32)
+    q = (rng.random((1500, 3)) * 1.6 - 0.3).astype(np.float32)
+    i1, d1 = R.selfcheck_nn(t, q, cell, 1e30 if rmax is None else rmax)
+    i2, d2 = oracle.nn_brute(q, t, rmax)
+    assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
+
+
+def test_host_grid_search_ties_nans_and_empty_target():
+    rng = np.random.default_rng(2)
+    base = np.round(rng.random((2000, 3)) * 10) / 10
+    t = np.concatenate([base, base[:500]]).astype(np.float32)
+    t[3] = np.nan
+    q = (np.round(rng.random((1000, 3)) * 20) / 20).astype(np.float32)
+    q[0] = np.nan
+    q[1] = [1e6, 0, 0]
+    for cell in (0.02, 0.05, 0.13):
+        i1, d1 = R.selfcheck_nn(t, q, cell, 0.07)
+        i2, d2 = oracle.nn_brute(q, t, 0.07)
+        assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
+    i1, _ = R.selfcheck_nn(np.zeros((0, 3), np.float32), q, 0.05, 0.1)
+    assert np.all(i1 == -1)
+
+
+def test_host_grid_search_on_surfaces():
+    src, tgt, T = synth.config1_pair(0)
+    q = oracle.transform(T, src)
+    for cell in (0.006, 0.0125, 0.03, 0.0505):
+        for rmax in (0.05, 0.01):
+            i1, d1 = R.selfcheck_nn(tgt, q, cell, rmax)
+            i2, d2 = oracle.nn_brute(q, tgt, rmax)
+            assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
+
+
+def test_host_solver_matches_oracle():
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(4)
+    for k in range(4):
+        a = (rng.random((400, 3)) * 2 - 1 + 3).astype(np.float32)
+        T = np.eye(4)
+        T[:3, :3] = Rotation.random(random_state=k).as_matrix()
+        T[:3, 3] = rng.random(3)
+        b = (a.astype(np.float64) @ T[:3, :3].T + T[:3, 3]).astype(np.float32)
+        T1 = R.selfcheck_solve(0, a, b, anchor=(3, 3, 3))
+        T2 = oracle.umeyama(a.astype(np.float64), b.astype(np.float64))
+        assert np.abs(T1 - T2).max() < 1e-6
+    # point-to-plane: the same linear least-squares solution as the oracle's
+    a = rng.random((800, 3)).astype(np.float32)
+    n = rng.normal(size=(800, 3))
+    n = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+    b = (a + rng.normal(size=a.shape) * 1e-3).astype(np.float32)
+    T1 = R.selfcheck_solve(1, a, b, n, anchor=(0.5, 0.5, 0.5))
+    s = a.astype(np.float64) - 0.5
+    d = b.astype(np.float64) - 0.5
+    A = np.concatenate([np.cross(s, n.astype(np.float64)), n], 1)
+    x = np.linalg.solve(A.T @ A, A.T @ np.sum(n * (d - s), 1))
+    assert np.abs(T1 - oracle.euler_to_T(x, (0.5, 0.5, 0.5))).max() < 1e-6
+
+
+@pytest.mark.skipif(_gpu_present(), reason="a device is present")
+def test_no_device_is_a_clean_error():
+    with pytest.raises(R.RtrError) as e:
+        R.IterativeClosestPoint()
+    assert e.value.code == L.ERR_DEVICE
+    h = C.c_void_p()
+    assert L.lib.rtr_batch_create(C.byref(h), 0) == L.ERR_DEVICE
+    assert L.lib.rtr_map_create(C.byref(h), 0, C.c_float(0.02), 100) == L.ERR_DEVICE
+
+
+def test_invalid_arguments_are_rejected():
+    assert L.lib.rtr_icp_align(None, None, None, 0, None, None) == L.ERR_INVALID_ARG
+    assert L.lib.rtr_map_integrate(None, None, 0, 0, None) == L.ERR_INVALID_ARG
+    with pytest.raises(ValueError):
+        R.selfcheck_nn(np.zeros((5, 2), np.float32), np.zeros((3, 3), np.float32), 0.1)
+
+
+def _build_cpp(tmp_path):
+    exe = tmp_path / "keypoint_icp"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "keypoint_icp.cpp"), "-o", str(exe),
+                    "-L", os.path.dirname(L.LIB_PATH), "-lrtr", "-Wl,-rpath," + os.path.dirname(L.LIB_PATH)],
+                   check=True)
+    return exe
+
+
+def test_cpp_header_compiles_links_and_does_io(tmp_path):
+    exe = _build_cpp(tmp_path)
+    out = tmp_path / "o.pcd"
+    r = subprocess.run([str(exe), golden("mcloud.pcd"), golden("5.pcd"), str(out), "--io-only"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "source 1909 target 1909" in r.stdout
+    assert out.read_bytes() == open(golden("mcloud.pcd"), "rb").read()
+
+
+@pytest.mark.gpu
+def test_cpp_keypoint_icp_on_device(tmp_path):
+    exe = _build_cpp(tmp_path)
+    out = tmp_path / "t.pcd"
+    r = subprocess.run([str(exe), golden("5.pcd"), golden("mcloud.pcd"), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    moved = read_xyz(str(out))
+    assert np.abs(moved - read_xyz(golden("mcloud.pcd"))).max() < 1e-5
