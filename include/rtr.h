@@ -114,6 +114,8 @@ int rtr_icp_set_input_target(rtr_icp* h, const float* xyz, size_t n, size_t stri
 int rtr_icp_set_target_normals(rtr_icp* h, const float* nxyz, size_t n, size_t stride_bytes);
 /* pcl::NormalEstimation with setKSearch(k) and viewpoint vp on the current target. */
 int rtr_icp_compute_target_normals(rtr_icp* h, int k, const float vp[3]);
+/* Copy out the target normals in target order (n_target x 3 floats). */
+int rtr_icp_get_target_normals(rtr_icp* h, float* nxyz);
 /* setMaxCorrespondenceDistance / setMaximumIterations / setTransformationEpsilon /
  * setEuclideanFitnessEpsilon, and the whole parameter block at once. */
 int rtr_icp_set_max_correspondence_distance(rtr_icp* h, double d);

@@ -70,6 +70,7 @@ _SIGS = {
     "rtr_icp_set_input_target": (_i, [_vp, _vp, _sz, _sz]),
     "rtr_icp_set_target_normals": (_i, [_vp, _vp, _sz, _sz]),
     "rtr_icp_compute_target_normals": (_i, [_vp, _i, _vp]),
+    "rtr_icp_get_target_normals": (_i, [_vp, _vp]),
     "rtr_icp_set_max_correspondence_distance": (_i, [_vp, _d]),
     "rtr_icp_set_maximum_iterations": (_i, [_vp, _i]),
     "rtr_icp_set_transformation_epsilon": (_i, [_vp, _d]),

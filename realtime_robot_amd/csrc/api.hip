@@ -191,6 +191,15 @@ int rtr_icp_compute_target_normals(rtr_icp* h, int k, const float vp[3]) {
   RTR_GUARD_END
 }
 
+int rtr_icp_get_target_normals(rtr_icp* h, float* nxyz) {
+  RTR_GUARD_BEGIN
+  if (!h || !nxyz) return RTR_ERR_INVALID_ARG;
+  int rc = icp_sync_clouds(h);
+  if (rc) return rc;
+  return h->eng.target_normals(0, nxyz);
+  RTR_GUARD_END
+}
+
 static int icp_update_param(rtr_icp* h, void (*f)(rtr_params&, double), double v) {
   if (!h) return RTR_ERR_INVALID_ARG;
   rtr_params p = h->eng.params();

@@ -608,6 +608,22 @@ int Engine::compute_target_normals(int k, const float vp[3]) {
   return RTR_OK;
 }
 
+int Engine::target_normals(int tgt, float* out) {
+  RTR_HIP(hipSetDevice(device_));
+  if (tgt < 0 || tgt >= (int)tgts_.size()) {
+    set_error("target %d out of range", tgt);
+    return RTR_ERR_INVALID_ARG;
+  }
+  if (!normals_valid_) {
+    set_error("no target normals have been set or computed");
+    return RTR_ERR_NOT_READY;
+  }
+  RTR_HIP(hipMemcpyAsync(out, tgt_nrm_raw_.as<float>() + 3 * (size_t)tgts_[tgt].raw_off,
+                         (size_t)tgts_[tgt].n * 12, hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  return RTR_OK;
+}
+
 // Float rows (3x4) of the transform each iteration of problem `prob` queried
 // with: entry i for iteration i, entry `iterations` the final transform.
 int Engine::history(int prob, float* rows12, int cap, int* n_out) {

@@ -69,6 +69,7 @@ class Engine {
   int fitness(double max_range, double* out);
   int grid_info(int tgt, float* cell, int dims[3], float anchor[3]);
   int history(int prob, float* rows12, int cap, int* n_out);
+  int target_normals(int tgt, float* out);
   const rtr_timing& timing() const { return timing_; }
   int n_problems() const { return (int)probs_.size(); }
   size_t source_size(int prob) const;

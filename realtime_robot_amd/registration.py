@@ -54,6 +54,7 @@ class IterativeClosestPoint:
     def setInputTarget(self, cloud):
         a = L.pts3(cloud)
         L.check(L.lib.rtr_icp_set_input_target(self._h, L.ptr(a), len(a), 12))
+        self._ntgt = len(a)
 
     def setTargetNormals(self, normals):
         a = L.pts3(normals)
@@ -62,6 +63,11 @@ class IterativeClosestPoint:
     def computeTargetNormals(self, k: int = 20, viewpoint=(0.0, 0.0, 0.0)):
         vp = np.asarray(viewpoint, np.float32)
         L.check(L.lib.rtr_icp_compute_target_normals(self._h, k, L.ptr(vp)))
+
+    def getTargetNormals(self) -> np.ndarray:
+        out = np.empty((self._ntgt, 3), np.float32)
+        L.check(L.lib.rtr_icp_get_target_normals(self._h, L.ptr(out)))
+        return out
 
     # --- parameters -------------------------------------------------------
     def setMaxCorrespondenceDistance(self, d: float):
