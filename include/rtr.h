@@ -185,6 +185,15 @@ int rtr_batch_get_grid(rtr_batch* b, int tgt, float* cell, int dims[3], float an
 int rtr_batch_get_history(rtr_batch* b, int prob, float* rows12, int cap, int* n_out);
 /* Fitness score of every problem at its final transform (see rtr_icp_get_fitness_score). */
 int rtr_batch_fitness(rtr_batch* b, double max_range, double* out);
+/* Registration judge at each problem's final transform.  A strided sample of
+ * up to max_points source points is moved by the final pose; a point is an
+ * inlier when its nearest target point is within the max correspondence
+ * distance; plane_rmse = RMS distance of the inliers to the plane fitted (PCA)
+ * to their k nearest target points (DBL_MAX with no inlier), inlier_frac =
+ * inliers / sampled points.  Either output is nullable.  No reference
+ * counterpart: it is the run-time acceptance test that decides whether a
+ * learned proposal needs the multi-start fallback. */
+int rtr_batch_judge(rtr_batch* b, int k, int max_points, double* plane_rmse, double* inlier_frac);
 
 /* ------------------------------------------------------------------ */
 /* Voxel-hash map: merging transformed clouds into the scene map.       */
@@ -227,6 +236,57 @@ int rtr_pcd_write_ascii(const char* path, const float* xyz, size_t n, size_t str
  * milliseconds per copy.  Calibrates profiler byte counters and measures the
  * HBM bandwidth the device sustains.  No reference counterpart. */
 int rtr_copy_probe(int device, size_t bytes, int reps, float* ms_avg);
+
+/* ------------------------------------------------------------------ */
+/* Learned pose proposer: farthest-point sampling (1024 points per      */
+/* cloud), shared point MLP 3-64-128-256 + max-pool per cloud, head     */
+/* 512-256-9 (6D rotation + translation), BF16 MFMA with FP32           */
+/* accumulation, trained with Adam on poses the GPU ICP converged to.   */
+/* Replaces the reference's coarse alignment stage that produces the    */
+/* initial pose for ICP: keypoint/plane-area matching, TDF yaw search   */
+/* and consensus (RealTimeRobot.cpp:45-105, matching.h:122-222,         */
+/* function.h:35-109).                                                  */
+/* ------------------------------------------------------------------ */
+typedef struct rtr_prop rtr_prop;
+
+/* Buffers for up to max_pairs (source, target) pairs per call. */
+int rtr_prop_create(rtr_prop** out, int device, int max_pairs);
+int rtr_prop_destroy(rtr_prop* h);
+/* Number of FP32 parameters (W1 16x64 (xyz rows 0-2, rows 3-15 zero-input
+ * padding), b1, W2 64x128, b2, W3 128x256, b3, W4 512x256, b4, W5 256x9, b5;
+ * matrices row-major (in x out)). */
+int rtr_prop_num_params(size_t* n);
+int rtr_prop_set_weights(rtr_prop* h, const float* w);
+int rtr_prop_get_weights(rtr_prop* h, float* w);
+/* Upload n_pairs (source, target) clouds (packed xyz); they stay resident. */
+int rtr_prop_set_clouds(rtr_prop* h, int n_pairs, const float* const* src, const size_t* ns,
+                        const float* const* tgt, const size_t* nt);
+/* Farthest-point samples of the resident clouds (clouds ordered src0, tgt0,
+ * src1, ...): 1024 points per cloud centred on the centroid of a strided
+ * subsample of at most 8192 points (pts: 2*n*1024*3), the centroids (2*n*3,
+ * nullable) and the chosen subsample indices (2*n*1024, nullable).  Selection
+ * starts at subsample index 0; ties go to the lowest index. */
+int rtr_prop_fps(rtr_prop* h, float* pts, float* centroids, int32_t* sel);
+/* FPS + network on the resident clouds: one 4x4 pose per pair (n*16 doubles,
+ * column-major) mapping the source onto the target. */
+int rtr_prop_propose(rtr_prop* h, double* poses);
+/* Device time of the last rtr_prop_propose (FPS + network), ms. */
+int rtr_prop_last_ms(rtr_prop* h, float* ms);
+/* Network on given centred FPS points (n*2*1024*3, clouds src0, tgt0, ...).
+ * Outputs (each nullable): head outputs O (n*9), poses R|t (n*12 doubles, R
+ * row-major, in the centred frames).  With labels (n*12 doubles, same
+ * layout): the mean loss mean_i ||(R - R*) s_i + t - t*||^2 over the source
+ * points and pairs, and its gradient w.r.t. the FP32 parameters. */
+int rtr_prop_forward_backward(rtr_prop* h, int n_pairs, const float* xyz, const double* labels,
+                              float* O, double* rt, float* loss, float* grad);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on a labelled batch. */
+int rtr_prop_train_step(rtr_prop* h, int n_pairs, const float* xyz, const double* labels, float lr,
+                        float* loss);
+/* The proposer's BF16 MFMA product on its own: C (M x N, FP32) = op(A) op(B)
+ * with A, B rounded to BF16 (A is M x K, or K x M when ta; B is K x N, or
+ * N x K when tb), K split over k_split workgroups accumulated atomically. */
+int rtr_gemm_bf16(int device, const float* A, int ta, const float* B, int tb, float* C, int M, int N,
+                  int K, int k_split);
 
 /* ------------------------------------------------------------------ */
 /* Self-check hooks: run the library's own geometry code on the host.   */

@@ -76,6 +76,7 @@ _vox = _sig("orc_voxel_accumulate", C.c_int64, [_f32p, C.c_int64, C.c_void_p, C.
                                                 _i64p, C.c_int64, _i64p])
 _normals = _sig("orc_normals", None, [_f32p, C.c_int64, C.c_int, _f32p, _f32p, _f32p])
 _downsample = _sig("orc_voxel_downsample", C.c_int64, [_f32p, C.c_int64, C.c_float, _f32p, C.c_int64, _i64p])
+_fps = _sig("orc_fps", None, [_f32p, C.c_int64, C.c_int64, C.c_int, _i32p, _f64p])
 _pcl_icp = _sig("orc_pcl_icp", C.c_int, [_f32p, C.c_int64, _f32p, C.c_int64, C.POINTER(IcpParams), _f64p,
                                          C.POINTER(IcpResult)])
 _pcl_batch = _sig("orc_pcl_icp_batch", C.c_int, [C.c_int, C.c_void_p, _i64p, C.c_void_p, _i64p,
@@ -242,6 +243,18 @@ def voxel_downsample(pts, leaf):
     counts = np.empty(max(len(p), 1), np.int64)
     nv = _downsample(p, len(p), np.float32(leaf), out, len(p), counts)
     return out[:nv].copy(), counts[:nv].copy()
+
+
+def fps(pts, k=1024, max_sub=8192):
+    """Farthest-point sample: (subsample indices (k,), FP64 centroid of the
+    subsample, the subsample itself (m, 3))."""
+    p = _pts(pts)
+    sel = np.empty(k, np.int32)
+    cen = np.empty(3)
+    _fps(p, len(p), int(max_sub), int(k), sel, cen)
+    m = min(len(p), max_sub)
+    sub = p[(np.arange(m, dtype=np.uint64) * np.uint64(len(p)) // np.uint64(max(m, 1))).astype(np.int64)]
+    return sel, cen, sub
 
 
 def pcl_icp(src, tgt, p: IcpParams, guess=None):

@@ -1100,3 +1100,42 @@ int orc_pcl_icp_batch(int npairs, const float* const* src, const int64_t* ns,
   pthread_mutex_destroy(&c.mu);
   return 0;
 }
+
+/* ---- farthest-point sampling (learned proposer input) ---- */
+void orc_fps(const float* pts, int64_t n, int64_t max_sub, int k, int32_t* sel, double* centroid) {
+  const int64_t m = n < max_sub ? n : max_sub;
+  float* sp = (float*)malloc(sizeof(float) * 3 * (size_t)(m > 0 ? m : 1));
+  float* dist = (float*)malloc(sizeof(float) * (size_t)(m > 0 ? m : 1));
+  double c[3] = {0, 0, 0};
+  for (int64_t i = 0; i < m; ++i) {
+    const int64_t src = (int64_t)(((uint64_t)i * (uint64_t)n) / (uint64_t)m);
+    for (int a = 0; a < 3; ++a) {
+      sp[3 * i + a] = pts[3 * src + a];
+      c[a] += sp[3 * i + a];
+    }
+    dist[i] = 3.0e38f;
+  }
+  for (int a = 0; a < 3; ++a) centroid[a] = m > 0 ? c[a] / (double)m : 0.0;
+  int64_t cur = 0;
+  for (int s = 0; s < k; ++s) {
+    sel[s] = (int32_t)cur;
+    if (m == 0) continue;
+    const float qx = sp[3 * cur], qy = sp[3 * cur + 1], qz = sp[3 * cur + 2];
+    float bv = -1.0f;
+    int64_t bi = 0;
+    for (int64_t i = 0; i < m; ++i) {
+      const float dx = sp[3 * i] - qx, dy = sp[3 * i + 1] - qy, dz = sp[3 * i + 2] - qz;
+      float d = dx * dx;
+      d = d + dy * dy;
+      d = d + dz * dz;
+      if (d < dist[i]) dist[i] = d;
+      if (dist[i] > bv) {
+        bv = dist[i];
+        bi = i;
+      }
+    }
+    cur = bi;
+  }
+  free(sp);
+  free(dist);
+}

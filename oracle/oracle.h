@@ -134,6 +134,14 @@ void orc_normals(const float* pts, int64_t n, int k, const float vp[3], float* n
 int64_t orc_voxel_downsample(const float* pts, int64_t n, float leaf, float* out, int64_t cap,
                              int64_t* counts);
 
+/* Farthest-point sampling as the learned proposer defines it: a strided
+ * subsample of m = min(n, max_sub) points (subsample i = point i*n/m), then k
+ * greedy picks starting at subsample index 0, each the point with the largest
+ * float squared distance ((dx*dx + dy*dy) + dz*dz) to the chosen set, lowest
+ * index on ties.  sel (k) receives subsample indices; centroid (3, FP64 mean
+ * of the subsample). */
+void orc_fps(const float* pts, int64_t n, int64_t max_sub, int k, int32_t* sel, double* centroid);
+
 /* ---- PCL-semantics CPU baseline (float, cumulative, kd-tree leaf 15) ---- */
 int orc_pcl_icp(const float* src, int64_t ns, const float* tgt, int64_t nt, const orc_icp_params* p,
                 const double guess[16], orc_icp_result* out);

@@ -99,6 +99,7 @@ _SIGS = {
     "rtr_batch_get_grid": (_i, [_vp, _i, _vp, _vp, _vp]),
     "rtr_batch_fitness": (_i, [_vp, _d, _vp]),
     "rtr_batch_get_history": (_i, [_vp, _i, _vp, _i, C.POINTER(_i)]),
+    "rtr_batch_judge": (_i, [_vp, _i, _i, _vp, _vp]),
     "rtr_map_create": (_i, [C.POINTER(_vp), _i, _f, _sz]),
     "rtr_map_destroy": (_i, [_vp]),
     "rtr_map_clear": (_i, [_vp]),
@@ -111,6 +112,18 @@ _SIGS = {
     "rtr_pcd_read": (_i, [C.c_char_p, _vp, _vp, _sz, C.POINTER(_sz)]),
     "rtr_pcd_write_ascii": (_i, [C.c_char_p, _vp, _sz, _sz]),
     "rtr_copy_probe": (_i, [_i, _sz, _i, C.POINTER(_f)]),
+    "rtr_prop_create": (_i, [C.POINTER(_vp), _i, _i]),
+    "rtr_prop_destroy": (_i, [_vp]),
+    "rtr_prop_num_params": (_i, [C.POINTER(_sz)]),
+    "rtr_prop_set_weights": (_i, [_vp, _vp]),
+    "rtr_prop_get_weights": (_i, [_vp, _vp]),
+    "rtr_prop_set_clouds": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "rtr_prop_fps": (_i, [_vp, _vp, _vp, _vp]),
+    "rtr_prop_propose": (_i, [_vp, _vp]),
+    "rtr_prop_last_ms": (_i, [_vp, C.POINTER(_f)]),
+    "rtr_prop_forward_backward": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rtr_prop_train_step": (_i, [_vp, _i, _vp, _vp, _f, C.POINTER(_f)]),
+    "rtr_gemm_bf16": (_i, [_i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _i]),
     "rtr_selfcheck_nn": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp]),
     "rtr_selfcheck_nn_stats": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp, _vp, _vp]),
     "rtr_selfcheck_solve": (_i, [_i, _vp, _vp, _vp, _sz, _vp, _vp]),

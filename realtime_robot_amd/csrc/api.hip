@@ -456,6 +456,13 @@ int rtr_batch_fitness(rtr_batch* b, double max_range, double* out) {
   RTR_GUARD_END
 }
 
+int rtr_batch_judge(rtr_batch* b, int k, int max_points, double* plane_rmse, double* inlier_frac) {
+  RTR_GUARD_BEGIN
+  if (!b) return RTR_ERR_INVALID_ARG;
+  return b->eng.judge(k, max_points, plane_rmse, inlier_frac);
+  RTR_GUARD_END
+}
+
 // ---------------------------------------------------------------------------
 // self-check hooks: the same geometry code on the host
 // ---------------------------------------------------------------------------

@@ -176,6 +176,7 @@ int Engine::set_clouds(int n_src, const float* const* src, const size_t* ns, con
   }
   grid_dirty_ = true;
   graph_dirty_ = true;
+  results_valid_ = false;
   if (!probs_.empty()) return upload_problems();
   return RTR_OK;
 }
@@ -317,6 +318,7 @@ int Engine::upload_problems() {
     RTR_HIP(hipMemcpy(items_buf_.p, items_.data(), items_.size() * sizeof(uint2), hipMemcpyHostToDevice));
   grid_dirty_ = true;
   graph_dirty_ = true;
+  results_valid_ = false;
   return RTR_OK;
 }
 
@@ -505,6 +507,7 @@ int Engine::align(const double* guesses, rtr_result* results) {
     set_error("work-queue wait exceeded its bound; results are incomplete");
     return RTR_ERR_DEVICE;
   }
+  results_valid_ = true;
   return RTR_OK;
 }
 
@@ -559,6 +562,10 @@ int Engine::transformed_source(int prob, float* out, size_t stride) {
 
 int Engine::fitness(double max_range, double* out) {
   RTR_HIP(hipSetDevice(device_));
+  if (!results_valid_) {
+    set_error("fitness needs a completed align of the current clouds and problems");
+    return RTR_ERR_NOT_READY;
+  }
   RTR_TRY(out_f32_.ensure(pt_total_ * 4));
   launch_fitness_d2(dev_view(), out_f32_.as<float>(), stream_);
   RTR_HIP(hipGetLastError());
@@ -578,6 +585,33 @@ int Engine::fitness(double max_range, double* out) {
       }
     }
     out[p] = m ? s / (double)m : DBL_MAX;
+  }
+  return RTR_OK;
+}
+
+int Engine::judge(int k, int max_pts, double* plane_rmse, double* inlier_frac) {
+  RTR_HIP(hipSetDevice(device_));
+  if (!results_valid_) {
+    set_error("judge needs a completed align of the current clouds and problems");
+    return RTR_ERR_NOT_READY;
+  }
+  if (k < 3 || k > 32 || max_pts < 1) {
+    set_error("judge: k must be in [3, 32] and max_pts >= 1 (got %d, %d)", k, max_pts);
+    return RTR_ERR_INVALID_ARG;
+  }
+  const size_t np = probs_.size();
+  if (np == 0) return RTR_OK;
+  RTR_TRY(out_f32_.ensure(np * 3 * sizeof(double)));
+  DevEngine e = dev_view();
+  launch_judge(e, k, max_pts, e.prm.r2f, reinterpret_cast<double*>(out_f32_.p), stream_);
+  RTR_HIP(hipGetLastError());
+  std::vector<double> h(np * 3);
+  RTR_HIP(hipMemcpyAsync(h.data(), out_f32_.p, np * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  for (size_t p = 0; p < np; ++p) {
+    const double s = h[3 * p], c = h[3 * p + 1], m = h[3 * p + 2];
+    if (plane_rmse) plane_rmse[p] = c > 0 ? std::sqrt(s / c) : DBL_MAX;
+    if (inlier_frac) inlier_frac[p] = m > 0 ? c / m : 0.0;
   }
   return RTR_OK;
 }

@@ -67,6 +67,7 @@ class Engine {
   int correspondences(int prob, const double T[16], double max_dist, int32_t* idx, float* d2);
   int transformed_source(int prob, float* out, size_t stride);
   int fitness(double max_range, double* out);
+  int judge(int k, int max_pts, double* plane_rmse, double* inlier_frac);
   int grid_info(int tgt, float* cell, int dims[3], float anchor[3]);
   int history(int prob, float* rows12, int cap, int* n_out);
   int target_normals(int tgt, float* out);
@@ -94,6 +95,7 @@ class Engine {
   bool profiling_ = false;
   bool rebuild_always_ = true;
   bool grid_dirty_ = true;
+  bool results_valid_ = false;  // states/grids hold a completed align of the current layout
   bool normals_valid_ = false;
   int cu_count_ = 0;
   rtr_params prm_;

@@ -1,0 +1,927 @@
+// Learned pose proposer: farthest-point sampling, a shared point MLP
+// 3-64-128-256 with max-pooling per cloud, and a head 512-256-9 (6D rotation +
+// translation) over the concatenated source/target features.  Every matrix
+// product runs in one hand-written BF16 MFMA kernel (v_mfma_f32_32x32x8bf16_1k,
+// FP32 accumulation); forward and backward are compositions of that kernel
+// and small elementwise kernels.  Trained with Adam on poses that the GPU ICP
+// converges to.
+//
+// The reference's counterpart is its hand-crafted coarse alignment (keypoints,
+// plane areas, TDF yaw search and exhaustive RANSAC, RealTimeRobot.cpp:45-105,
+// function.h:35-109): both produce the initial pose that ICP refines.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rtr.h"
+#include "engine.hpp"
+
+namespace rtr {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
+  // round to nearest even (NaN kept quiet)
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  const uint32_t r = 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)((u + r) >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+constexpr int kP = 1024;      // points per cloud after FPS
+constexpr int kC0 = 16;       // input channels (xyz padded to the MFMA K granularity)
+constexpr int kC1 = 64, kC2 = 128, kC3 = 256, kH = 256, kOut = 9;
+constexpr int kFpsMax = 8192; // stride pre-subsample before FPS
+
+// ---------------------------------------------------------------------------
+// GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N] (+ bias[N]) (ReLU)
+//   A(i,k) = ta ? A[k*lda + i] : A[i*lda + k]   (bf16)
+//   B(k,j) = tb ? B[j*ldb + k] : B[k*ldb + j]   (bf16)
+// Output FP32 (atomicAdd when split-K) or BF16.  64 x 64 tile per workgroup
+// (2 x 2 waves of 32 x 32 MFMA tiles), K staged through LDS 32 at a time.
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+  const uint16_t* A;
+  const uint16_t* B;
+  void* C;
+  const float* bias;
+  int M, N, K, lda, ldb, ldc;
+  int ta, tb, relu, out_bf16, atomic_out, k_split;
+};
+
+constexpr int kTM = 64, kTN = 64, kTK = 32;
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
+  // rows padded by 4 elements (8 B): 8-byte aligned bf16x4 operand reads
+  __shared__ __align__(16) uint16_t sA[kTM][kTK + 4];
+  __shared__ __align__(16) uint16_t sBt[kTN][kTK + 4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
+  // split-K range
+  const int kchunk = ((g.K + g.k_split - 1) / g.k_split + kTK - 1) / kTK * kTK;
+  const int kb = blockIdx.z * kchunk;
+  const int ke = min(g.K, kb + kchunk);
+  f32x16 acc;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+  for (int k0 = kb; k0 < ke; k0 += kTK) {
+    // stage A (64 x 32) and B (32 x 64, stored transposed): 2048 elements
+    // each, 8 per thread; consecutive threads walk the operand's contiguous
+    // dimension
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + e * 256;
+      int r, c;  // A tile row r (M), column c (K)
+      if (g.ta) {
+        r = idx % kTM;
+        c = idx / kTM;
+      } else {
+        r = idx / kTK;
+        c = idx % kTK;
+      }
+      const int gi = m0 + r, gk = k0 + c;
+      uint16_t v = 0;
+      if (gi < g.M && gk < ke) v = g.ta ? g.A[(size_t)gk * g.lda + gi] : g.A[(size_t)gi * g.lda + gk];
+      sA[r][c] = v;
+      int rk, cn;  // B tile row rk (K), column cn (N)
+      if (g.tb) {
+        rk = idx % kTK;
+        cn = idx / kTK;
+      } else {
+        rk = idx / kTN;
+        cn = idx % kTN;
+      }
+      const int gk2 = k0 + rk, gj = n0 + cn;
+      uint16_t u = 0;
+      if (gk2 < ke && gj < g.N) u = g.tb ? g.B[(size_t)gj * g.ldb + gk2] : g.B[(size_t)gk2 * g.ldb + gj];
+      sBt[cn][rk] = u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk += 8) {
+      const int kq = kk + (lane >> 5) * 4;
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&sA[wm * 32 + (lane & 31)][kq]);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(&sBt[wn * 32 + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // epilogue: lane l, register j -> row (j/4)*8 + (l/32)*4 + j%4, column l%32
+  const int col = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int row = m0 + wm * 32 + (j >> 2) * 8 + (lane >> 5) * 4 + (j & 3);
+    if (row >= g.M || col >= g.N) continue;
+    float v = acc[j];
+    if (g.bias && blockIdx.z == 0) v += g.bias[col];
+    if (g.relu) v = v > 0.f ? v : 0.f;
+    const size_t o = (size_t)row * g.ldc + col;
+    if (g.out_bf16)
+      reinterpret_cast<uint16_t*>(g.C)[o] = f2bf(v);
+    else if (g.atomic_out)
+      atomicAdd(reinterpret_cast<float*>(g.C) + o, v);
+    else
+      reinterpret_cast<float*>(g.C)[o] = v;
+  }
+}
+
+static void gemm(hipStream_t s, const uint16_t* A, int lda, int ta, const uint16_t* B, int ldb, int tb, void* C,
+                 int ldc, int M, int N, int K, const float* bias, int relu, int out_bf16, int k_split = 1) {
+  GemmArgs g;
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.bias = bias;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.ldc = ldc;
+  g.ta = ta;
+  g.tb = tb;
+  g.relu = relu;
+  g.out_bf16 = out_bf16;
+  g.atomic_out = k_split > 1;
+  g.k_split = k_split;
+  dim3 grid((M + kTM - 1) / kTM, (N + kTN - 1) / kTN, k_split);
+  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
+}
+
+// ---------------------------------------------------------------------------
+// farthest-point sampling
+// ---------------------------------------------------------------------------
+
+// One workgroup per cloud.  The cloud is first subsampled with a fixed stride
+// to at most kFpsMax points; its centroid is removed; then kP points are
+// chosen greedily, starting at subsample index 0, each time the point with
+// the largest distance to the chosen set (lowest index on ties).
+__global__ __launch_bounds__(256) void k_fps(const float* raw, const uint32_t* off, const uint32_t* cnt,
+                                             float* pts_out /* cloud * kP * 3 */, uint16_t* x0 /* rows of kC0 */,
+                                             float* centroid /* cloud * 3 */, int32_t* sel_out /* cloud*kP */) {
+  const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t n = cnt[c];
+  const float* p = raw + 3 * (size_t)off[c];
+  const int m = (int)min((uint32_t)kFpsMax, n);
+  __shared__ float sx[kFpsMax], sy[kFpsMax], sz[kFpsMax];
+  __shared__ float red_v[4];
+  __shared__ int red_i[4];
+  __shared__ double cen[4][3];
+  double ax = 0, ay = 0, az = 0;
+  for (int i = tid; i < m; i += 256) {
+    const size_t src = (size_t)((uint64_t)i * n / (uint64_t)m);
+    sx[i] = p[3 * src];
+    sy[i] = p[3 * src + 1];
+    sz[i] = p[3 * src + 2];
+    ax += sx[i];
+    ay += sy[i];
+    az += sz[i];
+  }
+#pragma unroll
+  for (int off2 = 32; off2 > 0; off2 >>= 1) {
+    ax += __shfl_xor(ax, off2, 64);
+    ay += __shfl_xor(ay, off2, 64);
+    az += __shfl_xor(az, off2, 64);
+  }
+  if (lane == 0) {
+    cen[w][0] = ax;
+    cen[w][1] = ay;
+    cen[w][2] = az;
+  }
+  __syncthreads();
+  const float cx = m > 0 ? (float)((cen[0][0] + cen[1][0] + cen[2][0] + cen[3][0]) / m) : 0.f;
+  const float cy = m > 0 ? (float)((cen[0][1] + cen[1][1] + cen[2][1] + cen[3][1]) / m) : 0.f;
+  const float cz = m > 0 ? (float)((cen[0][2] + cen[1][2] + cen[2][2] + cen[3][2]) / m) : 0.f;
+  if (tid == 0) {
+    centroid[3 * c] = cx;
+    centroid[3 * c + 1] = cy;
+    centroid[3 * c + 2] = cz;
+  }
+  constexpr int PER = kFpsMax / 256;
+  float dist[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) dist[k] = 3.0e38f;
+  int cur = 0;
+  for (int s = 0; s < kP; ++s) {
+    const float qx = m > 0 ? sx[cur] : 0.f, qy = m > 0 ? sy[cur] : 0.f, qz = m > 0 ? sz[cur] : 0.f;
+    if (tid == 0) {
+      const size_t o = ((size_t)c * kP + s);
+      pts_out[3 * o] = qx - cx;
+      pts_out[3 * o + 1] = qy - cy;
+      pts_out[3 * o + 2] = qz - cz;
+      if (sel_out) sel_out[o] = cur;
+    }
+    if (tid < kC0) {
+      const size_t o = ((size_t)c * kP + s) * kC0 + tid;
+      const float v = tid == 0 ? qx - cx : (tid == 1 ? qy - cy : (tid == 2 ? qz - cz : 0.f));
+      x0[o] = f2bf(v);
+    }
+    float bv = -1.f;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = tid + k * 256;
+      if (i < m) {
+        const float dx = sx[i] - qx, dy = sy[i] - qy, dz = sz[i] - qz;
+        float d = dx * dx;
+        d = d + dy * dy;
+        d = d + dz * dz;
+        dist[k] = fminf(dist[k], d);
+        if (dist[k] > bv || (dist[k] == bv && i < bi)) {
+          bv = dist[k];
+          bi = i;
+        }
+      }
+    }
+#pragma unroll
+    for (int off2 = 32; off2 > 0; off2 >>= 1) {
+      const float ov = __shfl_xor(bv, off2, 64);
+      const int oi = __shfl_xor(bi, off2, 64);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      red_v[w] = bv;
+      red_i[w] = bi;
+    }
+    __syncthreads();
+    float gv = red_v[0];
+    int gi = red_i[0];
+    for (int k = 1; k < 4; ++k)
+      if (red_v[k] > gv || (red_v[k] == gv && red_i[k] < gi)) {
+        gv = red_v[k];
+        gi = red_i[k];
+      }
+    __syncthreads();
+    cur = (gi >= 0 && gi < m) ? gi : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pooling, head, losses, optimiser
+// ---------------------------------------------------------------------------
+
+// Max over the kP points of each cloud, per channel; argmax = lowest row.
+__global__ void k_maxpool(const uint16_t* Y, int C, int n_clouds, float* G, int32_t* arg) {
+  const int c = blockIdx.x, ch = threadIdx.x + blockIdx.y * blockDim.x;
+  if (ch >= C) return;
+  const uint16_t* y = Y + (size_t)c * kP * C;
+  float best = -3.0e38f;
+  int bi = 0;
+  for (int r = 0; r < kP; ++r) {
+    const float v = bf2f(y[(size_t)r * C + ch]);
+    if (v > best) {
+      best = v;
+      bi = r;
+    }
+  }
+  G[(size_t)c * C + ch] = best;
+  arg[(size_t)c * C + ch] = bi;
+}
+
+// F[pair] = (G[src cloud], G[tgt cloud]) as bf16; clouds are ordered
+// (src 0, tgt 0, src 1, tgt 1, ...).
+__global__ void k_concat(const float* G, int n_pairs, uint16_t* F) {
+  const int p = blockIdx.x, ch = threadIdx.x;  // 512 threads
+  if (p >= n_pairs) return;
+  const float v = ch < kC3 ? G[(size_t)(2 * p) * kC3 + ch] : G[(size_t)(2 * p + 1) * kC3 + (ch - kC3)];
+  F[(size_t)p * (2 * kC3) + ch] = f2bf(v);
+}
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// 6D -> rotation by Gram-Schmidt (columns b1, b2, b3 = b1 x b2), translation
+// o[6..8].  Loss per pair: mean over the source's FPS points s of
+// ||(R - R*) s + (t - t*)||^2 = tr(dR S dR^T) + 2 dt^T dR m + |dt|^2 with
+// S = mean s s^T, m = mean s.  Writes R, t (poses) and, when labels are
+// given, the loss and dL/dO.
+__global__ void k_head(const float* O, int n_pairs, const float* pts /* cloud*kP*3 */, const double* lab /* n*12 */,
+                       double* pose /* n*12 (R row-major, t) */, float* dO, float* loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const float* o = O + (size_t)p * kOut;
+  double a1[3] = {o[0], o[1], o[2]}, a2[3] = {o[3], o[4], o[5]};
+  const double n1 = sqrt(dot3(a1, a1)) + 1e-12;
+  double b1[3] = {a1[0] / n1, a1[1] / n1, a1[2] / n1};
+  const double c = dot3(b1, a2);
+  double u2[3] = {a2[0] - c * b1[0], a2[1] - c * b1[1], a2[2] - c * b1[2]};
+  const double n2 = sqrt(dot3(u2, u2)) + 1e-12;
+  double b2[3] = {u2[0] / n2, u2[1] / n2, u2[2] / n2}, b3[3];
+  cross3(b1, b2, b3);
+  double R[9];  // row-major, columns b1 b2 b3
+  for (int r = 0; r < 3; ++r) {
+    R[r * 3] = b1[r];
+    R[r * 3 + 1] = b2[r];
+    R[r * 3 + 2] = b3[r];
+  }
+  const double t[3] = {o[6], o[7], o[8]};
+  for (int k = 0; k < 9; ++k) pose[(size_t)p * 12 + k] = R[k];
+  for (int k = 0; k < 3; ++k) pose[(size_t)p * 12 + 9 + k] = t[k];
+  if (!lab) return;
+  // moments of the source FPS points (cloud 2p)
+  double S[9] = {0}, mm[3] = {0};
+  const float* sp = pts + (size_t)(2 * p) * kP * 3;
+  for (int i = 0; i < kP; ++i) {
+    const double s[3] = {sp[3 * i], sp[3 * i + 1], sp[3 * i + 2]};
+    for (int a = 0; a < 3; ++a) {
+      mm[a] += s[a];
+      for (int b = 0; b < 3; ++b) S[a * 3 + b] += s[a] * s[b];
+    }
+  }
+  for (int a = 0; a < 3; ++a) mm[a] /= kP;
+  for (int k = 0; k < 9; ++k) S[k] /= kP;
+  const double* L = lab + (size_t)p * 12;
+  double dR[9], dt[3];
+  for (int k = 0; k < 9; ++k) dR[k] = R[k] - L[k];
+  for (int k = 0; k < 3; ++k) dt[k] = t[k] - L[9 + k];
+  double l = 0;
+  for (int r = 0; r < 3; ++r)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) l += dR[r * 3 + a] * S[a * 3 + b] * dR[r * 3 + b];
+  double dRm[3];
+  for (int r = 0; r < 3; ++r) dRm[r] = dR[r * 3] * mm[0] + dR[r * 3 + 1] * mm[1] + dR[r * 3 + 2] * mm[2];
+  l += 2.0 * dot3(dt, dRm) + dot3(dt, dt);
+  loss[p] = (float)l;
+  // gradients (the batch loss is the mean over pairs)
+  const double inv = 1.0 / n_pairs;
+  double gR[9];
+  for (int r = 0; r < 3; ++r)
+    for (int a = 0; a < 3; ++a) {
+      double v = 0;
+      for (int b = 0; b < 3; ++b) v += dR[r * 3 + b] * S[b * 3 + a];
+      gR[r * 3 + a] = (2.0 * v + 2.0 * dt[r] * mm[a]) * inv;
+    }
+  double gt[3];
+  for (int r = 0; r < 3; ++r) gt[r] = (2.0 * dt[r] + 2.0 * dRm[r]) * inv;
+  // back through Gram-Schmidt: columns of gR are dL/db1, dL/db2, dL/db3
+  double g1[3] = {gR[0], gR[3], gR[6]}, g2[3] = {gR[1], gR[4], gR[7]}, g3[3] = {gR[2], gR[5], gR[8]}, tmp[3];
+  cross3(b2, g3, tmp);  // b3 = b1 x b2
+  for (int a = 0; a < 3; ++a) g1[a] += tmp[a];
+  cross3(g3, b1, tmp);
+  for (int a = 0; a < 3; ++a) g2[a] += tmp[a];
+  const double gb2 = dot3(b2, g2);
+  double dU[3];
+  for (int a = 0; a < 3; ++a) dU[a] = (g2[a] - b2[a] * gb2) / n2;
+  const double dUb1 = dot3(dU, b1);
+  double da2[3];
+  for (int a = 0; a < 3; ++a) {
+    da2[a] = dU[a] - dUb1 * b1[a];
+    g1[a] += -dUb1 * a2[a] - c * dU[a];
+  }
+  const double gb1 = dot3(b1, g1);
+  float* d = dO + (size_t)p * kOut;
+  for (int a = 0; a < 3; ++a) {
+    d[a] = (float)((g1[a] - b1[a] * gb1) / n1);
+    d[3 + a] = (float)da2[a];
+    d[6 + a] = (float)gt[a];
+  }
+}
+
+// dY(pre-activation) = dY(post) * (Y > 0), as bf16 for the next product.
+__global__ void k_relu_back(const float* dY, const uint16_t* Y, size_t n, uint16_t* out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = bf2f(Y[i]) > 0.f ? f2bf(dY[i]) : (uint16_t)0;
+}
+
+// Max-pool + ReLU backward fused: the gradient of the pre-activation of
+// layer 3 is dG at each channel's argmax row when the pooled value is
+// positive (ReLU passes it), zero elsewhere; written as bf16.
+__global__ void k_pool_back(const float* dG /* cloud x C */, const int32_t* arg, const float* G, int C,
+                            size_t rows, uint16_t* out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows * C; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t row = i / C;
+    const int ch = (int)(i - row * C);
+    const size_t cl = row / kP;
+    const size_t gi = cl * C + ch;
+    const bool hit = arg[gi] == (int32_t)(row - cl * kP) && G[gi] > 0.f;
+    out[i] = hit ? f2bf(dG[gi]) : (uint16_t)0;
+  }
+}
+
+// dF (pairs x 512) -> dG (clouds x 256)
+__global__ void k_split(const float* dF, int n_pairs, float* dG) {
+  const int p = blockIdx.x, ch = threadIdx.x;
+  if (p >= n_pairs) return;
+  const float v = dF[(size_t)p * (2 * kC3) + ch];
+  if (ch < kC3)
+    dG[(size_t)(2 * p) * kC3 + ch] = v;
+  else
+    dG[(size_t)(2 * p + 1) * kC3 + (ch - kC3)] = v;
+}
+
+// column sums of a bf16 matrix (bias gradients), split over row chunks
+__global__ void k_colsum(const uint16_t* Y, size_t rows, int C, float* out) {
+  const int ch = threadIdx.x + blockIdx.y * blockDim.x;
+  if (ch >= C) return;
+  const size_t chunk = (rows + gridDim.x - 1) / gridDim.x;
+  const size_t r0 = blockIdx.x * chunk, r1 = min(rows, r0 + chunk);
+  float s = 0.f;
+  for (size_t r = r0; r < r1; ++r) s += bf2f(Y[r * C + ch]);
+  atomicAdd(out + ch, s);
+}
+__global__ void k_colsum_f32(const float* Y, size_t rows, int C, float* out) {
+  const int ch = threadIdx.x + blockIdx.y * blockDim.x;
+  if (ch >= C) return;
+  const size_t chunk = (rows + gridDim.x - 1) / gridDim.x;
+  const size_t r0 = blockIdx.x * chunk, r1 = min(rows, r0 + chunk);
+  float s = 0.f;
+  for (size_t r = r0; r < r1; ++r) s += Y[r * C + ch];
+  atomicAdd(out + ch, s);
+}
+
+__global__ void k_to_bf16(const float* in, size_t n, uint16_t* out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = f2bf(in[i]);
+}
+
+// xyz rows (FP32) -> network input rows of kC0 bf16 (channels 3.. zero)
+__global__ void k_pack_x0(const float* xyz, size_t rows, uint16_t* x0) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows * kC0; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / kC0;
+    const int c = (int)(i - r * kC0);
+    x0[i] = c < 3 ? f2bf(xyz[3 * r + c]) : (uint16_t)0;
+  }
+}
+
+__global__ void k_zero(float* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes the BF16 copy.
+__global__ void k_adam(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, uint16_t* wb) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    wb[i] = f2bf(wi);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+// parameter layout (FP32 master and BF16 copy share offsets)
+struct ParamLayout {
+  size_t W1, b1, W2, b2, W3, b3, W4, b4, W5, b5, total;
+  ParamLayout() {
+    size_t o = 0;
+    W1 = o; o += (size_t)kC0 * kC1;
+    b1 = o; o += kC1;
+    W2 = o; o += (size_t)kC1 * kC2;
+    b2 = o; o += kC2;
+    W3 = o; o += (size_t)kC2 * kC3;
+    b3 = o; o += kC3;
+    W4 = o; o += (size_t)2 * kC3 * kH;
+    b4 = o; o += kH;
+    W5 = o; o += (size_t)kH * kOut;
+    b5 = o; o += kOut;
+    total = o;
+  }
+};
+
+struct Proposer {
+  int device = 0, max_pairs = 0, n_pairs = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ParamLayout L;
+  int64_t step = 0;
+  DevBuf w, wb, g, gb_unused, m, v;
+  DevBuf raw, roff, rcnt;           // resident raw clouds for proposals
+  DevBuf pts, x0, cen, sel;         // FPS outputs
+  DevBuf y1, y2, y3, G, arg, F, h1, O, pose, lab, dO, loss;
+  DevBuf dtmp, dbf, dG, dF, dH1, dH1b, dOb, dbias;
+  std::vector<uint32_t> h_off, h_cnt;
+  float last_ms = 0.f;
+
+  int init(int dev, int maxp) {
+    device = dev;
+    max_pairs = maxp;
+    int n = 0;
+    RTR_HIP(hipGetDeviceCount(&n));
+    if (dev < 0 || dev >= n) {
+      set_error("device %d not available", dev);
+      return RTR_ERR_DEVICE;
+    }
+    RTR_HIP(hipSetDevice(device));
+    RTR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    RTR_HIP(hipEventCreate(&ev0));
+    RTR_HIP(hipEventCreate(&ev1));
+    const size_t P = L.total;
+    const size_t clouds = 2 * (size_t)maxp, rows = clouds * kP;
+    int rc = 0;
+    DevBuf* bufs[] = {&w, &wb, &g, &m, &v};
+    const size_t sz[] = {P * 4, P * 2, P * 4, P * 4, P * 4};
+    for (int i = 0; i < 5 && !rc; ++i) rc = bufs[i]->ensure(sz[i]);
+    if (!rc) rc = pts.ensure(rows * 12);
+    if (!rc) rc = x0.ensure(rows * kC0 * 2);
+    if (!rc) rc = cen.ensure(clouds * 12);
+    if (!rc) rc = sel.ensure(rows * 4);
+    if (!rc) rc = y1.ensure(rows * kC1 * 2);
+    if (!rc) rc = y2.ensure(rows * kC2 * 2);
+    if (!rc) rc = y3.ensure(rows * kC3 * 2);
+    if (!rc) rc = G.ensure(clouds * kC3 * 4);
+    if (!rc) rc = arg.ensure(clouds * kC3 * 4);
+    if (!rc) rc = F.ensure((size_t)maxp * 2 * kC3 * 2);
+    if (!rc) rc = h1.ensure((size_t)maxp * kH * 2);
+    if (!rc) rc = O.ensure((size_t)maxp * kOut * 4);
+    if (!rc) rc = pose.ensure((size_t)maxp * 12 * 8);
+    if (!rc) rc = lab.ensure((size_t)maxp * 12 * 8);
+    if (!rc) rc = dO.ensure((size_t)maxp * kOut * 4);
+    if (!rc) rc = loss.ensure((size_t)maxp * 4);
+    if (!rc) rc = dtmp.ensure(rows * kC3 * 4);
+    if (!rc) rc = dbf.ensure(rows * kC3 * 2);
+    if (!rc) rc = dG.ensure(clouds * kC3 * 4);
+    if (!rc) rc = dF.ensure((size_t)maxp * 2 * kC3 * 4);
+    if (!rc) rc = dH1.ensure((size_t)maxp * kH * 4);
+    if (!rc) rc = dH1b.ensure((size_t)maxp * kH * 2);
+    if (!rc) rc = dOb.ensure((size_t)maxp * 16 * 2);
+    if (rc) return rc;
+    RTR_HIP(hipMemset(m.p, 0, P * 4));
+    RTR_HIP(hipMemset(v.p, 0, P * 4));
+    return RTR_OK;
+  }
+
+  ~Proposer() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    DevBuf* all[] = {&w, &wb, &g, &m, &v, &raw, &roff, &rcnt, &pts, &x0, &cen, &sel, &y1, &y2, &y3, &G,
+                     &arg, &F, &h1, &O, &pose, &lab, &dO, &loss, &dtmp, &dbf, &dG, &dF, &dH1, &dH1b, &dOb};
+    for (DevBuf* b : all) b->release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int set_weights(const float* host) {
+    RTR_HIP(hipSetDevice(device));
+    RTR_HIP(hipMemcpy(w.p, host, L.total * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, stream, w.as<float>(), L.total, wb.as<uint16_t>());
+    RTR_HIP(hipGetLastError());
+    RTR_HIP(hipStreamSynchronize(stream));
+    return RTR_OK;
+  }
+  int get_weights(float* host) {
+    RTR_HIP(hipSetDevice(device));
+    RTR_HIP(hipMemcpy(host, w.p, L.total * 4, hipMemcpyDeviceToHost));
+    return RTR_OK;
+  }
+
+  int set_clouds(int np, const float* const* src, const size_t* ns, const float* const* tgt, const size_t* nt) {
+    RTR_HIP(hipSetDevice(device));
+    if (np < 1 || np > max_pairs) {
+      set_error("pairs %d outside [1, %d]", np, max_pairs);
+      return RTR_ERR_INVALID_ARG;
+    }
+    n_pairs = np;
+    h_off.assign(2 * np, 0);
+    h_cnt.assign(2 * np, 0);
+    size_t tot = 0;
+    for (int p = 0; p < np; ++p) {
+      h_off[2 * p] = (uint32_t)tot;
+      h_cnt[2 * p] = (uint32_t)ns[p];
+      tot += ns[p];
+      h_off[2 * p + 1] = (uint32_t)tot;
+      h_cnt[2 * p + 1] = (uint32_t)nt[p];
+      tot += nt[p];
+    }
+    int rc = raw.ensure(tot * 12);
+    if (!rc) rc = roff.ensure(2 * np * 4);
+    if (!rc) rc = rcnt.ensure(2 * np * 4);
+    if (rc) return rc;
+    for (int p = 0; p < np; ++p) {
+      RTR_HIP(hipMemcpyAsync(raw.as<float>() + 3 * (size_t)h_off[2 * p], src[p], ns[p] * 12,
+                             hipMemcpyHostToDevice, stream));
+      RTR_HIP(hipMemcpyAsync(raw.as<float>() + 3 * (size_t)h_off[2 * p + 1], tgt[p], nt[p] * 12,
+                             hipMemcpyHostToDevice, stream));
+    }
+    RTR_HIP(hipMemcpyAsync(roff.p, h_off.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
+    RTR_HIP(hipMemcpyAsync(rcnt.p, h_cnt.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
+    RTR_HIP(hipStreamSynchronize(stream));
+    return RTR_OK;
+  }
+
+  void enqueue_fps() {
+    hipLaunchKernelGGL(k_fps, dim3(2 * n_pairs), dim3(256), 0, stream, raw.as<float>(), roff.as<uint32_t>(),
+                       rcnt.as<uint32_t>(), pts.as<float>(), x0.as<uint16_t>(), cen.as<float>(),
+                       sel.as<int32_t>());
+  }
+
+  // FPS points given directly (training): centred xyz, n_pairs * 2 * kP
+  int set_points(int np, const float* xyz) {
+    RTR_HIP(hipSetDevice(device));
+    if (np < 1 || np > max_pairs) {
+      set_error("pairs %d outside [1, %d]", np, max_pairs);
+      return RTR_ERR_INVALID_ARG;
+    }
+    n_pairs = np;
+    const size_t rows = (size_t)2 * np * kP;
+    RTR_HIP(hipMemcpyAsync(pts.p, xyz, rows * 12, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_pack_x0, dim3(1024), dim3(256), 0, stream, pts.as<float>(), rows, x0.as<uint16_t>());
+    RTR_HIP(hipGetLastError());
+    return RTR_OK;
+  }
+
+  const uint16_t* wbp(size_t off) const { return wb.as<uint16_t>() + off; }
+  const float* wp(size_t off) const { return w.as<float>() + off; }
+
+  void enqueue_forward(bool with_labels) {
+    const int B = n_pairs, C = 2 * B;
+    const int rows = C * kP;
+    gemm(stream, x0.as<uint16_t>(), kC0, 0, wbp(L.W1), kC1, 0, y1.p, kC1, rows, kC1, kC0, wp(L.b1), 1, 1);
+    gemm(stream, y1.as<uint16_t>(), kC1, 0, wbp(L.W2), kC2, 0, y2.p, kC2, rows, kC2, kC1, wp(L.b2), 1, 1);
+    gemm(stream, y2.as<uint16_t>(), kC2, 0, wbp(L.W3), kC3, 0, y3.p, kC3, rows, kC3, kC2, wp(L.b3), 1, 1);
+    hipLaunchKernelGGL(k_maxpool, dim3(C, 1), dim3(kC3), 0, stream, y3.as<uint16_t>(), kC3, C, G.as<float>(),
+                       arg.as<int32_t>());
+    hipLaunchKernelGGL(k_concat, dim3(B), dim3(2 * kC3), 0, stream, G.as<float>(), B, F.as<uint16_t>());
+    gemm(stream, F.as<uint16_t>(), 2 * kC3, 0, wbp(L.W4), kH, 0, h1.p, kH, B, kH, 2 * kC3, wp(L.b4), 1, 1);
+    gemm(stream, h1.as<uint16_t>(), kH, 0, wbp(L.W5), kOut, 0, O.p, kOut, B, kOut, kH, wp(L.b5), 0, 0);
+    hipLaunchKernelGGL(k_head, dim3((B + 63) / 64), dim3(64), 0, stream, O.as<float>(), B, pts.as<float>(),
+                       with_labels ? lab.as<double>() : nullptr, pose.as<double>(), dO.as<float>(),
+                       loss.as<float>());
+  }
+
+  // gradients of the mean loss into g (FP32)
+  void enqueue_backward() {
+    const int B = n_pairs, C = 2 * B;
+    const size_t rows = (size_t)C * kP;
+    float* gp = g.as<float>();
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, stream, gp, L.total);
+    const unsigned cs_blocks = 64;
+    // head layer 2: O = H1 W5 + b5
+    hipLaunchKernelGGL(k_to_bf16, dim3(64), dim3(256), 0, stream, dO.as<float>(), (size_t)B * kOut,
+                       dOb.as<uint16_t>());
+    gemm(stream, h1.as<uint16_t>(), kH, 1, dOb.as<uint16_t>(), kOut, 0, gp + L.W5, kOut, kH, kOut, B, nullptr, 0, 0);
+    hipLaunchKernelGGL(k_colsum_f32, dim3(1, 1), dim3(64), 0, stream, dO.as<float>(), (size_t)B, kOut, gp + L.b5);
+    gemm(stream, dOb.as<uint16_t>(), kOut, 0, wbp(L.W5), kOut, 1, dH1.p, kH, B, kH, kOut, nullptr, 0, 0);
+    hipLaunchKernelGGL(k_relu_back, dim3(64), dim3(256), 0, stream, dH1.as<float>(), h1.as<uint16_t>(),
+                       (size_t)B * kH, dH1b.as<uint16_t>());
+    // head layer 1: H1 = relu(F W4 + b4)
+    gemm(stream, F.as<uint16_t>(), 2 * kC3, 1, dH1b.as<uint16_t>(), kH, 0, gp + L.W4, kH, 2 * kC3, kH, B, nullptr,
+         0, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(1, 1), dim3(kH), 0, stream, dH1b.as<uint16_t>(), (size_t)B, kH, gp + L.b4);
+    gemm(stream, dH1b.as<uint16_t>(), kH, 0, wbp(L.W4), kH, 1, dF.p, 2 * kC3, B, 2 * kC3, kH, nullptr, 0, 0);
+    hipLaunchKernelGGL(k_split, dim3(B), dim3(2 * kC3), 0, stream, dF.as<float>(), B, dG.as<float>());
+    // max-pool + ReLU: dY3 nonzero at the argmax rows only
+    hipLaunchKernelGGL(k_pool_back, dim3(2048), dim3(256), 0, stream, dG.as<float>(), arg.as<int32_t>(),
+                       G.as<float>(), kC3, rows, dbf.as<uint16_t>());
+    const int ks = (int)std::min<size_t>(256, rows / 1024 + 1);
+    // layer 3: Y3 = relu(Y2 W3 + b3)
+    gemm(stream, y2.as<uint16_t>(), kC2, 1, dbf.as<uint16_t>(), kC3, 0, gp + L.W3, kC3, kC2, kC3, (int)rows, nullptr,
+         0, 0, ks);
+    hipLaunchKernelGGL(k_colsum, dim3(cs_blocks, 1), dim3(kC3), 0, stream, dbf.as<uint16_t>(), rows, kC3, gp + L.b3);
+    gemm(stream, dbf.as<uint16_t>(), kC3, 0, wbp(L.W3), kC3, 1, dtmp.p, kC2, (int)rows, kC2, kC3, nullptr, 0, 0);
+    hipLaunchKernelGGL(k_relu_back, dim3(1024), dim3(256), 0, stream, dtmp.as<float>(), y2.as<uint16_t>(),
+                       rows * kC2, dbf.as<uint16_t>());
+    // layer 2
+    gemm(stream, y1.as<uint16_t>(), kC1, 1, dbf.as<uint16_t>(), kC2, 0, gp + L.W2, kC2, kC1, kC2, (int)rows, nullptr,
+         0, 0, ks);
+    hipLaunchKernelGGL(k_colsum, dim3(cs_blocks, 1), dim3(kC2), 0, stream, dbf.as<uint16_t>(), rows, kC2, gp + L.b2);
+    gemm(stream, dbf.as<uint16_t>(), kC2, 0, wbp(L.W2), kC2, 1, dtmp.p, kC1, (int)rows, kC1, kC2, nullptr, 0, 0);
+    hipLaunchKernelGGL(k_relu_back, dim3(1024), dim3(256), 0, stream, dtmp.as<float>(), y1.as<uint16_t>(),
+                       rows * kC1, dbf.as<uint16_t>());
+    // layer 1
+    gemm(stream, x0.as<uint16_t>(), kC0, 1, dbf.as<uint16_t>(), kC1, 0, gp + L.W1, kC1, kC0, kC1, (int)rows, nullptr,
+         0, 0, ks);
+    hipLaunchKernelGGL(k_colsum, dim3(cs_blocks, 1), dim3(kC1), 0, stream, dbf.as<uint16_t>(), rows, kC1, gp + L.b1);
+  }
+
+  int upload_labels(const double* labels) {
+    RTR_HIP(hipMemcpyAsync(lab.p, labels, (size_t)n_pairs * 12 * 8, hipMemcpyHostToDevice, stream));
+    return RTR_OK;
+  }
+};
+
+}  // namespace rtr
+
+using rtr::Proposer;
+using rtr::set_error;
+
+struct rtr_prop {
+  Proposer p;
+};
+
+#define PROP_GUARD(x)                     \
+  try {                                   \
+    x                                     \
+  } catch (...) {                         \
+    set_error("host allocation failed");  \
+    return RTR_ERR_CAPACITY;              \
+  }
+
+extern "C" {
+
+int rtr_prop_create(rtr_prop** out, int device, int max_pairs) {
+  PROP_GUARD(if (!out || max_pairs < 1) return RTR_ERR_INVALID_ARG; *out = nullptr; rtr_prop* h = new rtr_prop();
+             int rc = h->p.init(device, max_pairs); if (rc) {
+               delete h;
+               return rc;
+             } *out = h;
+             return RTR_OK;)
+}
+
+int rtr_prop_destroy(rtr_prop* h) {
+  delete h;
+  return RTR_OK;
+}
+
+int rtr_prop_num_params(size_t* n) {
+  if (!n) return RTR_ERR_INVALID_ARG;
+  *n = rtr::ParamLayout().total;
+  return RTR_OK;
+}
+
+int rtr_prop_set_weights(rtr_prop* h, const float* w) {
+  if (!h || !w) return RTR_ERR_INVALID_ARG;
+  return h->p.set_weights(w);
+}
+
+int rtr_prop_get_weights(rtr_prop* h, float* w) {
+  if (!h || !w) return RTR_ERR_INVALID_ARG;
+  return h->p.get_weights(w);
+}
+
+int rtr_prop_set_clouds(rtr_prop* h, int n_pairs, const float* const* src, const size_t* ns,
+                        const float* const* tgt, const size_t* nt) {
+  PROP_GUARD(if (!h || !src || !ns || !tgt || !nt) return RTR_ERR_INVALID_ARG;
+             return h->p.set_clouds(n_pairs, src, ns, tgt, nt);)
+}
+
+// FPS + forward on the resident clouds; poses (n*16 column-major) map each
+// source onto its target.
+int rtr_prop_propose(rtr_prop* h, double* poses) {
+  using namespace rtr;
+  if (!h || !poses) return RTR_ERR_INVALID_ARG;
+  Proposer& P = h->p;
+  RTR_HIP(hipSetDevice(P.device));
+  if (P.n_pairs < 1 || P.h_cnt.empty()) {
+    set_error("no clouds set");
+    return RTR_ERR_NOT_READY;
+  }
+  RTR_HIP(hipEventRecord(P.ev0, P.stream));
+  P.enqueue_fps();
+  P.enqueue_forward(false);
+  RTR_HIP(hipGetLastError());
+  RTR_HIP(hipEventRecord(P.ev1, P.stream));
+  std::vector<double> pose((size_t)P.n_pairs * 12);
+  std::vector<float> cen((size_t)2 * P.n_pairs * 3);
+  RTR_HIP(hipMemcpyAsync(pose.data(), P.pose.p, pose.size() * 8, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipMemcpyAsync(cen.data(), P.cen.p, cen.size() * 4, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipStreamSynchronize(P.stream));
+  RTR_HIP(hipEventElapsedTime(&P.last_ms, P.ev0, P.ev1));
+  for (int p = 0; p < P.n_pairs; ++p) {
+    const double* R = &pose[(size_t)p * 12];
+    const double* t = R + 9;
+    const float* cs = &cen[(size_t)(2 * p) * 3];
+    const float* ct = &cen[(size_t)(2 * p + 1) * 3];
+    double* T = poses + (size_t)p * 16;
+    // T = Tr(ct) [R, t] Tr(-cs)
+    for (int i = 0; i < 16; ++i) T[i] = 0.0;
+    T[15] = 1.0;
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) T[c * 4 + r] = R[r * 3 + c];
+      T[12 + r] = t[r] + ct[r] - (R[r * 3] * cs[0] + R[r * 3 + 1] * cs[1] + R[r * 3 + 2] * cs[2]);
+    }
+  }
+  return RTR_OK;
+}
+
+int rtr_prop_last_ms(rtr_prop* h, float* ms) {
+  if (!h || !ms) return RTR_ERR_INVALID_ARG;
+  *ms = h->p.last_ms;
+  return RTR_OK;
+}
+
+// FPS of the resident clouds: centred points (2*n_pairs*1024*3), centroids
+// (2*n_pairs*3) and selected subsample indices (nullable).
+int rtr_prop_fps(rtr_prop* h, float* pts, float* centroids, int32_t* sel) {
+  using namespace rtr;
+  if (!h || !pts) return RTR_ERR_INVALID_ARG;
+  Proposer& P = h->p;
+  RTR_HIP(hipSetDevice(P.device));
+  if (P.h_cnt.empty()) {
+    set_error("no clouds set");
+    return RTR_ERR_NOT_READY;
+  }
+  P.enqueue_fps();
+  RTR_HIP(hipGetLastError());
+  const size_t rows = (size_t)2 * P.n_pairs * kP;
+  RTR_HIP(hipMemcpyAsync(pts, P.pts.p, rows * 12, hipMemcpyDeviceToHost, P.stream));
+  if (centroids)
+    RTR_HIP(hipMemcpyAsync(centroids, P.cen.p, (size_t)2 * P.n_pairs * 12, hipMemcpyDeviceToHost, P.stream));
+  if (sel) RTR_HIP(hipMemcpyAsync(sel, P.sel.p, rows * 4, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipStreamSynchronize(P.stream));
+  return RTR_OK;
+}
+
+// Forward on given centred FPS points (n_pairs*2*1024*3, clouds src0, tgt0,
+// src1, ...); outputs: raw head outputs O (n*9, nullable), R|t (n*12, nullable),
+// and with labels (n*12: R row-major, t, in the centred frames) the mean loss
+// and the gradient of the mean loss (nullable).
+int rtr_prop_forward_backward(rtr_prop* h, int n_pairs, const float* xyz, const double* labels, float* O_out,
+                              double* rt_out, float* loss_out, float* grad_out) {
+  using namespace rtr;
+  if (!h || !xyz) return RTR_ERR_INVALID_ARG;
+  Proposer& P = h->p;
+  int rc = P.set_points(n_pairs, xyz);
+  if (rc) return rc;
+  if (labels) RTR_HIP(hipMemcpyAsync(P.lab.p, labels, (size_t)n_pairs * 12 * 8, hipMemcpyHostToDevice, P.stream));
+  P.enqueue_forward(labels != nullptr);
+  if (labels && grad_out) P.enqueue_backward();
+  RTR_HIP(hipGetLastError());
+  if (O_out) RTR_HIP(hipMemcpyAsync(O_out, P.O.p, (size_t)n_pairs * kOut * 4, hipMemcpyDeviceToHost, P.stream));
+  if (rt_out) RTR_HIP(hipMemcpyAsync(rt_out, P.pose.p, (size_t)n_pairs * 12 * 8, hipMemcpyDeviceToHost, P.stream));
+  std::vector<float> l(labels ? n_pairs : 0);
+  if (labels) RTR_HIP(hipMemcpyAsync(l.data(), P.loss.p, (size_t)n_pairs * 4, hipMemcpyDeviceToHost, P.stream));
+  if (labels && grad_out) RTR_HIP(hipMemcpyAsync(grad_out, P.g.p, P.L.total * 4, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipStreamSynchronize(P.stream));
+  if (loss_out && labels) {
+    double s = 0;
+    for (float v : l) s += v;
+    *loss_out = (float)(s / n_pairs);
+  }
+  return RTR_OK;
+}
+
+// One Adam step on a batch (same inputs as rtr_prop_forward_backward).
+int rtr_prop_train_step(rtr_prop* h, int n_pairs, const float* xyz, const double* labels, float lr, float* loss_out) {
+  using namespace rtr;
+  if (!h || !xyz || !labels) return RTR_ERR_INVALID_ARG;
+  Proposer& P = h->p;
+  int rc = P.set_points(n_pairs, xyz);
+  if (rc) return rc;
+  RTR_HIP(hipMemcpyAsync(P.lab.p, labels, (size_t)n_pairs * 12 * 8, hipMemcpyHostToDevice, P.stream));
+  P.enqueue_forward(true);
+  P.enqueue_backward();
+  P.step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)P.step), c2 = 1.f - powf(b2, (float)P.step);
+  hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, P.stream, P.w.as<float>(), P.g.as<float>(), P.m.as<float>(),
+                     P.v.as<float>(), P.L.total, lr, b1, b2, 1e-8f, c1, c2, P.wb.as<uint16_t>());
+  RTR_HIP(hipGetLastError());
+  std::vector<float> l(n_pairs);
+  RTR_HIP(hipMemcpyAsync(l.data(), P.loss.p, (size_t)n_pairs * 4, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipStreamSynchronize(P.stream));
+  if (loss_out) {
+    double s = 0;
+    for (float v : l) s += v;
+    *loss_out = (float)(s / n_pairs);
+  }
+  return RTR_OK;
+}
+
+// Plain BF16 MFMA product for tests: C (M x N, FP32) = op(A) op(B), A/B given
+// as FP32 and rounded to BF16 on the host.
+int rtr_gemm_bf16(int device, const float* A, int ta, const float* B, int tb, float* C, int M, int N, int K,
+                  int k_split) {
+  using namespace rtr;
+  if (!A || !B || !C || M < 1 || N < 1 || K < 1 || k_split < 1) return RTR_ERR_INVALID_ARG;
+  RTR_HIP(hipSetDevice(device));
+  std::vector<uint16_t> a((size_t)M * K), b((size_t)K * N);
+  for (size_t i = 0; i < a.size(); ++i) a[i] = f2bf(A[i]);
+  for (size_t i = 0; i < b.size(); ++i) b[i] = f2bf(B[i]);
+  DevBuf da, db, dc;
+  int rc = da.ensure(a.size() * 2);
+  if (!rc) rc = db.ensure(b.size() * 2);
+  if (!rc) rc = dc.ensure((size_t)M * N * 4);
+  if (rc) return rc;
+  RTR_HIP(hipMemcpy(da.p, a.data(), a.size() * 2, hipMemcpyHostToDevice));
+  RTR_HIP(hipMemcpy(db.p, b.data(), b.size() * 2, hipMemcpyHostToDevice));
+  RTR_HIP(hipMemset(dc.p, 0, (size_t)M * N * 4));
+  gemm(nullptr, da.as<uint16_t>(), ta ? M : K, ta, db.as<uint16_t>(), tb ? K : N, tb, dc.p, N, M, N, K, nullptr, 0, 0,
+       k_split);
+  RTR_HIP(hipGetLastError());
+  RTR_HIP(hipMemcpy(C, dc.p, (size_t)M * N * 4, hipMemcpyDeviceToHost));
+  da.release();
+  db.release();
+  dc.release();
+  return RTR_OK;
+}
+
+}  // extern "C"

@@ -51,3 +51,49 @@ def select_best(results, rel_corr: float = 0.98):
     ok = np.where(ncorr >= rel_corr * ncorr.max())[0]
     mse = np.array([results[i]["mse"] for i in ok])
     return int(ok[np.argmin(mse)])
+
+
+def _rodrigues(axes, angles):
+    a = axes / np.linalg.norm(axes, axis=1, keepdims=True)
+    K = np.zeros((len(a), 3, 3))
+    K[:, 0, 1], K[:, 0, 2], K[:, 1, 2] = -a[:, 2], a[:, 1], -a[:, 0]
+    K[:, 1, 0], K[:, 2, 0], K[:, 2, 1] = a[:, 2], -a[:, 1], a[:, 0]
+    s, c = np.sin(angles)[:, None, None], np.cos(angles)[:, None, None]
+    return np.eye(3) + s * K + (1 - c) * (K @ K)
+
+
+def so3_rotations(n: int = 8, pool: int = 4096, seed: int = 12345):
+    """n rotations spread over SO(3): identity, then greedy farthest-point
+    picks (geodesic angle) from a fixed pseudo-random pool of rotations
+    (uniform on SO(3))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    q = rng.normal(size=(pool, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    chosen = [np.array([1.0, 0.0, 0.0, 0.0])]
+    dist = 2 * np.arccos(np.clip(np.abs(q @ chosen[0]), 0, 1))
+    for _ in range(n - 1):
+        i = int(np.argmax(dist))
+        chosen.append(q[i])
+        dist = np.minimum(dist, 2 * np.arccos(np.clip(np.abs(q @ q[i]), 0, 1)))
+    Q = np.array(chosen)
+    w, x, y, z = Q[:, 0], Q[:, 1], Q[:, 2], Q[:, 3]
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+    return R
+
+
+def so3_start_poses(src_centroid, tgt_centroid, n_starts: int = 8):
+    """Starts for arbitrary orientation: the source rotated about its
+    centroid by each of so3_rotations(n_starts), centroid moved onto the
+    target's."""
+    cs = np.asarray(src_centroid, np.float64)
+    ct = np.asarray(tgt_centroid, np.float64)
+    out = []
+    for R in so3_rotations(n_starts):
+        T = np.eye(4)
+        T[:3, :3] = R
+        T[:3, 3] = ct - R @ cs
+        out.append(T)
+    return out

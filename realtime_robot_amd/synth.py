@@ -84,8 +84,10 @@ def apply(T, pts) -> np.ndarray:
     return (p @ T[:3, :3].T + T[:3, 3]).astype(np.float32)
 
 
-def random_composite(rng: np.random.Generator):
-    """A random piece of box-composite furniture (0.4-2.0 m extents)."""
+def random_composite(rng: np.random.Generator, asymmetric: bool = False):
+    """A random piece of box-composite furniture (0.4-2.0 m extents).  With
+    asymmetric=True every table carries its off-centre drawer block, so no
+    piece has a proper rotational symmetry (its pose is identifiable)."""
     kind = rng.integers(0, 3)
     ext = rng.uniform(0.4, 2.0)
     boxes = []
@@ -95,7 +97,7 @@ def random_composite(rng: np.random.Generator):
         for sx in (-1, 1):
             for sy in (-1, 1):
                 boxes.append(((sx * (w / 2 - 0.03), sy * (d / 2 - 0.03), (h - 0.04) / 2), (0.05, 0.05, h - 0.04), 0.0))
-        if rng.random() < 0.5:  # drawer block
+        if rng.random() < 0.5 or asymmetric:  # drawer block
             boxes.append(((w / 4, 0, h * 0.7), (w / 3, d * 0.9, h * 0.3), 0.0))
     elif kind == 1:  # chair with random proportions
         s = rng.uniform(0.4, 0.6)
@@ -157,3 +159,103 @@ def rmse_between(T_a, T_b, pts) -> float:
     p = np.asarray(pts, np.float64)
     d = p @ (T_a[:3, :3] - T_b[:3, :3]).T + (T_a[:3, 3] - T_b[:3, 3])
     return float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+
+
+_FACE_AX = np.array([0, 0, 1, 1, 2, 2])
+_FACE_SIGN = np.array([-1.0, 1.0, -1.0, 1.0, -1.0, 1.0])
+
+
+def _faces_array(boxes):
+    """Vectorised box_faces: origins, edge vectors (n_faces, 3) and areas."""
+    c = np.array([b[0] for b in boxes], np.float64)
+    sz = np.array([b[1] for b in boxes], np.float64)
+    yaw = np.array([b[2] for b in boxes], np.float64)
+    nb = len(boxes)
+    cy, sy = np.cos(yaw), np.sin(yaw)
+    R = np.zeros((nb, 3, 3))
+    R[:, 0, 0], R[:, 0, 1], R[:, 1, 0], R[:, 1, 1], R[:, 2, 2] = cy, -sy, sy, cy, 1.0
+    ax = np.tile(_FACE_AX, nb)
+    sign = np.tile(_FACE_SIGN, nb)
+    bi = np.repeat(np.arange(nb), 6)
+    u = (ax + 1) % 3
+    v = (ax + 2) % 3
+    u, v = np.minimum(u, v), np.maximum(u, v)
+    s6 = sz[bi]
+    k = np.arange(6 * nb)
+    o = np.zeros((6 * nb, 3))
+    o[k, ax] = sign * s6[k, ax] / 2
+    o[k, u] = -s6[k, u] / 2
+    o[k, v] = -s6[k, v] / 2
+    eu = np.zeros((6 * nb, 3))
+    eu[k, u] = s6[k, u]
+    ev = np.zeros((6 * nb, 3))
+    ev[k, v] = s6[k, v]
+    Rb = R[bi]
+    O = c[bi] + np.einsum("kij,kj->ki", Rb, o)
+    return O, np.einsum("kij,kj->ki", Rb, eu), np.einsum("kij,kj->ki", Rb, ev), s6[k, u] * s6[k, v]
+
+
+def sample_many(pieces, counts, rng: np.random.Generator, noise: float = 0.0):
+    """Area-weighted surface samples of many box composites at once (the
+    points of a cloud come grouped by face).  Returns one float32
+    (counts[i], 3) array per piece (piece frame)."""
+    counts = np.asarray(counts, np.int64)
+    Os, Us, Vs, per_face = [], [], [], []
+    cache = {}
+    for c, boxes in enumerate(pieces):
+        key = id(boxes)
+        if key not in cache:
+            cache[key] = _faces_array(boxes)
+        O, U, V, a = cache[key]
+        Os.append(O)
+        Us.append(U)
+        Vs.append(V)
+        per_face.append(rng.multinomial(int(counts[c]), a / a.sum()))
+    O = np.concatenate(Os).astype(np.float32)
+    U = np.concatenate(Us).astype(np.float32)
+    V = np.concatenate(Vs).astype(np.float32)
+    nf = np.concatenate(per_face)
+    uv = rng.random((int(nf.sum()), 2), dtype=np.float32)
+    p = np.repeat(O, nf, axis=0)
+    p += np.repeat(U, nf, axis=0) * uv[:, :1]
+    p += np.repeat(V, nf, axis=0) * uv[:, 1:2]
+    if noise > 0:
+        p += np.float32(noise) * rng.standard_normal(p.shape, dtype=np.float32)
+    return np.split(p, np.cumsum(counts)[:-1])
+
+
+def config4_batch(n_pairs: int = 256, seed_offset: int = 0, ns_range=(2000, 50000), nt_range=(10000, 200000),
+                  max_deg: float = 180.0, max_t: float = 0.1, noise: float = 0.001):
+    """Learned-proposal workload: pairs of asymmetric random box composites
+    with log-uniform sizes (as the batched workload), N(0, noise) per axis,
+    the source moved by a rotation U[0, max_deg] about a uniform axis and a
+    translation U[-max_t, max_t]^3.  Returns (sources, targets, T_gt)."""
+    srcs, tgts, gts = [], [], []
+    for p in range(n_pairs):
+        rng = rng_for(4, seed_offset + p)
+        boxes = random_composite(rng, asymmetric=True)
+        ns = loguniform_int(rng, *ns_range)
+        nt = loguniform_int(rng, *nt_range)
+        tgt, src0 = sample_many([boxes, boxes], [nt, ns], rng, noise)
+        T_src = random_rigid(rng, max_deg, max_t)
+        srcs.append(apply(T_src, src0))
+        tgts.append(tgt)
+        gts.append(np.linalg.inv(T_src))
+    return srcs, tgts, gts
+
+
+def training_base(n_pairs: int, seed: int, ns_range=(2000, 6000), nt_range=(4000, 12000), noise: float = 0.001,
+                  chunk: int = 512):
+    """Canonical training pairs (source and target independent samples of the
+    same asymmetric composite, both in the piece frame), yielded in chunks of
+    (sources, targets)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    done = 0
+    while done < n_pairs:
+        m = min(chunk, n_pairs - done)
+        pieces = [random_composite(rng, asymmetric=True) for _ in range(m)]
+        ns = [loguniform_int(rng, *ns_range) for _ in range(m)]
+        nt = [loguniform_int(rng, *nt_range) for _ in range(m)]
+        clouds = sample_many(pieces + pieces, nt + ns, rng, noise)
+        yield clouds[m:], clouds[:m]
+        done += m

@@ -1,0 +1,213 @@
+"""GPU parity of the learned pose proposer (librtr rtr_prop_* / rtr_gemm_bf16)
+against the CPU oracle (oracle/mlp.py, oracle.fps), plus the registration
+judge and the learned-init pipeline's gain over ICP without the network."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp
+import realtime_robot_amd as R
+from realtime_robot_amd import learned, multistart, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb,ks", [
+    (64, 64, 32, 0, 0, 1), (100, 70, 45, 0, 0, 1), (37, 130, 300, 1, 0, 1), (200, 256, 9, 0, 1, 1),
+    (16, 64, 4096, 1, 0, 8), (128, 256, 8192, 1, 0, 16), (300, 9, 256, 0, 0, 1), (65, 33, 1000, 1, 1, 3)])
+def test_gemm_bf16_matches_numpy(M, N, K, ta, tb, ks):
+    """The MFMA lane layout, transposes and split-K: FP32-accumulated BF16
+    products against the FP64 product of the same BF16-rounded operands."""
+    import ctypes as C
+    from realtime_robot_amd import _lib as L
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
+    Cm = np.empty((M, N), np.float32)
+    L.check(L.lib.rtr_gemm_bf16(0, L.ptr(A), ta, L.ptr(B), tb, L.ptr(Cm), M, N, K, ks))
+    Ab = mlp.bf16(A.T if ta else A)
+    Bb = mlp.bf16(B.T if tb else B)
+    ref = Ab @ Bb
+    bound = 2e-6 * (np.abs(Ab) @ np.abs(Bb)) + 1e-30
+    print(f"gemm {M}x{N}x{K} ta{ta} tb{tb} ks{ks}: max |err|/bound {np.max(np.abs(Cm - ref) / bound):.3f}")
+    assert np.all(np.abs(Cm - ref) <= bound)
+
+
+def test_fps_matches_oracle_bit_exact():
+    rng = np.random.default_rng(5)
+    boxes = synth.random_composite(rng, asymmetric=True)
+    clouds = synth.sample_many([boxes] * 6, [500, 1024, 5000, 8192, 20000, 60000], rng, 0.001)
+    src, tgt = clouds[0::2], clouds[1::2]
+    prop = learned.Proposer(max_pairs=3)
+    prop.set_clouds(src, tgt)
+    pts, cen, sel = prop.fps()
+    order = [src[0], tgt[0], src[1], tgt[1], src[2], tgt[2]]
+    for c, cloud in enumerate(order):
+        s_ref, c_ref, sub = oracle.fps(cloud)
+        assert np.array_equal(sel[c], s_ref), f"cloud {c} ({len(cloud)} pts): selection differs"
+        assert np.allclose(cen[c], c_ref, rtol=0, atol=1e-6)
+        exp = sub[s_ref] - cen[c]
+        assert np.max(np.abs(pts[c] - exp)) <= 1e-6
+    print("fps: selections identical for sizes", [len(c) for c in order])
+
+
+def _labels(rng, n):
+    lab = np.zeros((n, 12))
+    for i in range(n):
+        q = synth.random_rotation(rng, 180.0)
+        lab[i, :9] = q.reshape(-1)
+        lab[i, 9:] = rng.uniform(-0.05, 0.05, 3)
+    return lab
+
+
+def _fps_inputs(n_pairs, seed):
+    srcs, tgts, _ = synth.config4_batch(n_pairs, seed_offset=seed, ns_range=(2000, 5000), nt_range=(4000, 9000))
+    prop = learned.Proposer(max_pairs=n_pairs)
+    prop.set_clouds(srcs, tgts)
+    pts, _, _ = prop.fps()
+    return pts.reshape(-1, 3)
+
+
+def test_forward_and_gradient_match_oracle():
+    """BF16 MFMA forward/backward against the oracle emulating the same BF16
+    roundings (FP64 accumulation); the oracle's own gradient is pinned by
+    finite differences in the CPU suite."""
+    n = 3
+    rng = np.random.default_rng(11)
+    xyz = _fps_inputs(n, 500)
+    lab = _labels(rng, n)
+    w = learned.init_weights(3)
+    w[-9:] += rng.normal(0, 0.3, 9).astype(np.float32)
+    prop = learned.Proposer(max_pairs=n, weights=w)
+    out = prop.forward_backward(xyz, lab)
+    ref = mlp.forward_backward(w, xyz, lab, emulate_bf16=True)
+    e_o = _rel(out["O"], ref["O"])
+    e_l = abs(out["loss"] - ref["loss"]) / abs(ref["loss"])
+    print(f"forward: rel err O {e_o:.2e}, loss {e_l:.2e}")
+    assert e_o < 1e-3 and e_l < 1e-3
+    assert np.allclose(out["R"], ref["R"], atol=2e-3)
+    off = 0
+    for name, shp in mlp._SHAPES:
+        sz = int(np.prod(shp))
+        g, gr = out["grad"][off:off + sz], ref["grad"][off:off + sz]
+        e = _rel(g, gr) if np.linalg.norm(gr) > 0 else float(np.abs(g).max())
+        print(f"grad {name}: rel err {e:.2e} (|g| {np.linalg.norm(gr):.3e})")
+        assert e < 2e-2, name
+        off += sz
+    assert _rel(out["grad"], ref["grad"]) < 1e-2
+
+
+def test_adam_step_matches_definition():
+    n = 2
+    rng = np.random.default_rng(12)
+    xyz = _fps_inputs(n, 600)
+    lab = _labels(rng, n)
+    w0 = learned.init_weights(4)
+    prop = learned.Proposer(max_pairs=n, weights=w0)
+    g = prop.forward_backward(xyz, lab)["grad"].astype(np.float64)
+    lr = 1e-3
+    prop.train_step(xyz, lab, lr)
+    w1 = prop.get_weights().astype(np.float64)
+    # first Adam step: bias-corrected moments are g and g^2
+    exp = w0 - lr * g / (np.abs(g) + 1e-8)
+    print("adam: max |w1 - exp|", np.abs(w1 - exp).max())
+    assert np.abs(w1 - exp).max() < 2e-6
+
+
+def test_propose_composes_centred_pose():
+    """With W5 = 0 the head output is b5: R = Gram-Schmidt(b5[0:6]), t =
+    b5[6:9], and the proposal is Tr(c_tgt) [R | t] Tr(-c_src)."""
+    srcs, tgts, _ = synth.config4_batch(2, seed_offset=700, ns_range=(3000, 6000), nt_range=(5000, 9000))
+    w = learned.init_weights(5)
+    P = mlp.unpack(w)
+    P["W5"][:] = 0
+    P["b5"][:] = [0.3, -0.5, 0.8, 0.9, 0.2, -0.1, 0.05, -0.02, 0.01]
+    prop = learned.Proposer(max_pairs=2, weights=w)
+    prop.set_clouds(srcs, tgts)
+    T = prop.propose()
+    _, cen, _ = prop.fps()
+    Rr, t = mlp.gram_schmidt(P["b5"][None, :].astype(np.float64))
+    for p in range(2):
+        cs, ct = cen[2 * p].astype(np.float64), cen[2 * p + 1].astype(np.float64)
+        exp = np.eye(4)
+        exp[:3, :3] = Rr[0]
+        exp[:3, 3] = t[0] + ct - Rr[0] @ cs
+        assert np.abs(T[p] - exp).max() < 1e-6
+
+
+def test_training_reduces_loss():
+    n = 8
+    rng = np.random.default_rng(13)
+    xyz = _fps_inputs(n, 800)
+    lab = _labels(rng, n)
+    prop = learned.Proposer(max_pairs=n, weights=learned.init_weights(6))
+    losses = [prop.train_step(xyz, lab, 1e-3) for _ in range(150)]
+    print("train: loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.3 * losses[0]
+
+
+def test_judge_separates_right_and_wrong_poses():
+    srcs, tgts, gts = synth.config4_batch(4, seed_offset=900, ns_range=(3000, 8000), nt_range=(10000, 30000))
+    b = R.BatchRegistration()
+    b.set_clouds(srcs, tgts)
+    b.set_problems(range(4), range(4))
+    with pytest.raises(R.RtrError):
+        b.judge()                      # no align yet
+    b.set_params(learned.default_params())
+    b.align(gts)
+    good, good_frac = b.judge()
+    bad_poses = []
+    for p in range(4):
+        Rz = multistart._rot([0, 0, 1], 90.0)
+        D = np.eye(4)
+        D[:3, :3] = Rz
+        c = gts[p][:3, :3] @ srcs[p].astype(np.float64).mean(0) + gts[p][:3, 3]
+        D[:3, 3] = c - Rz @ c
+        bad_poses.append(D @ gts[p])
+    b.set_params(R.make_params(max_iterations=1, max_correspondence_distance=0.05))
+    b.align(bad_poses)
+    bad, bad_frac = b.judge()
+    print("judge: right", good, good_frac, "wrong", bad, bad_frac)
+    assert np.all(good < 2e-3) and np.all(good_frac > 0.95)
+    assert np.all((bad > 2e-3) | (bad_frac < 0.5))
+
+
+WEIGHTS = learned.WEIGHTS_PATH
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained proposer weights in the tree")
+def test_learned_init_gains_over_icp_without_it():
+    """Ablation on 48 pairs with rotations U[0, 180 deg]: ICP from the
+    network's proposal vs ICP from centroid alignment (the same pipeline
+    without the network), judged against the GPU ICP started at the true pose."""
+    n = 48
+    srcs, tgts, gts = synth.config4_batch(n, seed_offset=5000, ns_range=(3000, 12000), nt_range=(10000, 40000))
+    w, _ = learned.load_weights()
+    prop = learned.Proposer(max_pairs=n, weights=w)
+    prop.set_clouds(srcs, tgts)
+    proposals = prop.propose()
+    b = R.BatchRegistration()
+    b.set_clouds(srcs, tgts)
+    b.set_problems(range(n), range(n))
+    b.set_params(learned.default_params())
+    ref = [r["T"] for r in b.align(gts)]
+    learned_T = [r["T"] for r in b.align(proposals)]
+    base = []
+    for p in range(n):
+        T = np.eye(4)
+        T[:3, 3] = tgts[p].astype(np.float64).mean(0) - srcs[p].astype(np.float64).mean(0)
+        base.append(T)
+    plain_T = [r["T"] for r in b.align(base)]
+    ok_l = np.array([synth.rmse_between(learned_T[p], ref[p], srcs[p]) <= 1e-3 for p in range(n)])
+    ok_p = np.array([synth.rmse_between(plain_T[p], ref[p], srcs[p]) <= 1e-3 for p in range(n)])
+    rot = learned.rotation_error_deg(proposals[:, :3, :3], np.stack(gts)[:, :3, :3])
+    print(f"ablation: learned-init {ok_l.mean():.2f}, without network {ok_p.mean():.2f}; "
+          f"proposal rotation error median {np.median(rot):.1f} deg")
+    assert ok_l.mean() >= ok_p.mean() + 0.25
