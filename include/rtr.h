@@ -188,12 +188,15 @@ int rtr_batch_fitness(rtr_batch* b, double max_range, double* out);
 /* Registration judge at each problem's final transform.  A strided sample of
  * up to max_points source points is moved by the final pose; a point is an
  * inlier when its nearest target point is within the max correspondence
- * distance; plane_rmse = RMS distance of the inliers to the plane fitted (PCA)
- * to their k nearest target points (DBL_MAX with no inlier), inlier_frac =
- * inliers / sampled points.  Either output is nullable.  No reference
- * counterpart: it is the run-time acceptance test that decides whether a
- * learned proposal needs the multi-start fallback. */
-int rtr_batch_judge(rtr_batch* b, int k, int max_points, double* plane_rmse, double* inlier_frac);
+ * distance.  For each inlier the k nearest target points are fitted with a
+ * plane (PCA); where that neighbourhood is planar (surface variation
+ * l0/(l0+l1+l2) <= max_curvature) the inlier's distance to the plane counts.
+ * Outputs (each nullable): plane_rmse = RMS of those distances (DBL_MAX with
+ * none), inlier_frac = inliers / sampled points, planar_frac = planar inliers
+ * / inliers.  No reference counterpart: it is the run-time acceptance test
+ * that decides whether a learned proposal needs the multi-start fallback. */
+int rtr_batch_judge(rtr_batch* b, int k, int max_points, double max_curvature, double* plane_rmse,
+                    double* inlier_frac, double* planar_frac);
 
 /* ------------------------------------------------------------------ */
 /* Voxel-hash map: merging transformed clouds into the scene map.       */

@@ -99,7 +99,7 @@ _SIGS = {
     "rtr_batch_get_grid": (_i, [_vp, _i, _vp, _vp, _vp]),
     "rtr_batch_fitness": (_i, [_vp, _d, _vp]),
     "rtr_batch_get_history": (_i, [_vp, _i, _vp, _i, C.POINTER(_i)]),
-    "rtr_batch_judge": (_i, [_vp, _i, _i, _vp, _vp]),
+    "rtr_batch_judge": (_i, [_vp, _i, _i, _d, _vp, _vp, _vp]),
     "rtr_map_create": (_i, [C.POINTER(_vp), _i, _f, _sz]),
     "rtr_map_destroy": (_i, [_vp]),
     "rtr_map_clear": (_i, [_vp]),

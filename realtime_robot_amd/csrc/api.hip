@@ -456,10 +456,11 @@ int rtr_batch_fitness(rtr_batch* b, double max_range, double* out) {
   RTR_GUARD_END
 }
 
-int rtr_batch_judge(rtr_batch* b, int k, int max_points, double* plane_rmse, double* inlier_frac) {
+int rtr_batch_judge(rtr_batch* b, int k, int max_points, double max_curvature, double* plane_rmse,
+                    double* inlier_frac, double* planar_frac) {
   RTR_GUARD_BEGIN
   if (!b) return RTR_ERR_INVALID_ARG;
-  return b->eng.judge(k, max_points, plane_rmse, inlier_frac);
+  return b->eng.judge(k, max_points, max_curvature, plane_rmse, inlier_frac, planar_frac);
   RTR_GUARD_END
 }
 

@@ -589,29 +589,32 @@ int Engine::fitness(double max_range, double* out) {
   return RTR_OK;
 }
 
-int Engine::judge(int k, int max_pts, double* plane_rmse, double* inlier_frac) {
+int Engine::judge(int k, int max_pts, double max_curv, double* plane_rmse, double* inlier_frac,
+                  double* planar_frac) {
   RTR_HIP(hipSetDevice(device_));
   if (!results_valid_) {
     set_error("judge needs a completed align of the current clouds and problems");
     return RTR_ERR_NOT_READY;
   }
-  if (k < 3 || k > 32 || max_pts < 1) {
-    set_error("judge: k must be in [3, 32] and max_pts >= 1 (got %d, %d)", k, max_pts);
+  if (k < 3 || k > 32 || max_pts < 1 || !(max_curv > 0.0)) {
+    set_error("judge: k must be in [3, 32], max_pts >= 1, max_curvature > 0 (got %d, %d, %g)", k, max_pts,
+              max_curv);
     return RTR_ERR_INVALID_ARG;
   }
   const size_t np = probs_.size();
   if (np == 0) return RTR_OK;
-  RTR_TRY(out_f32_.ensure(np * 3 * sizeof(double)));
+  RTR_TRY(out_f32_.ensure(np * 4 * sizeof(double)));
   DevEngine e = dev_view();
-  launch_judge(e, k, max_pts, e.prm.r2f, reinterpret_cast<double*>(out_f32_.p), stream_);
+  launch_judge(e, k, max_pts, e.prm.r2f, max_curv, reinterpret_cast<double*>(out_f32_.p), stream_);
   RTR_HIP(hipGetLastError());
-  std::vector<double> h(np * 3);
-  RTR_HIP(hipMemcpyAsync(h.data(), out_f32_.p, np * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  std::vector<double> h(np * 4);
+  RTR_HIP(hipMemcpyAsync(h.data(), out_f32_.p, np * 4 * sizeof(double), hipMemcpyDeviceToHost, stream_));
   RTR_HIP(hipStreamSynchronize(stream_));
   for (size_t p = 0; p < np; ++p) {
-    const double s = h[3 * p], c = h[3 * p + 1], m = h[3 * p + 2];
-    if (plane_rmse) plane_rmse[p] = c > 0 ? std::sqrt(s / c) : DBL_MAX;
+    const double s = h[4 * p], pc = h[4 * p + 1], c = h[4 * p + 2], m = h[4 * p + 3];
+    if (plane_rmse) plane_rmse[p] = pc > 0 ? std::sqrt(s / pc) : DBL_MAX;
     if (inlier_frac) inlier_frac[p] = m > 0 ? c / m : 0.0;
+    if (planar_frac) planar_frac[p] = c > 0 ? pc / c : 0.0;
   }
   return RTR_OK;
 }

@@ -67,7 +67,7 @@ class Engine {
   int correspondences(int prob, const double T[16], double max_dist, int32_t* idx, float* d2);
   int transformed_source(int prob, float* out, size_t stride);
   int fitness(double max_range, double* out);
-  int judge(int k, int max_pts, double* plane_rmse, double* inlier_frac);
+  int judge(int k, int max_pts, double max_curv, double* plane_rmse, double* inlier_frac, double* planar_frac);
   int grid_info(int tgt, float* cell, int dims[3], float anchor[3]);
   int history(int prob, float* rows12, int cap, int* n_out);
   int target_normals(int tgt, float* out);

@@ -803,16 +803,21 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
 // One workgroup per problem.  A strided sample of up to max_pts source points
 // (in the stored order) is moved by the problem's final pose (float, the same
 // rounding as the ICP queries).  A point is an inlier when its nearest target
-// point lies within the correspondence radius (d2 <= r2f); for an inlier the
+// point lies within the correspondence radius (d2 <= r2f).  For an inlier the
 // k nearest target points give the local plane (PCA through their centroid,
-// FP64) and its squared distance to that plane is summed.  Unlike the
-// point-to-point MSE, this residual does not grow with the target's sampling
-// spacing, so one threshold serves sparse and dense targets.
-// out[3p..3p+2] = {sum of squared plane distances, inliers, sampled points}.
-__global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max_pts, float r2f, double* out) {
+// FP64); when that neighbourhood is planar (surface variation
+// l0 / (l0 + l1 + l2) <= max_curv, PCL's curvature) the inlier's squared
+// distance to the plane is summed.  Edges and corners (several faces in one
+// neighbourhood) are left out of the sum: there the fitted plane is not the
+// surface.  Unlike the point-to-point MSE, this residual does not grow with
+// the target's sampling spacing, so one threshold serves sparse and dense
+// targets.
+// out[4p..4p+3] = {sum of squared plane distances, planar inliers, inliers,
+// sampled points}.
+__global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max_pts, float r2f, double max_curv, double* out) {
   __shared__ uint64_t s_key[kMaxK * kNrmBlock];
   __shared__ uint32_t s_slot[kMaxK * kNrmBlock];
-  __shared__ double s_red[2][kNrmBlock / 64];
+  __shared__ double s_red[3][kNrmBlock / 64];
   const int prob = blockIdx.x;
   const Problem pr = e.probs[prob];
   const SourceDesc sd = e.srcs[pr.src];
@@ -822,7 +827,7 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
   tf_rows_f32(e.st[prob].T, m);
   const uint32_t n = sd.n;
   const uint32_t ms = n < (uint32_t)max_pts ? n : (uint32_t)max_pts;
-  double ssum = 0.0, cnt = 0.0;
+  double ssum = 0.0, cnt = 0.0, pcnt = 0.0;
   for (uint32_t j = threadIdx.x; j < ms; j += kNrmBlock) {
     const uint32_t sj = sd.off + (uint32_t)(((uint64_t)j * n) / ms);
     float qx, qy, qz;
@@ -840,10 +845,7 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     v.slot = s_slot + threadIdx.x;
     ring_walk(g, qx, qy, qz, FLT_MAX, v);
     cnt += 1.0;
-    if (v.cnt < 3) {
-      ssum += (double)key_d2(b.key);
-      continue;
-    }
+    if (v.cnt < 3) continue;
     double mean[3] = {0, 0, 0};
     for (int i = 0; i < v.cnt; ++i) {
       const float4 p = e.tgt_sorted[v.slot[i * kNrmBlock]];
@@ -864,36 +866,44 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     int mi = 0;
     if (C[4] < C[mi * 4]) mi = 1;
     if (C[8] < C[mi * 4]) mi = 2;
+    const double lsum = C[0] + C[4] + C[8];
+    if (!(lsum > 0.0) || C[mi * 4] > max_curv * lsum) continue;
     const double dist = V[mi] * (qx - mean[0]) + V[3 + mi] * (qy - mean[1]) + V[6 + mi] * (qz - mean[2]);
     ssum += dist * dist;
+    pcnt += 1.0;
   }
   // fixed-order block reduction
   for (int off = 32; off > 0; off >>= 1) {
     ssum += __shfl_xor(ssum, off, 64);
     cnt += __shfl_xor(cnt, off, 64);
+    pcnt += __shfl_xor(pcnt, off, 64);
   }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     s_red[0][w] = ssum;
-    s_red[1][w] = cnt;
+    s_red[1][w] = pcnt;
+    s_red[2][w] = cnt;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double a = 0.0, c = 0.0;
+    double a = 0.0, pc = 0.0, c = 0.0;
     for (int i = 0; i < kNrmBlock / 64; ++i) {
       a += s_red[0][i];
-      c += s_red[1][i];
+      pc += s_red[1][i];
+      c += s_red[2][i];
     }
-    out[3 * prob] = a;
-    out[3 * prob + 1] = c;
-    out[3 * prob + 2] = (double)ms;
+    out[4 * prob] = a;
+    out[4 * prob + 1] = pc;
+    out[4 * prob + 2] = c;
+    out[4 * prob + 3] = (double)ms;
   }
 }
 
-void launch_judge(const DevEngine& e, int k, int max_pts, float r2f, double* out, hipStream_t s) {
+void launch_judge(const DevEngine& e, int k, int max_pts, float r2f, double max_curv, double* out,
+                  hipStream_t s) {
   if (e.n_prob == 0) return;
   if (k > kMaxK) k = kMaxK;
-  hipLaunchKernelGGL(k_judge, dim3(e.n_prob), dim3(kNrmBlock), 0, s, e, k, max_pts, r2f, out);
+  hipLaunchKernelGGL(k_judge, dim3(e.n_prob), dim3(kNrmBlock), 0, s, e, k, max_pts, r2f, max_curv, out);
 }
 
 }  // namespace rtr

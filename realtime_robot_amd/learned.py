@@ -15,9 +15,11 @@ point network proposes it instead:
     every training pair is a base pair whose source is moved by a fresh random
     rigid motion (rotation U[0, 180 deg] about a uniform axis, translation
     U[-0.1, 0.1]^3 m), with the label composed accordingly.
-  * ``LearnedRegistration``: propose -> batch ICP refine -> judge (RMS
-    distance of the inliers to their k-NN PCA plane, rtr_batch_judge) ->
-    8-start ICP for the pairs whose RMSE is above the threshold.
+  * ``LearnedRegistration``: propose -> batch ICP refine -> judge
+    (rtr_batch_judge: RMS distance of the inliers to their k-NN PCA plane
+    where the neighbourhood is planar, and the inlier fraction) -> 8-start ICP
+    for the pairs whose RMSE is above 2 mm or whose inlier fraction is below
+    0.8.
   * ``MultiStartRegistration``: the plain multi-start ICP it is compared with
     (and falls back to).
 """
@@ -345,8 +347,8 @@ class LearnedRegistration:
     """Learned proposal -> ICP refine -> judge -> multi-start fallback."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
-                 min_inlier_frac: float = 0.5, judge_k: int = 20, judge_points: int = 2048, max_pairs: int = 256,
-                 device: int = 0):
+                 min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 2048,
+                 max_curvature: float = 0.05, max_pairs: int = 256, device: int = 0):
         if weights is None:
             weights, _ = load_weights()
         self.params = params if params is not None else default_params()
@@ -358,6 +360,7 @@ class LearnedRegistration:
         self.min_inlier_frac = min_inlier_frac
         self.judge_k = judge_k
         self.judge_points = judge_points
+        self.max_curvature = max_curvature
         self.stats = {}
 
     def set_clouds(self, sources, targets, src_centroids=None, tgt_centroids=None):
@@ -374,7 +377,7 @@ class LearnedRegistration:
         t1 = time.perf_counter()
         self.reg.set_problems(self._ids, self._ids)
         res = self.reg.align(poses)
-        rmse, frac = self.reg.judge(self.judge_k, self.judge_points)
+        rmse, frac, pfrac = self.reg.judge(self.judge_k, self.judge_points, self.max_curvature)
         t2 = time.perf_counter()
         fail = np.where((rmse > self.rmse_threshold) | (frac < self.min_inlier_frac))[0]
         if len(fail):
@@ -387,6 +390,7 @@ class LearnedRegistration:
                 res[i] = r[multistart.select_best(r)]
         t3 = time.perf_counter()
         self.stats = {"n_fallback": int(len(fail)), "proposals": poses, "judge_rmse": rmse, "judge_frac": frac,
+                      "judge_planar_frac": pfrac,
                       "ms_propose": (t1 - t0) * 1e3, "ms_refine": (t2 - t1) * 1e3, "ms_fallback": (t3 - t2) * 1e3,
                       "ms_propose_device": self.prop.last_ms()}
         return res

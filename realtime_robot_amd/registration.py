@@ -270,15 +270,19 @@ class BatchRegistration:
         L.check(L.lib.rtr_batch_fitness(self._h, float(max_range), L.ptr(out)))
         return out
 
-    def judge(self, k: int = 20, max_points: int = 2048):
-        """(plane_rmse, inlier_frac) per problem at its final transform: RMS
-        distance of the inliers (nearest target point within the max
-        correspondence distance) to their k-NN PCA plane, over a strided
-        sample of up to max_points source points."""
+    def judge(self, k: int = 20, max_points: int = 2048, max_curvature: float = 0.05):
+        """(plane_rmse, inlier_frac, planar_frac) per problem at its final
+        transform: RMS distance of the inliers (nearest target point within
+        the max correspondence distance) to their k-NN PCA plane, counted
+        where that neighbourhood is planar (surface variation <=
+        max_curvature), over a strided sample of up to max_points source
+        points."""
         rmse = np.empty(self.n_problems)
         frac = np.empty(self.n_problems)
-        L.check(L.lib.rtr_batch_judge(self._h, int(k), int(max_points), L.ptr(rmse), L.ptr(frac)))
-        return rmse, frac
+        pfrac = np.empty(self.n_problems)
+        L.check(L.lib.rtr_batch_judge(self._h, int(k), int(max_points), float(max_curvature), L.ptr(rmse),
+                                      L.ptr(frac), L.ptr(pfrac)))
+        return rmse, frac, pfrac
 
 
 class VoxelMap:

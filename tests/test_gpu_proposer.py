@@ -162,7 +162,7 @@ def test_judge_separates_right_and_wrong_poses():
         b.judge()                      # no align yet
     b.set_params(learned.default_params())
     b.align(gts)
-    good, good_frac = b.judge()
+    good, good_frac, good_pf = b.judge()
     bad_poses = []
     for p in range(4):
         Rz = multistart._rot([0, 0, 1], 90.0)
@@ -173,10 +173,10 @@ def test_judge_separates_right_and_wrong_poses():
         bad_poses.append(D @ gts[p])
     b.set_params(R.make_params(max_iterations=1, max_correspondence_distance=0.05))
     b.align(bad_poses)
-    bad, bad_frac = b.judge()
-    print("judge: right", good, good_frac, "wrong", bad, bad_frac)
-    assert np.all(good < 2e-3) and np.all(good_frac > 0.95)
-    assert np.all((bad > 2e-3) | (bad_frac < 0.5))
+    bad, bad_frac, bad_pf = b.judge()
+    print("judge: right", good, good_frac, good_pf, "wrong", bad, bad_frac, bad_pf)
+    assert np.all(good < 2e-3) and np.all(good_frac > 0.95) and np.all(good_pf > 0.3)
+    assert np.all((bad > 2e-3) | (bad_frac < 0.8))
 
 
 WEIGHTS = learned.WEIGHTS_PATH
