@@ -177,6 +177,13 @@ int Engine::set_clouds(int n_src, const float* const* src, const size_t* ns, con
   grid_dirty_ = true;
   graph_dirty_ = true;
   results_valid_ = false;
+  // a problem set that names clouds which no longer exist is dropped (the
+  // caller sets problems again); one that still fits is re-laid out
+  for (const Problem& pr : probs_)
+    if (pr.src >= srcs_.size() || pr.tgt >= tgts_.size()) {
+      probs_.clear();
+      break;
+    }
   if (!probs_.empty()) return upload_problems();
   return RTR_OK;
 }
@@ -236,6 +243,13 @@ size_t Engine::source_size(int prob) const {
 int Engine::upload_problems() {
   RTR_HIP(hipSetDevice(device_));
   RTR_HIP(hipStreamSynchronize(stream_));
+  for (const Problem& pr : probs_)
+    if (pr.src >= srcs_.size() || pr.tgt >= tgts_.size()) {
+      set_error("problem references a missing cloud (src %u of %zu, tgt %u of %zu)", pr.src, srcs_.size(), pr.tgt,
+                tgts_.size());
+      probs_.clear();
+      return RTR_ERR_INVALID_ARG;
+    }
   // target cell budgets and the requested cell edge
   const double rmax = prm_.max_correspondence_distance;
   float h_req = prm_.cell_size > 0 ? prm_.cell_size : 0.f;

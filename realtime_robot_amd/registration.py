@@ -188,6 +188,7 @@ class BatchRegistration:
         self._h = h
         self._keep = []
         self.n_problems = 0
+        self._max_ids = (-1, -1)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -204,6 +205,8 @@ class BatchRegistration:
         nt = (C.c_size_t * len(tgt))(*[len(t) for t in tgt])
         L.check(L.lib.rtr_batch_set_clouds(self._h, len(src), sp, ns, len(tgt), tp, nt))
         self._ns = [len(s) for s in src]
+        if self._max_ids[0] >= len(src) or self._max_ids[1] >= len(tgt):
+            self.n_problems = 0             # the library dropped the problem set
 
     def set_target_normals(self, normals):
         nr = [L.pts3(n) for n in normals]
@@ -219,6 +222,7 @@ class BatchRegistration:
         t = np.ascontiguousarray(tgt_ids, np.int32)
         L.check(L.lib.rtr_batch_set_problems(self._h, len(s), L.ptr(s), L.ptr(t)))
         self.n_problems = len(s)
+        self._max_ids = (int(s.max(initial=-1)), int(t.max(initial=-1)))
 
     def set_params(self, p: L.Params):
         L.check(L.lib.rtr_batch_set_params(self._h, C.byref(p)))

@@ -171,3 +171,27 @@ def test_batch_shared_clouds_multistart():
     for g, r in zip(guesses, res):
         ref = oracle.icp(src, tgt, oracle.params(max_iter=25, r_max=0.05, eps_T=1e-8), guess=g)
         assert rot_err(r["T"], ref["T"]) < TOL_R and trans_err(r["T"], ref["T"]) < TOL_T
+
+
+def test_batch_resize_drops_stale_problems():
+    """Replacing the clouds with fewer clouds drops a problem set that names
+    clouds which no longer exist (regression: it was re-laid out with
+    out-of-range cloud indices); a set that still fits is kept."""
+    srcs, tgts, gts = synth.config3_batch(6, seed_offset=950, ns_range=(2000, 3000), nt_range=(4000, 6000))
+    b = R.BatchRegistration()
+    b.set_clouds(srcs, tgts)
+    b.set_problems(range(6), range(6))
+    b.set_params(R.make_params(max_iterations=10, max_correspondence_distance=0.05))
+    b.align(gts)
+    b.set_clouds(srcs[:2], tgts[:2])
+    with pytest.raises(R.RtrError) as e:
+        b.align(gts[:2])
+    assert e.value.code == R._lib.ERR_NOT_READY
+    b.set_problems(range(2), range(2))
+    r2 = b.align(gts[:2])
+    b.set_clouds(srcs[2:4], tgts[2:4])        # same count: problems 0, 1 still valid
+    r3 = b.align(gts[2:4])
+    assert len(r2) == len(r3) == 2
+    for p in range(2):
+        ref = oracle.icp(srcs[2 + p], tgts[2 + p], oracle.params(max_iter=10, r_max=0.05), guess=gts[2 + p])
+        assert rot_err(r3[p]["T"], ref["T"]) < TOL_R and trans_err(r3[p]["T"], ref["T"]) < TOL_T
