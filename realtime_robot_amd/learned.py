@@ -213,11 +213,12 @@ class TrainingSet:
         n = len(idx)
         A = random_rigid_batch(rng, n, max_deg, max_t)
         Q = A[:, :3, :3]
-        src = np.einsum("nkj,nij->nki", self.src_pts[idx].astype(np.float64), Q).astype(np.float32)
+        xyz = np.empty((n, 2, KP, 3), np.float32)
+        np.matmul(self.src_pts[idx], np.transpose(Q, (0, 2, 1)).astype(np.float32), out=xyz[:, 0])
+        xyz[:, 1] = self.tgt_pts[idx]
         cs = np.einsum("nij,nj->ni", Q, self.cs[idx]) + A[:, :3, 3]
         T = self.T_label[idx] @ np.linalg.inv(A)
-        xyz = np.stack([src, self.tgt_pts[idx]], axis=1).reshape(-1, 3)
-        return xyz, centred_labels(T, cs, self.ct[idx])
+        return xyz.reshape(-1, 3), centred_labels(T, cs, self.ct[idx])
 
 
 def build_training_set(n_base: int, seed: int = 20240, chunk: int = 512, device: int = 0, log=print,
@@ -264,19 +265,27 @@ def train(ts: TrainingSet, steps: int, batch: int = 256, lr: float = 1e-3, lr_mi
           warmup: int = 200, seed: int = 0, weights=None, device: int = 0, log=print, log_every: int = 500):
     """Adam on the device with a warm-up + cosine learning rate.  Returns
     (weights, loss history)."""
+    from concurrent.futures import ThreadPoolExecutor
     prop = Proposer(max_pairs=batch, device=device, weights=init_weights(seed) if weights is None else weights)
     rng = np.random.default_rng(seed + 1)
-    order = rng.permutation(len(ts))
-    pos = 0
+    state = {"order": rng.permutation(len(ts)), "pos": 0}
+
+    def next_batch():
+        if state["pos"] + batch > len(state["order"]):
+            state["order"] = rng.permutation(len(ts))
+            state["pos"] = 0
+        idx = state["order"][state["pos"]:state["pos"] + batch]
+        state["pos"] += batch
+        return ts.batch(idx, rng)
+
     hist = []
     t0 = time.time()
+    pool = ThreadPoolExecutor(1)          # builds the next batch while the device trains
+    fut = pool.submit(next_batch)
     for step in range(steps):
-        if pos + batch > len(order):
-            order = rng.permutation(len(ts))
-            pos = 0
-        idx = order[pos:pos + batch]
-        pos += batch
-        xyz, lab = ts.batch(idx, rng)
+        xyz, lab = fut.result()
+        if step + 1 < steps:
+            fut = pool.submit(next_batch)
         if step < warmup:
             lr_t = lr * (step + 1) / warmup
         else:
@@ -287,6 +296,7 @@ def train(ts: TrainingSet, steps: int, batch: int = 256, lr: float = 1e-3, lr_mi
         if (step + 1) % log_every == 0 or step + 1 == steps:
             log(f"step {step + 1}/{steps} loss {np.mean(hist[-log_every:]):.5f} lr {lr_t:.2e} "
                 f"({time.time() - t0:.1f} s)")
+    pool.shutdown()
     return prop.get_weights(), np.array(hist)
 
 
