@@ -1,20 +1,29 @@
-"""Benchmark: registrations/s on one MI355X for batched multi-start ICP.
+"""Benchmark: registrations/s at <= 1 mm RMSE on one MI355X, learned-init + ICP
+against plain multi-start ICP.
 
-Workload: 256 independent model-to-scene pairs of random box-composite
-furniture (source sizes log-uniform 2k-50k, target sizes log-uniform
-10k-200k, init rotation U[0, 30 deg]), point-to-point ICP with max
-correspondence distance 0.05 m, <= 50 iterations, transformation epsilon
-1e-8, 8 starts per pair, all starts of all pairs in one persistent launch
-replayed from a hipGraph.  A step = target grid builds + source sorts +
-every ICP iteration of every start + best-start selection, with the clouds
-already resident in HBM.  The engine's launch/layout parameters are
-autotuned during warmup (untimed).
+Workload (printed as config.workload): 256 independent pairs of asymmetric
+random box-composite furniture, source sizes log-uniform 2k-50k points,
+target sizes log-uniform 10k-200k, N(0, 1 mm) noise, the source moved by a
+rotation U[0, 180 deg] about a uniform axis and a translation
+U[-0.1, 0.1]^3 m; point-to-point ICP with max correspondence distance 0.05 m,
+<= 50 iterations, transformation epsilon 1e-8.  Clouds are resident in HBM
+before timing starts.
+
+One step of the learned path = farthest-point sampling + the BF16 network on
+8 orientation hypotheses per pair + candidate scoring + one batched ICP
+refine of every pair's best proposal + the registration judge + 8-start ICP
+for the pairs the judge rejects + best-start selection.  One step of the
+plain path = 8-start ICP of every pair (starts spread over SO(3)) + best-start
+selection.  A registration counts when its pose is within 1 mm RMSE (over
+the source points) of the reference pose: the GPU ICP started at the
+generating pose, itself checked against the FP64 CPU oracle on a sample.
 
 Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -29,56 +38,58 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "registrations/sec @≤1mm RMSE vs FP64: learned-init+ICP vs plain ICP, 1 GPU"
 R_MAX, MAX_ITER, EPS = 0.05, 50, 1e-8
+OK_RMSE = 1e-3
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def start_guesses(srcs, tgts, n_starts):
-    from realtime_robot_amd import multistart
-    guesses = []
-    for s, t in zip(srcs, tgts):
-        guesses.extend(multistart.start_poses(s.astype(np.float64).mean(0), t.astype(np.float64).mean(0), n_starts))
-    return np.stack(guesses)
-
-
-def cpu_baseline(srcs, tgts, guesses, n_starts, gpu_best, budget_s):
-    """PCL-semantics CPU ICP (float kd-tree, leaf 15) on a bounded sample of
-    the same pairs and starts on the host cores; the FP64 oracle on part of
-    the sample is the accuracy checker of the GPU results."""
+def cpu_baseline(srcs, tgts, starts, n_starts, budget_s):
+    """PCL-semantics CPU ICP (float kd-tree, leaf 15, one start per thread) of
+    the plain multi-start path on a bounded sample of the same pairs."""
     import oracle
-    from realtime_robot_amd import multistart, synth
     cores = min(16, os.cpu_count() or 1)
     p = oracle.params(max_iter=MAX_ITER, r_max=R_MAX, eps_T=EPS)
     done, t_cpu = 0, 0.0
     while done < len(srcs) and t_cpu < budget_s:
-        batch = list(range(done, min(done + max(cores // 2, 1), len(srcs))))
+        batch = list(range(done, min(done + max(cores // n_starts, 1), len(srcs))))
         S, T, G = [], [], []
         for i in batch:
             for s in range(n_starts):
                 S.append(srcs[i])
                 T.append(tgts[i])
-                G.append(guesses[i * n_starts + s])
+                G.append(starts[i][s])
         t0 = time.perf_counter()
         oracle.pcl_icp_batch(S, T, p, G, nthreads=cores)
         t_cpu += time.perf_counter() - t0
         done += len(batch)
-    n_chk = min(done, 8)
+    return {"value": done / t_cpu, "unit": "registrations/s", "cores": cores, "kind": "port",
+            "sample": f"first {done} pairs x {n_starts} starts of the same workload (plain multi-start), "
+                      f"PCL-semantics float kd-tree ICP (oracle/oracle.c orc_pcl_icp_batch), one start per "
+                      f"thread, {t_cpu:.1f} s"}
 
-    def oracle_best(i):
-        res = [oracle.icp(srcs[i], tgts[i], p, guess=guesses[i * n_starts + s]) for s in range(n_starts)]
-        return i, res[multistart.select_best(res)]["T"]
+
+def fp64_check(srcs, tgts, gts, T_ref, inits, T_gpu, n_chk):
+    """FP64 oracle on a sample: (a) from the generating pose vs the GPU
+    reference pose, (b) from the same init as the learned path's final ICP vs
+    its GPU result."""
+    import oracle
+    from realtime_robot_amd import synth
+    p = oracle.params(max_iter=MAX_ITER, r_max=R_MAX, eps_T=EPS)
+    cores = min(16, os.cpu_count() or 1)
+
+    def one(i):
+        a = oracle.icp(srcs[i], tgts[i], p, guess=gts[i])["T"]
+        b = oracle.icp(srcs[i], tgts[i], p, guess=inits[i])["T"]
+        return synth.rmse_between(T_ref[i], a, srcs[i]), synth.rmse_between(T_gpu[i], b, srcs[i])
 
     with ThreadPoolExecutor(cores) as ex:
-        refs = list(ex.map(oracle_best, range(n_chk)))
-    rmse = [synth.rmse_between(gpu_best[i], T, srcs[i]) for i, T in refs]
-    base = {"value": done / t_cpu, "unit": "registrations/s", "cores": cores, "kind": "port",
-            "sample": f"{done} pairs x {n_starts} starts of the same workload, PCL-semantics float kd-tree ICP "
-                      f"(oracle/oracle.c orc_pcl_icp_batch), one start per thread, {t_cpu:.1f} s"}
-    acc = {"checked_pairs": n_chk, "max_rmse_vs_fp64_m": float(max(rmse)),
-           "within_1mm": int(sum(r <= 1e-3 for r in rmse))}
-    return base, acc
+        r = list(ex.map(one, range(n_chk)))
+    ra, rb = [x[0] for x in r], [x[1] for x in r]
+    return {"checked_pairs": n_chk, "reference_vs_fp64_max_rmse_m": float(max(ra)),
+            "learned_vs_fp64_same_init_max_rmse_m": float(max(rb)),
+            "within_1mm": int(sum(x <= OK_RMSE for x in rb))}
 
 
 def main():
@@ -88,15 +99,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--starts", type=int, default=8)
+    ap.add_argument("--hypotheses", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
-                    help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile' "
-                         "(skips the autotuner)")
+                    help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--fp64-sample", type=int, default=8)
     ap.add_argument("--account-sample", type=int, default=24)
-    ap.add_argument("--copy-bytes", type=int, default=1 << 30,
-                    help="bytes of the HBM copy probe (calibrates profiler byte counters)")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--leg", choices=["both", "refine"], default="both",
+                    help="'refine': only the learned path's refine launches (for a kernel profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): registrations are independent and "
@@ -105,130 +118,164 @@ def main():
 
     import realtime_robot_amd as R
     from realtime_robot_amd import _lib as L
-    from realtime_robot_amd import accounting, autotune, multistart, synth
+    from realtime_robot_amd import accounting, autotune, learned, multistart, synth
 
     t0 = time.perf_counter()
-    srcs, tgts, gts = synth.config3_batch(args.pairs)
-    S = args.starts
-    guesses = start_guesses(srcs, tgts, S)
-    log(f"workload built in {time.perf_counter() - t0:.1f}s: {args.pairs} pairs, "
+    srcs, tgts, gts = synth.config4_batch(args.pairs)
+    n = args.pairs
+    cs = np.stack([s.astype(np.float64).mean(0) for s in srcs])
+    ct = np.stack([t.astype(np.float64).mean(0) for t in tgts])
+    log(f"workload built in {time.perf_counter() - t0:.1f}s: {n} pairs, "
         f"{sum(len(s) for s in srcs)} source / {sum(len(t) for t in tgts)} target points")
+    params = learned.default_params()
 
-    b = R.BatchRegistration(0)
-    b.set_clouds(srcs, tgts)
-    ids = np.repeat(np.arange(args.pairs), S)
-    b.set_problems(ids, ids)
-    b.set_params(R.make_params(max_iterations=MAX_ITER, max_correspondence_distance=R_MAX,
-                               transformation_epsilon=EPS))
-    n_prob = args.pairs * S
+    # reference poses: GPU ICP from the generating pose
+    ref = R.BatchRegistration()
+    ref.set_clouds(srcs, tgts)
+    ref.set_problems(range(n), range(n))
+    ref.set_params(params)
+    T_ref = [r["T"] for r in ref.align(gts)]
+    del ref
+
+    weights, meta = learned.load_weights()
+    lr = learned.LearnedRegistration(weights, params, n_starts=args.starts, hypotheses=args.hypotheses,
+                                     max_pairs=n)
+    lr.set_clouds(srcs, tgts, cs, ct)
+    ms = None
+    if args.leg == "both":
+        ms = learned.MultiStartRegistration(params, n_starts=args.starts)
+        ms.set_clouds(srcs, tgts, cs, ct)
     tuned = None
     if args.tuning:
         f = [float(v) for v in args.tuning.split(",")]
         tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4]))
-        autotune.apply(b, tuned)
-    elif not args.no_autotune:
-        key = autotune.bucket_key(n_prob, np.mean([len(s) for s in srcs]), np.mean([len(t) for t in tgts]), 0)
-        tuned, tuned_ms, table = autotune.tune(b, guesses, key=None, use_cache=False)
-        log(f"autotune: {tuned} ({tuned_ms:.1f} ms); table {[(c, round(m, 1)) for c, m in table]}")
-    res = (L.Result * n_prob)()
-    g = np.ascontiguousarray(np.transpose(guesses, (0, 2, 1))).reshape(-1)
+    elif not args.no_autotune and ms is not None:
+        tuned, tuned_ms, table = autotune.tune(ms.reg, ms.guesses, key=None, use_cache=False)
+        log(f"autotune (plain multi-start launch): {tuned} ({tuned_ms:.1f} ms)")
+    if tuned is not None:
+        autotune.apply(lr.reg, tuned)
+        if ms is not None:
+            autotune.apply(ms.reg, tuned)
 
-    def step():
-        L.check(L.lib.rtr_batch_align(b._h, g.ctypes.data, res))
-        best = []
-        for p in range(args.pairs):
-            rr = [{"n_corr": res[p * S + s].n_corr, "mse": res[p * S + s].mse, "state": res[p * S + s].state}
-                  for s in range(S)]
-            best.append(p * S + multistart.select_best(rr))
-        return best
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        t_s = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        return (time.perf_counter() - t_s) / steps, out
 
-    for _ in range(args.warmup):
-        step()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        best = step()
-    t_el = time.perf_counter() - t_start
-    ms_step = 1000.0 * t_el / args.steps
-    value = args.pairs / (t_el / args.steps)
-    log(f"{args.steps} steps, {ms_step:.2f} ms/step, {value:.1f} registrations/s")
+    def n_ok(results):
+        return np.array([synth.rmse_between(results[i]["T"], T_ref[i], srcs[i]) <= OK_RMSE for i in range(n)])
 
-    # dominant kernel: the persistent ICP kernel, timed with HIP events on the engine's stream
-    b.set_profiling(True)
+    if args.leg == "refine":
+        # the learned path's refine launch alone, repeated (kernel profiling)
+        _, _, props = lr.prop.propose_multi(lr.rots, lr.tau)
+        lr.reg.set_problems(range(n), range(n))
+        for _ in range(args.warmup + args.steps):
+            lr.reg.align(props)
+        log("refine launches done")
+        return
+
+    t_l, res_l = timed(lr.align, args.steps, args.warmup)
+    st = dict(lr.stats)
+    ok_l = n_ok(res_l)
+    t_p, res_p = timed(ms.align, args.steps, args.warmup)
+    ok_p = n_ok(res_p)
+    value = ok_l.sum() / t_l
+    plain_value = ok_p.sum() / t_p
+    log(f"learned: {t_l * 1e3:.1f} ms/step, {ok_l.sum()}/{n} within 1 mm, {st['n_fallback']} fallbacks "
+        f"-> {value:.1f} reg/s; plain: {t_p * 1e3:.1f} ms/step, {ok_p.sum()}/{n} -> {plain_value:.1f} reg/s")
+    prop_rot = learned.rotation_error_deg(st["proposals"][:, :3, :3], np.stack(gts)[:, :3, :3])
+
+    # dominant kernel: k_icp of the learned path's refine launch, HIP events on the engine's stream
+    lr.reg.set_problems(range(n), range(n))
+    lr.reg.set_profiling(True)
     icp_ms, grid_ms = [], []
     for _ in range(3):
-        L.check(L.lib.rtr_batch_align(b._h, g.ctypes.data, res))
-        t = b.timing()
+        res = lr.reg.align(st["proposals"])
+        t = lr.reg.timing()
         icp_ms.append(t["ms_icp"])
         grid_ms.append(t["ms_grid"])
-    b.set_profiling(False)
-    iters = np.array([res[i].iterations for i in range(n_prob)], np.int64)
-    ns = np.array([len(srcs[i]) for i in ids], np.int64)
-    nt = np.array([len(tgts[i]) for i in ids], np.int64)
+    lr.reg.set_profiling(False)
+    iters = np.array([r["iterations"] for r in res], np.int64)
+    ns = np.array([len(s) for s in srcs], np.int64)
+    nt = np.array([len(t) for t in tgts], np.int64)
     b_min_total = float(np.sum(iters * (12 * ns + 12 * nt)))
-    # B_iter on a seeded sample of problems from the transform of every iteration
     rng = np.random.default_rng(0)
-    sample = rng.choice(n_prob, min(args.account_sample, n_prob), replace=False)
+    sample = rng.choice(n, min(args.account_sample, n), replace=False)
     s_iter = s_min = 0
-    counts = {}
     for p in sample:
-        pair = int(ids[p])
-        if pair not in counts:
-            counts[pair] = accounting.TargetCounts(tgts[pair], R_MAX)
-        hist = b.history(int(p), MAX_ITER)
-        for it in range(res[p].iterations):
-            bi, bm = accounting.iteration_bytes(srcs[pair], counts[pair], hist[it])
+        counts = accounting.TargetCounts(tgts[p], R_MAX)
+        hist = lr.reg.history(int(p), MAX_ITER)
+        for it in range(res[p]["iterations"]):
+            bi, bm = accounting.iteration_bytes(srcs[p], counts, hist[it])
             s_iter += bi
             s_min += bm
     ratio = s_iter / s_min if s_min else 1.0
     b_iter_total = b_min_total * ratio
     t_icp = float(np.median(icp_ms)) / 1e3
     achieved = b_iter_total / t_icp / 1e9
-    import ctypes
     copy_ms = ctypes.c_float()
     L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 5, ctypes.byref(copy_ms)))
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms.value / 1e3) / 1e9
 
-    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this
-    # workload (copy-probe calibrated); reported only when the tuning matches the profiled one
     traffic, traffic_src = None, None
-    prof = os.path.join(ROOT, "profiles", "k_icp_multistart_256x8.json")
-    if os.path.exists(prof) and tuned is not None and args.pairs == 256 and S == 8:
+    prof = os.path.join(ROOT, "profiles", "k_icp_learned_refine_256.json")
+    if os.path.exists(prof) and tuned is not None and n == 256:
         with open(prof) as f:
             pj = json.load(f)
         if [round(v, 4) for v in pj["tuning"]] == [round(float(v), 4) for v in tuned]:
-            traffic, traffic_src = float(pj["traffic_bytes"]), "profiles/k_icp_multistart_256x8.md"
+            traffic, traffic_src = float(pj["traffic_bytes"]), "profiles/k_icp_learned_refine_256.md"
 
-    best_T = [L.rowmajor(res[k].T) for k in best]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": float(value),
         "unit": "registrations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": t_l * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 geometry, fp64 solves",
+        "dtype": "fp32 geometry, fp64 solves, bf16 network (fp32 accumulation)",
         "data": "synthetic",
-        "config": {"workload": f"batched plain multi-start point-to-point ICP, {args.pairs} box-composite pairs "
-                               f"(Ns logU[2k,50k], Nt logU[10k,200k], init rot U[0,30deg]), {S} starts, "
-                               f"r_max {R_MAX} m, <={MAX_ITER} iterations, eps {EPS}",
-                   "pairs": args.pairs, "starts": S, "problems": n_prob, "iterations_total": int(iters.sum()),
-                   "tuning": list(tuned) if tuned else None},
+        "config": {"workload": f"learned pose proposal + ICP refine vs plain {args.starts}-start ICP: {n} "
+                               f"asymmetric box-composite pairs (Ns logU[2k,50k], Nt logU[10k,200k], noise 1 mm, "
+                               f"rotation U[0,180deg], t U[-0.1,0.1] m), r_max {R_MAX} m, <={MAX_ITER} iterations, "
+                               f"eps {EPS}; counted when within {OK_RMSE * 1e3:.0f} mm RMSE of the reference pose",
+                   "pairs": n, "hypotheses": args.hypotheses, "starts": args.starts,
+                   "tuning": list(tuned) if tuned else None,
+                   "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],
+                   "learned_ms_per_step": t_l * 1e3, "learned_ms_propose": st["ms_propose"],
+                   "learned_ms_propose_device": st["ms_propose_device"], "learned_ms_refine_judge": st["ms_refine"],
+                   "learned_ms_fallback": st["ms_fallback"],
+                   "proposal_rot_err_deg_median": float(np.median(prop_rot)),
+                   "proposal_rot_err_deg_p90": float(np.percentile(prop_rot, 90)),
+                   "plain_value": float(plain_value), "plain_ok": int(ok_p.sum()), "plain_ms_per_step": t_p * 1e3,
+                   "speedup_vs_plain": float(value / plain_value) if plain_value > 0 else None,
+                   "weights": {k: meta[k] for k in ("base_pairs", "pairs_seen", "steps")}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_icp (persistent correspondence search + moments + FP64 solve)",
+                     "kernel": "k_icp (persistent correspondence search + moments + FP64 solve), learned-path "
+                               "refine launch (256 problems)",
                      "kernel_ms": t_icp * 1e3, "grid_build_ms": float(np.median(grid_ms)),
                      "algorithmic_bytes": b_iter_total, "compulsory_bytes": b_min_total,
-                     "achieved_compulsory_gbs": b_min_total / t_icp / 1e9,
-                     "b_iter_over_b_min": ratio, "b_iter_sample_problems": int(len(sample)),
+                     "achieved_compulsory_gbs": b_min_total / t_icp / 1e9, "b_iter_over_b_min": ratio,
+                     "iterations_total": int(iters.sum()), "b_iter_sample_problems": int(len(sample)),
                      "hbm_copy_gbs": copy_gbs, "copy_probe_bytes": args.copy_bytes},
     }
     if not args.no_cpu_baseline:
-        cb, acc = cpu_baseline(srcs, tgts, guesses, S, best_T, args.cpu_budget)
-        out["cpu_baseline"] = cb
+        starts = [multistart.so3_start_poses(cs[i], ct[i], args.starts) for i in range(n)]
+        out["cpu_baseline"] = cpu_baseline(srcs, tgts, starts, args.starts, args.cpu_budget)
+        rejected = set(np.where((st["judge_rmse"] > lr.rmse_threshold) |
+                                (st["judge_frac"] < lr.min_inlier_frac))[0].tolist())
+        # the init the learned path's final ICP ran from (fallback pairs are left out)
+        inits = [None if i in rejected else st["proposals"][i] for i in range(n)]
+        chk = [i for i in range(n) if inits[i] is not None][:args.fp64_sample]
+        acc = fp64_check([srcs[i] for i in chk], [tgts[i] for i in chk], [gts[i] for i in chk],
+                         [T_ref[i] for i in chk], [inits[i] for i in chk], [res_l[i]["T"] for i in chk], len(chk))
         out["accuracy"] = acc
     print(json.dumps(out), flush=True)
 

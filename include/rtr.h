@@ -273,7 +273,17 @@ int rtr_prop_fps(rtr_prop* h, float* pts, float* centroids, int32_t* sel);
 /* FPS + network on the resident clouds: one 4x4 pose per pair (n*16 doubles,
  * column-major) mapping the source onto the target. */
 int rtr_prop_propose(rtr_prop* h, double* poses);
-/* Device time of the last rtr_prop_propose (FPS + network), ms. */
+/* k hypotheses per pair (1 <= k <= 64; k*n <= max_pairs): the network on the
+ * source samples rotated by each rots[j] (k row-major 3x3), candidate j's pose
+ * composed with rots[j], each candidate scored by the truncated chamfer of
+ * the farthest-point samples: mean over the source samples of
+ * min(squared distance to the nearest target sample, tau^2).  Outputs are
+ * pair-major (p*k + j): poses (n*k*16, column-major, nullable), scores (n*k,
+ * nullable), and best (n*16), the lowest-scoring candidate of each pair
+ * (lowest j on ties). */
+int rtr_prop_propose_multi(rtr_prop* h, int k, const double* rots, float tau, double* poses,
+                           double* scores, double* best);
+/* Device time of the last rtr_prop_propose / rtr_prop_propose_multi, ms. */
 int rtr_prop_last_ms(rtr_prop* h, float* ms);
 /* Network on given centred FPS points (n*2*1024*3, clouds src0, tgt0, ...).
  * Outputs (each nullable): head outputs O (n*9), poses R|t (n*12 doubles, R

@@ -44,6 +44,7 @@ constexpr int kP = 1024;      // points per cloud after FPS
 constexpr int kC0 = 16;       // input channels (xyz padded to the MFMA K granularity)
 constexpr int kC1 = 64, kC2 = 128, kC3 = 256, kH = 256, kOut = 9;
 constexpr int kFpsMax = 8192; // stride pre-subsample before FPS
+constexpr int kMaxHyp = 64;   // hypothesis rotations per proposal call
 
 // ---------------------------------------------------------------------------
 // GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N] (+ bias[N]) (ReLU)
@@ -454,6 +455,89 @@ __global__ void k_to_bf16(const float* in, size_t n, uint16_t* out) {
     out[i] = f2bf(in[i]);
 }
 
+// Hypothesis inputs: the n source FPS clouds rotated by each of the k
+// rotations, then the n target clouds once:
+// clouds [Q_0 src_0 .. Q_0 src_{n-1}, Q_1 src_0, .., Q_{k-1} src_{n-1}, tgt_0 .. tgt_{n-1}]
+// (pts holds the FPS clouds as src0, tgt0, src1, ...; rots k row-major 3x3).
+__global__ void k_pack_multi(const float* pts, int n, int k, const float* rots, uint16_t* x0) {
+  const size_t total = (size_t)(k + 1) * n * kP * kC0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / kC0;
+    const int c = (int)(i - r * kC0);
+    const size_t cl = r / kP, row = r - cl * kP;
+    float v = 0.f;
+    if (c < 3) {
+      if (cl < (size_t)k * n) {
+        const size_t j = cl / n, p = cl - j * n;
+        const float* q = pts + ((2 * p) * kP + row) * 3;
+        const float* Q = rots + 9 * j + 3 * c;
+        v = Q[0] * q[0];
+        v = v + Q[1] * q[1];
+        v = v + Q[2] * q[2];
+      } else {
+        const size_t p = cl - (size_t)k * n;
+        v = pts[((2 * p + 1) * kP + row) * 3 + c];
+      }
+    }
+    x0[i] = f2bf(v);
+  }
+}
+
+// F row r = j*n + p: (G of rotated source cloud r, G of target cloud k*n + p)
+__global__ void k_concat_multi(const float* G, int n, int k, uint16_t* F) {
+  const int r = blockIdx.x, ch = threadIdx.x;  // 512 threads
+  if (r >= k * n) return;
+  const int p = r % n;
+  const float v = ch < kC3 ? G[(size_t)r * kC3 + ch] : G[((size_t)k * n + p) * kC3 + (ch - kC3)];
+  F[(size_t)r * (2 * kC3) + ch] = f2bf(v);
+}
+
+// Candidate score: truncated chamfer between the pair's source FPS points
+// moved by the candidate (R_r Q_j, t_r) and its target FPS points, both in
+// their centred frames: mean over the 1024 source points of
+// min(min_q |R s + t - q|^2, tau2).  One workgroup per candidate r = j*n + p;
+// the target points are staged in LDS.
+__global__ __launch_bounds__(256) void k_score(const float* pts, const double* pose, const float* rots, int n,
+                                               int k, float tau2, double* score) {
+  __shared__ float tx[kP], ty[kP], tz[kP];
+  __shared__ double red[4];
+  const int r = blockIdx.x;
+  if (r >= k * n) return;
+  const int j = r / n, p = r - j * n;
+  const float* tp = pts + (size_t)(2 * p + 1) * kP * 3;
+  for (int i = threadIdx.x; i < kP; i += 256) {
+    tx[i] = tp[3 * i];
+    ty[i] = tp[3 * i + 1];
+    tz[i] = tp[3 * i + 2];
+  }
+  const double* P = pose + (size_t)r * 12;
+  const float* Q = rots + 9 * j;
+  float M[9];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      M[a * 3 + b] = (float)(P[a * 3] * Q[b] + P[a * 3 + 1] * Q[3 + b] + P[a * 3 + 2] * Q[6 + b]);
+  const float t0 = (float)P[9], t1 = (float)P[10], t2 = (float)P[11];
+  __syncthreads();
+  const float* sp = pts + (size_t)(2 * p) * kP * 3;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kP; i += 256) {
+    const float sx = sp[3 * i], sy = sp[3 * i + 1], sz = sp[3 * i + 2];
+    const float qx = M[0] * sx + M[1] * sy + M[2] * sz + t0;
+    const float qy = M[3] * sx + M[4] * sy + M[5] * sz + t1;
+    const float qz = M[6] * sx + M[7] * sy + M[8] * sz + t2;
+    float best = tau2;
+    for (int q = 0; q < kP; ++q) {
+      const float dx = qx - tx[q], dy = qy - ty[q], dz = qz - tz[q];
+      best = fminf(best, dx * dx + dy * dy + dz * dz);
+    }
+    acc += best;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) score[r] = (red[0] + red[1] + red[2] + red[3]) / kP;
+}
+
 // xyz rows (FP32) -> network input rows of kC0 bf16 (channels 3.. zero)
 __global__ void k_pack_x0(const float* xyz, size_t rows, uint16_t* x0) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows * kC0; i += (size_t)gridDim.x * blockDim.x) {
@@ -507,16 +591,20 @@ struct ParamLayout {
 };
 
 struct Proposer {
-  int device = 0, max_pairs = 0, n_pairs = 0;
+  int device = 0, max_pairs = 0;
+  int n_pairs = 0;     // pairs of the batch in the network buffers
+  int n_resident = 0;  // pairs of the resident raw clouds (set_clouds)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ParamLayout L;
   int64_t step = 0;
-  DevBuf w, wb, g, gb_unused, m, v;
+  DevBuf w, wb, g, m, v;
   DevBuf raw, roff, rcnt;           // resident raw clouds for proposals
   DevBuf pts, x0, cen, sel;         // FPS outputs
   DevBuf y1, y2, y3, G, arg, F, h1, O, pose, lab, dO, loss;
-  DevBuf dtmp, dbf, dG, dF, dH1, dH1b, dOb, dbias;
+  DevBuf dtmp, dbf, dG, dF, dH1, dH1b, dOb;
+  DevBuf rots, score;               // hypothesis rotations, candidate scores
+  bool train_ready = false;
   std::vector<uint32_t> h_off, h_cnt;
   float last_ms = 0.f;
 
@@ -535,10 +623,8 @@ struct Proposer {
     RTR_HIP(hipEventCreate(&ev1));
     const size_t P = L.total;
     const size_t clouds = 2 * (size_t)maxp, rows = clouds * kP;
-    int rc = 0;
-    DevBuf* bufs[] = {&w, &wb, &g, &m, &v};
-    const size_t sz[] = {P * 4, P * 2, P * 4, P * 4, P * 4};
-    for (int i = 0; i < 5 && !rc; ++i) rc = bufs[i]->ensure(sz[i]);
+    int rc = w.ensure(P * 4);
+    if (!rc) rc = wb.ensure(P * 2);
     if (!rc) rc = pts.ensure(rows * 12);
     if (!rc) rc = x0.ensure(rows * kC0 * 2);
     if (!rc) rc = cen.ensure(clouds * 12);
@@ -552,27 +638,44 @@ struct Proposer {
     if (!rc) rc = h1.ensure((size_t)maxp * kH * 2);
     if (!rc) rc = O.ensure((size_t)maxp * kOut * 4);
     if (!rc) rc = pose.ensure((size_t)maxp * 12 * 8);
-    if (!rc) rc = lab.ensure((size_t)maxp * 12 * 8);
-    if (!rc) rc = dO.ensure((size_t)maxp * kOut * 4);
-    if (!rc) rc = loss.ensure((size_t)maxp * 4);
+    if (!rc) rc = rots.ensure(kMaxHyp * 9 * 4);
+    if (!rc) rc = score.ensure((size_t)maxp * 8);
+    return rc;
+  }
+
+  // Gradient, optimiser and backward buffers: allocated on the first
+  // training call only (an inference-only proposer never holds them).
+  int ensure_train() {
+    if (train_ready) return RTR_OK;
+    const size_t P = L.total;
+    const size_t clouds = 2 * (size_t)max_pairs, rows = clouds * kP;
+    const size_t maxp = (size_t)max_pairs;
+    int rc = g.ensure(P * 4);
+    if (!rc) rc = m.ensure(P * 4);
+    if (!rc) rc = v.ensure(P * 4);
+    if (!rc) rc = lab.ensure(maxp * 12 * 8);
+    if (!rc) rc = dO.ensure(maxp * kOut * 4);
+    if (!rc) rc = loss.ensure(maxp * 4);
     if (!rc) rc = dtmp.ensure(rows * kC3 * 4);
     if (!rc) rc = dbf.ensure(rows * kC3 * 2);
     if (!rc) rc = dG.ensure(clouds * kC3 * 4);
-    if (!rc) rc = dF.ensure((size_t)maxp * 2 * kC3 * 4);
-    if (!rc) rc = dH1.ensure((size_t)maxp * kH * 4);
-    if (!rc) rc = dH1b.ensure((size_t)maxp * kH * 2);
-    if (!rc) rc = dOb.ensure((size_t)maxp * 16 * 2);
+    if (!rc) rc = dF.ensure(maxp * 2 * kC3 * 4);
+    if (!rc) rc = dH1.ensure(maxp * kH * 4);
+    if (!rc) rc = dH1b.ensure(maxp * kH * 2);
+    if (!rc) rc = dOb.ensure(maxp * 16 * 2);
     if (rc) return rc;
     RTR_HIP(hipMemset(m.p, 0, P * 4));
     RTR_HIP(hipMemset(v.p, 0, P * 4));
+    train_ready = true;
     return RTR_OK;
   }
 
   ~Proposer() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    DevBuf* all[] = {&w, &wb, &g, &m, &v, &raw, &roff, &rcnt, &pts, &x0, &cen, &sel, &y1, &y2, &y3, &G,
-                     &arg, &F, &h1, &O, &pose, &lab, &dO, &loss, &dtmp, &dbf, &dG, &dF, &dH1, &dH1b, &dOb};
+    DevBuf* all[] = {&w,  &wb,  &g,  &m,  &v,   &raw, &roff, &rcnt, &pts, &x0,  &cen,  &sel,  &y1,
+                     &y2, &y3,  &G,  &arg, &F,  &h1,  &O,    &pose, &lab, &dO,  &loss, &dtmp, &dbf,
+                     &dG, &dF,  &dH1, &dH1b, &dOb, &rots, &score};
     for (DevBuf* b : all) b->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -600,6 +703,7 @@ struct Proposer {
       return RTR_ERR_INVALID_ARG;
     }
     n_pairs = np;
+    n_resident = 0;
     h_off.assign(2 * np, 0);
     h_cnt.assign(2 * np, 0);
     size_t tot = 0;
@@ -624,10 +728,14 @@ struct Proposer {
     RTR_HIP(hipMemcpyAsync(roff.p, h_off.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
     RTR_HIP(hipMemcpyAsync(rcnt.p, h_cnt.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
     RTR_HIP(hipStreamSynchronize(stream));
+    n_resident = np;
     return RTR_OK;
   }
 
+  // FPS of the resident clouds into the network buffers (n_pairs becomes the
+  // resident count).
   void enqueue_fps() {
+    n_pairs = n_resident;
     hipLaunchKernelGGL(k_fps, dim3(2 * n_pairs), dim3(256), 0, stream, raw.as<float>(), roff.as<uint32_t>(),
                        rcnt.as<uint32_t>(), pts.as<float>(), x0.as<uint16_t>(), cen.as<float>(),
                        sel.as<int32_t>());
@@ -665,6 +773,27 @@ struct Proposer {
     hipLaunchKernelGGL(k_head, dim3((B + 63) / 64), dim3(64), 0, stream, O.as<float>(), B, pts.as<float>(),
                        with_labels ? lab.as<double>() : nullptr, pose.as<double>(), dO.as<float>(),
                        loss.as<float>());
+  }
+
+  // Network over k hypotheses per resident pair (after enqueue_fps): point
+  // MLP over (k+1)*n clouds, head over k*n rows, then candidate scores.
+  void enqueue_forward_multi(int k, float tau2) {
+    const int n = n_pairs, C = (k + 1) * n, B = k * n;
+    const int rows = C * kP;
+    hipLaunchKernelGGL(k_pack_multi, dim3(2048), dim3(256), 0, stream, pts.as<float>(), n, k, rots.as<float>(),
+                       x0.as<uint16_t>());
+    gemm(stream, x0.as<uint16_t>(), kC0, 0, wbp(L.W1), kC1, 0, y1.p, kC1, rows, kC1, kC0, wp(L.b1), 1, 1);
+    gemm(stream, y1.as<uint16_t>(), kC1, 0, wbp(L.W2), kC2, 0, y2.p, kC2, rows, kC2, kC1, wp(L.b2), 1, 1);
+    gemm(stream, y2.as<uint16_t>(), kC2, 0, wbp(L.W3), kC3, 0, y3.p, kC3, rows, kC3, kC2, wp(L.b3), 1, 1);
+    hipLaunchKernelGGL(k_maxpool, dim3(C, 1), dim3(kC3), 0, stream, y3.as<uint16_t>(), kC3, C, G.as<float>(),
+                       arg.as<int32_t>());
+    hipLaunchKernelGGL(k_concat_multi, dim3(B), dim3(2 * kC3), 0, stream, G.as<float>(), n, k, F.as<uint16_t>());
+    gemm(stream, F.as<uint16_t>(), 2 * kC3, 0, wbp(L.W4), kH, 0, h1.p, kH, B, kH, 2 * kC3, wp(L.b4), 1, 1);
+    gemm(stream, h1.as<uint16_t>(), kH, 0, wbp(L.W5), kOut, 0, O.p, kOut, B, kOut, kH, wp(L.b5), 0, 0);
+    hipLaunchKernelGGL(k_head, dim3((B + 63) / 64), dim3(64), 0, stream, O.as<float>(), B, pts.as<float>(),
+                       nullptr, pose.as<double>(), nullptr, nullptr);
+    hipLaunchKernelGGL(k_score, dim3(B), dim3(256), 0, stream, pts.as<float>(), pose.as<double>(),
+                       rots.as<float>(), n, k, tau2, score.as<double>());
   }
 
   // gradients of the mean loss into g (FP32)
@@ -780,7 +909,7 @@ int rtr_prop_propose(rtr_prop* h, double* poses) {
   if (!h || !poses) return RTR_ERR_INVALID_ARG;
   Proposer& P = h->p;
   RTR_HIP(hipSetDevice(P.device));
-  if (P.n_pairs < 1 || P.h_cnt.empty()) {
+  if (P.n_resident < 1) {
     set_error("no clouds set");
     return RTR_ERR_NOT_READY;
   }
@@ -818,6 +947,74 @@ int rtr_prop_last_ms(rtr_prop* h, float* ms) {
   return RTR_OK;
 }
 
+// k hypotheses per resident pair: the network on the source FPS points
+// rotated by each rots[j] (k row-major 3x3), candidate j's pose composed with
+// rots[j], every candidate scored by the truncated chamfer (tau) of the FPS
+// points; outputs pair-major (p*k + j): poses (n*k*16 column-major, nullable),
+// scores (n*k, nullable), and the best-scoring pose per pair (n*16; lowest j
+// on ties).
+int rtr_prop_propose_multi(rtr_prop* h, int k, const double* rots_in, float tau, double* poses, double* scores,
+                           double* best) {
+  using namespace rtr;
+  if (!h || !rots_in || !best || k < 1 || k > kMaxHyp || !(tau > 0.f)) return RTR_ERR_INVALID_ARG;
+  Proposer& P = h->p;
+  RTR_HIP(hipSetDevice(P.device));
+  if (P.n_resident < 1) {
+    set_error("no clouds set");
+    return RTR_ERR_NOT_READY;
+  }
+  const int n = P.n_resident;
+  if ((size_t)k * n > (size_t)P.max_pairs || (size_t)(k + 1) * n > 2 * (size_t)P.max_pairs) {
+    set_error("%d hypotheses x %d pairs exceed the proposer's capacity (%d pairs)", k, n, P.max_pairs);
+    return RTR_ERR_CAPACITY;
+  }
+  float rf[kMaxHyp * 9];
+  for (int i = 0; i < 9 * k; ++i) rf[i] = (float)rots_in[i];
+  RTR_HIP(hipMemcpyAsync(P.rots.p, rf, (size_t)k * 9 * 4, hipMemcpyHostToDevice, P.stream));
+  RTR_HIP(hipEventRecord(P.ev0, P.stream));
+  P.enqueue_fps();
+  P.enqueue_forward_multi(k, tau * tau);
+  RTR_HIP(hipGetLastError());
+  RTR_HIP(hipEventRecord(P.ev1, P.stream));
+  std::vector<double> pose((size_t)k * n * 12), sc((size_t)k * n);
+  std::vector<float> cen((size_t)2 * n * 3);
+  RTR_HIP(hipMemcpyAsync(pose.data(), P.pose.p, pose.size() * 8, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipMemcpyAsync(sc.data(), P.score.p, sc.size() * 8, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipMemcpyAsync(cen.data(), P.cen.p, cen.size() * 4, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipStreamSynchronize(P.stream));
+  RTR_HIP(hipEventElapsedTime(&P.last_ms, P.ev0, P.ev1));
+  for (int p = 0; p < n; ++p) {
+    const float* cs = &cen[(size_t)(2 * p) * 3];
+    const float* ct = &cen[(size_t)(2 * p + 1) * 3];
+    int jb = 0;
+    for (int j = 0; j < k; ++j) {
+      const size_t r = (size_t)j * n + p;
+      if (sc[r] < sc[(size_t)jb * n + p]) jb = j;
+      if (scores) scores[(size_t)p * k + j] = sc[r];
+    }
+    for (int j = 0; j < k; ++j) {
+      if (!poses && j != jb) continue;
+      const size_t r = (size_t)j * n + p;
+      const double* Rn = &pose[r * 12];
+      const double* t = Rn + 9;
+      const double* Q = rots_in + 9 * j;
+      double Rm[9];  // R = Rn Q
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Rm[a * 3 + b] = Rn[a * 3] * Q[b] + Rn[a * 3 + 1] * Q[3 + b] + Rn[a * 3 + 2] * Q[6 + b];
+      double T[16];
+      for (int i = 0; i < 16; ++i) T[i] = 0.0;
+      T[15] = 1.0;
+      for (int a = 0; a < 3; ++a) {
+        for (int b = 0; b < 3; ++b) T[b * 4 + a] = Rm[a * 3 + b];
+        T[12 + a] = t[a] + ct[a] - (Rm[a * 3] * cs[0] + Rm[a * 3 + 1] * cs[1] + Rm[a * 3 + 2] * cs[2]);
+      }
+      if (poses) memcpy(poses + ((size_t)p * k + j) * 16, T, sizeof(T));
+      if (j == jb) memcpy(best + (size_t)p * 16, T, sizeof(T));
+    }
+  }
+  return RTR_OK;
+}
+
 // FPS of the resident clouds: centred points (2*n_pairs*1024*3), centroids
 // (2*n_pairs*3) and selected subsample indices (nullable).
 int rtr_prop_fps(rtr_prop* h, float* pts, float* centroids, int32_t* sel) {
@@ -825,7 +1022,7 @@ int rtr_prop_fps(rtr_prop* h, float* pts, float* centroids, int32_t* sel) {
   if (!h || !pts) return RTR_ERR_INVALID_ARG;
   Proposer& P = h->p;
   RTR_HIP(hipSetDevice(P.device));
-  if (P.h_cnt.empty()) {
+  if (P.n_resident < 1) {
     set_error("no clouds set");
     return RTR_ERR_NOT_READY;
   }
@@ -849,7 +1046,8 @@ int rtr_prop_forward_backward(rtr_prop* h, int n_pairs, const float* xyz, const 
   using namespace rtr;
   if (!h || !xyz) return RTR_ERR_INVALID_ARG;
   Proposer& P = h->p;
-  int rc = P.set_points(n_pairs, xyz);
+  int rc = labels ? P.ensure_train() : RTR_OK;
+  if (!rc) rc = P.set_points(n_pairs, xyz);
   if (rc) return rc;
   if (labels) RTR_HIP(hipMemcpyAsync(P.lab.p, labels, (size_t)n_pairs * 12 * 8, hipMemcpyHostToDevice, P.stream));
   P.enqueue_forward(labels != nullptr);
@@ -874,7 +1072,8 @@ int rtr_prop_train_step(rtr_prop* h, int n_pairs, const float* xyz, const double
   using namespace rtr;
   if (!h || !xyz || !labels) return RTR_ERR_INVALID_ARG;
   Proposer& P = h->p;
-  int rc = P.set_points(n_pairs, xyz);
+  int rc = P.ensure_train();
+  if (!rc) rc = P.set_points(n_pairs, xyz);
   if (rc) return rc;
   RTR_HIP(hipMemcpyAsync(P.lab.p, labels, (size_t)n_pairs * 12 * 8, hipMemcpyHostToDevice, P.stream));
   P.enqueue_forward(true);

@@ -142,6 +142,18 @@ class Proposer:
         L.check(L.lib.rtr_prop_propose(self._h, L.ptr(cm)))
         return np.ascontiguousarray(np.transpose(cm.reshape(-1, 4, 4), (0, 2, 1)))
 
+    def propose_multi(self, rotations, tau: float = 0.1):
+        """k hypotheses per pair from the source samples rotated by each of
+        `rotations` (k, 3, 3): (poses (n, k, 4, 4), scores (n, k), best (n, 4, 4))."""
+        R = np.ascontiguousarray(rotations, np.float64).reshape(-1, 9)
+        k, n = len(R), self.n_pairs
+        cm = np.empty((n, k, 16))
+        sc = np.empty((n, k))
+        best = np.empty((n, 16))
+        L.check(L.lib.rtr_prop_propose_multi(self._h, k, L.ptr(R), float(tau), L.ptr(cm), L.ptr(sc), L.ptr(best)))
+        poses = np.ascontiguousarray(np.transpose(cm.reshape(n, k, 4, 4), (0, 1, 3, 2)))
+        return poses, sc, np.ascontiguousarray(np.transpose(best.reshape(n, 4, 4), (0, 2, 1)))
+
     def last_ms(self) -> float:
         ms = C.c_float()
         L.check(L.lib.rtr_prop_last_ms(self._h, C.byref(ms)))
@@ -354,15 +366,24 @@ def default_params():
 
 
 class LearnedRegistration:
-    """Learned proposal -> ICP refine -> judge -> multi-start fallback."""
+    """Learned proposal -> ICP refine -> judge -> multi-start fallback.
+
+    With hypotheses = k > 1 the network runs on the source samples rotated by
+    k fixed orientations spread over SO(3) (the first the identity) and the
+    candidate with the lowest truncated-chamfer score (rtr_prop_propose_multi)
+    is the proposal ICP refines."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 2048,
-                 max_curvature: float = 0.05, max_pairs: int = 256, device: int = 0):
+                 max_curvature: float = 0.02, hypotheses: int = 8, tau: float = 0.1, max_pairs: int = 256,
+                 device: int = 0):
         if weights is None:
             weights, _ = load_weights()
         self.params = params if params is not None else default_params()
-        self.prop = Proposer(max_pairs=max_pairs, device=device, weights=weights)
+        self.hypotheses = hypotheses
+        self.rots = multistart.so3_rotations(hypotheses) if hypotheses > 1 else None
+        self.tau = tau
+        self.prop = Proposer(max_pairs=max_pairs * max(1, hypotheses), device=device, weights=weights)
         self.reg = BatchRegistration(device)
         self.reg.set_params(self.params)
         self.n_starts = n_starts
@@ -383,7 +404,10 @@ class LearnedRegistration:
 
     def align(self):
         t0 = time.perf_counter()
-        poses = self.prop.propose()
+        if self.hypotheses > 1:
+            _, _, poses = self.prop.propose_multi(self.rots, self.tau)
+        else:
+            poses = self.prop.propose()
         t1 = time.perf_counter()
         self.reg.set_problems(self._ids, self._ids)
         res = self.reg.align(poses)

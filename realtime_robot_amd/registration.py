@@ -274,7 +274,7 @@ class BatchRegistration:
         L.check(L.lib.rtr_batch_fitness(self._h, float(max_range), L.ptr(out)))
         return out
 
-    def judge(self, k: int = 20, max_points: int = 2048, max_curvature: float = 0.05):
+    def judge(self, k: int = 20, max_points: int = 2048, max_curvature: float = 0.02):
         """(plane_rmse, inlier_frac, planar_frac) per problem at its final
         transform: RMS distance of the inliers (nearest target point within
         the max correspondence distance) to their k-NN PCA plane, counted
