@@ -152,10 +152,18 @@ def main():
     elif not args.no_autotune and ms is not None:
         tuned, tuned_ms, table = autotune.tune(ms.reg, ms.guesses, key=None, use_cache=False)
         log(f"autotune (plain multi-start launch): {tuned} ({tuned_ms:.1f} ms)")
-    if tuned is not None:
+    tuned_refine = tuned
+    if tuned is not None and ms is not None:
+        autotune.apply(ms.reg, tuned)
+    if not args.tuning and not args.no_autotune:
+        # the refine launch (one problem per pair) is tuned on its own launch
+        _, _, props = lr.prop.propose_multi(lr.rots, lr.tau)
+        lr.reg.set_problems(range(n), range(n))
+        tuned_refine, tr_ms, _ = autotune.tune(lr.reg, props, candidates=autotune.LATENCY_CANDIDATES, key=None,
+                                               use_cache=False)
+        log(f"autotune (learned refine launch): {tuned_refine} ({tr_ms:.1f} ms)")
+    elif tuned is not None:
         autotune.apply(lr.reg, tuned)
-        if ms is not None:
-            autotune.apply(ms.reg, tuned)
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):
@@ -222,10 +230,10 @@ def main():
 
     traffic, traffic_src = None, None
     prof = os.path.join(ROOT, "profiles", "k_icp_learned_refine_256.json")
-    if os.path.exists(prof) and tuned is not None and n == 256:
+    if os.path.exists(prof) and tuned_refine is not None and n == 256:
         with open(prof) as f:
             pj = json.load(f)
-        if [round(v, 4) for v in pj["tuning"]] == [round(float(v), 4) for v in tuned]:
+        if [round(v, 4) for v in pj["tuning"]] == [round(float(v), 4) for v in tuned_refine]:
             traffic, traffic_src = float(pj["traffic_bytes"]), "profiles/k_icp_learned_refine_256.md"
 
     out = {
@@ -247,6 +255,7 @@ def main():
                                f"eps {EPS}; counted when within {OK_RMSE * 1e3:.0f} mm RMSE of the reference pose",
                    "pairs": n, "hypotheses": args.hypotheses, "starts": args.starts,
                    "tuning": list(tuned) if tuned else None,
+                   "tuning_refine": list(tuned_refine) if tuned_refine else None,
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],
                    "learned_ms_per_step": t_l * 1e3, "learned_ms_propose": st["ms_propose"],
                    "learned_ms_propose_device": st["ms_propose_device"], "learned_ms_refine_judge": st["ms_refine"],

@@ -32,6 +32,15 @@ CANDIDATES = [
     (0.25, 2.0, 512, 0, 0),
     (0.25, 2.0, 1024, 2, 0),
 ]
+# Few problems per launch (one refine per pair, a small fallback set): the
+# launch is bound by each problem's per-iteration latency, so smaller work
+# items (more workgroups per problem per iteration) are tried as well.
+LATENCY_CANDIDATES = CANDIDATES + [
+    (0.25, 2.0, 256, 0, 0),
+    (0.35, 2.0, 256, 0, 0),
+    (0.2, 3.0, 256, 0, 0),
+    (0.35, 2.0, 512, 0, 0),
+]
 
 
 def _arch() -> str:
