@@ -150,7 +150,8 @@ def main():
         f = [float(v) for v in args.tuning.split(",")]
         tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4]))
     elif not args.no_autotune and ms is not None:
-        tuned, tuned_ms, table = autotune.tune(ms.reg, ms.guesses, key=None, use_cache=False)
+        tuned, tuned_ms, table = autotune.tune(ms.reg, ms.guesses, candidates=autotune.LATENCY_CANDIDATES, key=None,
+                                               use_cache=False)
         log(f"autotune (plain multi-start launch): {tuned} ({tuned_ms:.1f} ms)")
     tuned_refine = tuned
     if tuned is not None and ms is not None:
