@@ -129,13 +129,15 @@ def main():
         f"{sum(len(s) for s in srcs)} source / {sum(len(t) for t in tgts)} target points")
     params = learned.default_params()
 
-    # reference poses: GPU ICP from the generating pose
-    ref = R.BatchRegistration()
-    ref.set_clouds(srcs, tgts)
-    ref.set_problems(range(n), range(n))
-    ref.set_params(params)
-    T_ref = [r["T"] for r in ref.align(gts)]
-    del ref
+    T_ref = None
+    if args.leg == "both":
+        # reference poses: GPU ICP from the generating pose
+        ref = R.BatchRegistration()
+        ref.set_clouds(srcs, tgts)
+        ref.set_problems(range(n), range(n))
+        ref.set_params(params)
+        T_ref = [r["T"] for r in ref.align(gts)]
+        del ref
 
     weights, meta = learned.load_weights()
     lr = learned.LearnedRegistration(weights, params, n_starts=args.starts, hypotheses=args.hypotheses,
@@ -178,12 +180,23 @@ def main():
         return np.array([synth.rmse_between(results[i]["T"], T_ref[i], srcs[i]) <= OK_RMSE for i in range(n)])
 
     if args.leg == "refine":
-        # the learned path's refine launch alone, repeated (kernel profiling)
+        # the learned path's refine launch alone, repeated (kernel profiling):
+        # graph-replayed launches, then HIP-event-timed ones, then the copy
+        # probe that calibrates the profiler's byte counters
         _, _, props = lr.prop.propose_multi(lr.rots, lr.tau)
         lr.reg.set_problems(range(n), range(n))
         for _ in range(args.warmup + args.steps):
+            res = lr.reg.align(props)
+        lr.reg.set_profiling(True)
+        icp_ms = []
+        for _ in range(3):
             lr.reg.align(props)
-        log("refine launches done")
+            icp_ms.append(lr.reg.timing()["ms_icp"])
+        lr.reg.set_profiling(False)
+        copy_ms = ctypes.c_float()
+        L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 3, ctypes.byref(copy_ms)))
+        log(f"refine launches done: k_icp HIP-event median {np.median(icp_ms):.3f} ms "
+            f"({sum(r['iterations'] for r in res)} iterations); copy probe {copy_ms.value:.3f} ms")
         return
 
     t_l, res_l = timed(lr.align, args.steps, args.warmup)

@@ -156,28 +156,60 @@ RTR_HD bool ring_done(const GridView& g, float qx, float qy, float qz, int cx, i
   return false;
 }
 
+// Distance from q to the slab [ox + a*h, ox + (b+1)*h] along one axis
+// (0 inside), shortened by g.margin and clamped at 0 -- the per-axis bound of
+// ring_done.
+RTR_HD float slab_gap(float q, float o, float h, float margin, int a, int b) {
+  const float lo = o + (float)a * h, hi = o + (float)(b + 1) * h;
+  float d = q < lo ? lo - q : (q > hi ? q - hi : 0.f);
+  d = d - margin;
+  return d > 0.f ? d : 0.f;
+}
+
+// Lower bound on the float d2 from q to every point stored in the cells
+// x0..x1 of row (y, z): the Euclidean distance to the box of those cells with
+// each axis gap shortened by g.margin (float rounding of the cell assignment,
+// as in ring_done), squared and scaled by (1 - 2e-6), which undercuts the
+// float d2 of any point in the box.
+RTR_HD float row_lb2(const GridView& g, float qx, float qy, float qz, int x0, int x1, int y, int z) {
+  const float ax = slab_gap(qx, g.ox, g.h, g.margin, x0, x1);
+  const float ay = slab_gap(qy, g.oy, g.h, g.margin, y, y);
+  const float az = slab_gap(qz, g.oz, g.h, g.margin, z, z);
+  return ((ax * ax + ay * ay) + az * az) * 0.999998f;
+}
+
+// A row segment can be skipped when its lower bound exceeds both the best
+// found so far (no point there can win, ties included: the comparison is
+// strict) and the acceptance bound r2f (no point there can be accepted; the
+// accepted nearest neighbour, and whether there is one, stay exact).
+RTR_HD bool row_skip(float lb2, float best, float r2f) { return lb2 > (best < r2f ? best : r2f); }
+
 // The 3x3x3 block around (cx, cy, cz) (rings 0 and 1) as nine x-rows; each
-// x-row of cells is one contiguous range of the cell-sorted array.
+// x-row of cells is one contiguous range of the cell-sorted array.  The
+// query's own row goes first, then the four side rows, then the corners, each
+// skipped when its box is beyond the best found so far (or beyond r2f).
+constexpr int kBlockRowOrder[9][2] = {{0, 0}, {-1, 0}, {1, 0}, {0, -1}, {0, 1},
+                                      {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};  // (dy, dz)
+
 template <class V>
-RTR_HD void walk_block(const GridView& g, int cx, int cy, int cz, V& vis) {
+RTR_HD void walk_block(const GridView& g, float qx, float qy, float qz, int cx, int cy, int cz, float r2f,
+                       V& vis) {
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = cz + dz;
-    if (z < 0 || z >= g.dz) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = cy + dy;
-      if (y < 0 || y >= g.dy) continue;
-      const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
-      vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
-    }
+  for (int k = 0; k < 9; ++k) {
+    const int y = cy + kBlockRowOrder[k][0], z = cz + kBlockRowOrder[k][1];
+    if (z < 0 || z >= g.dz || y < 0 || y >= g.dy) continue;
+    if (k > 0 && row_skip(row_lb2(g, qx, qy, qz, x0, x1, y, z), vis.bound(), r2f)) continue;
+    const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+    vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
   }
 }
 
 // Cells at Chebyshev distance exactly k (k >= 2): full x-rows on the y/z
 // faces of the shell, the two end cells elsewhere.
 template <class V>
-RTR_HD void walk_shell(const GridView& g, int cx, int cy, int cz, int k, V& vis) {
+RTR_HD void walk_shell(const GridView& g, float qx, float qy, float qz, int cx, int cy, int cz, int k,
+                       float r2f, V& vis) {
   for (int dz = -k; dz <= k; ++dz) {
     const int z = cz + dz;
     if (z < 0 || z >= g.dz) continue;
@@ -189,10 +221,13 @@ RTR_HD void walk_shell(const GridView& g, int cx, int cy, int cz, int k, V& vis)
       if (zface || dy == -k || dy == k) {
         const int x0 = clampi(cx - k, 0, g.dx - 1);
         const int x1 = clampi(cx + k, 0, g.dx - 1);
+        if (row_skip(row_lb2(g, qx, qy, qz, x0, x1, y, z), vis.bound(), r2f)) continue;
         vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
       } else {
-        if (cx - k >= 0) vis.scan(g.cell_start[row + cx - k], g.cell_start[row + cx - k + 1]);
-        if (cx + k < g.dx) vis.scan(g.cell_start[row + cx + k], g.cell_start[row + cx + k + 1]);
+        if (cx - k >= 0 && !row_skip(row_lb2(g, qx, qy, qz, cx - k, cx - k, y, z), vis.bound(), r2f))
+          vis.scan(g.cell_start[row + cx - k], g.cell_start[row + cx - k + 1]);
+        if (cx + k < g.dx && !row_skip(row_lb2(g, qx, qy, qz, cx + k, cx + k, y, z), vis.bound(), r2f))
+          vis.scan(g.cell_start[row + cx + k], g.cell_start[row + cx + k + 1]);
       }
     }
   }
@@ -262,7 +297,7 @@ RTR_HD void walk_far(const GridView& g, float qx, float qy, float qz, int cx, in
       walk_thick_shell(g, cx, cy, cz, k, kn, vis);
       k = kn;
     } else {
-      walk_shell(g, cx, cy, cz, k + 1, vis);
+      walk_shell(g, qx, qy, qz, cx, cy, cz, k + 1, r2f, vis);
       k = k + 1;
     }
   }
@@ -276,7 +311,7 @@ RTR_HD void walk_outer(const GridView& g, float qx, float qy, float qz, int cx, 
   const int kmax = (g.dx > g.dy ? (g.dx > g.dz ? g.dx : g.dz) : (g.dy > g.dz ? g.dy : g.dz));
   for (int k = k0; k <= kmax; ++k) {
     if (ring_done(g, qx, qy, qz, cx, cy, cz, k, r2f, vis)) return;
-    walk_shell(g, cx, cy, cz, k + 1, vis);
+    walk_shell(g, qx, qy, qz, cx, cy, cz, k + 1, r2f, vis);
   }
 }
 
@@ -311,7 +346,7 @@ RTR_HD void ring_walk_dt(const GridView& g, const uint8_t* dist, int skip_ring, 
   if (D >= 2)
     k0 = D - 1;
   else
-    walk_block(g, cx, cy, cz, vis);
+    walk_block(g, qx, qy, qz, cx, cy, cz, r2f, vis);
   walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
 }
 
@@ -321,7 +356,7 @@ template <class V>
 RTR_HD void ring_walk(const GridView& g, float qx, float qy, float qz, float r2f, V& vis) {
   int cx, cy, cz;
   if (!query_cell(g, qx, qy, qz, cx, cy, cz)) return;
-  walk_block(g, cx, cy, cz, vis);
+  walk_block(g, qx, qy, qz, cx, cy, cz, r2f, vis);
   walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, 1);
 }
 

@@ -191,14 +191,16 @@ struct GlobalVis {
 
 // The 3x3x3 block straight from global memory: the 18 row bounds are loaded
 // together before any row is scanned.
-__device__ __forceinline__ void walk_block_global(const GridView& g, int cx, int cy, int cz,
-                                                  GlobalVis& vis) {
+__device__ __forceinline__ void walk_block_global(const GridView& g, float qx, float qy, float qz, int cx, int cy,
+                                                  int cz, float r2f, GlobalVis& vis) {
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
+  // all 18 row bounds in flight together; rows are then scanned own row
+  // first, skipping any whose box lies beyond the best so far (or beyond r2f)
   uint32_t rb[9], re[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const int z = cz + k / 3 - 1, y = cy + k % 3 - 1;
+    const int y = cy + kBlockRowOrder[k][0], z = cz + kBlockRowOrder[k][1];
     if (z >= 0 && z < g.dz && y >= 0 && y < g.dy) {
       const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
       rb[k] = g.cell_start[row + x0];
@@ -207,8 +209,14 @@ __device__ __forceinline__ void walk_block_global(const GridView& g, int cx, int
       rb[k] = re[k] = 0u;
     }
   }
+  vis.scan(rb[0], re[0]);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) vis.scan(rb[k], re[k]);
+  for (int k = 1; k < 9; ++k) {
+    if (rb[k] == re[k]) continue;
+    const int y = cy + kBlockRowOrder[k][0], z = cz + kBlockRowOrder[k][1];
+    if (row_skip(row_lb2(g, qx, qy, qz, x0, x1, y, z), vis.bound(), r2f)) continue;
+    vis.scan(rb[k], re[k]);
+  }
 }
 
 template <class TileT>
@@ -409,7 +417,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
           }
         }
       } else {
-        walk_block_global(g, cx, cy, cz, vis);
+        walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
       }
       walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
       if (vis.key != ~0ull && key_d2(vis.key) <= r2f) {
