@@ -170,6 +170,15 @@ int rtr_batch_set_params(rtr_batch* b, const rtr_params* p);
  * (and <= 0 for the first three) keep the current setting. */
 int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point, int item_pts,
                          int blocks_per_cu, int use_tile);
+/* Grid policy.  always = 1 (default): every align rebuilds the target grids
+ * and source orders inside its launch sequence (a registration pays for its
+ * own search structure, as PCL's setInputTarget builds its kd-tree).
+ * always = 0: grids are built by the first align after the clouds,
+ * parameters or tuning change, or by rtr_batch_build, and reused by later
+ * aligns, also across rtr_batch_set_problems. */
+int rtr_batch_set_rebuild(rtr_batch* b, int always);
+/* Build the target grids and source orders now. */
+int rtr_batch_build(rtr_batch* b);
 /* Enable/disable hipGraph replay of align (default on). */
 int rtr_batch_set_graph(rtr_batch* b, int on);
 int rtr_batch_set_profiling(rtr_batch* b, int on);

@@ -410,6 +410,19 @@ int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point,
   RTR_GUARD_END
 }
 
+int rtr_batch_set_rebuild(rtr_batch* b, int always) {
+  if (!b) return RTR_ERR_INVALID_ARG;
+  b->eng.set_rebuild_always(always != 0);
+  return RTR_OK;
+}
+
+int rtr_batch_build(rtr_batch* b) {
+  RTR_GUARD_BEGIN
+  if (!b) return RTR_ERR_INVALID_ARG;
+  return b->eng.build();
+  RTR_GUARD_END
+}
+
 int rtr_batch_set_graph(rtr_batch* b, int on) {
   if (!b) return RTR_ERR_INVALID_ARG;
   b->eng.set_graph(on != 0);

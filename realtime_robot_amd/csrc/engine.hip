@@ -185,7 +185,7 @@ int Engine::set_clouds(int n_src, const float* const* src, const size_t* ns, con
       break;
     }
   if (!probs_.empty()) return upload_problems();
-  return RTR_OK;
+  return layout_targets();
 }
 
 int Engine::set_target_normals(const float* const* nrm, const size_t* stride) {
@@ -217,7 +217,7 @@ int Engine::set_problems(int n, const int* src_id, const int* tgt_id) {
     probs_[p].tgt = (uint32_t)tgt_id[p];
   }
   graph_dirty_ = true;
-  return upload_problems();
+  return layout_problems();
 }
 
 int Engine::set_params(const rtr_params& p) {
@@ -241,15 +241,16 @@ size_t Engine::source_size(int prob) const {
 // Descriptors, budgets and the item list: everything that depends on sizes
 // and parameters.  Allocates; never called inside an align.
 int Engine::upload_problems() {
+  RTR_TRY(layout_targets());
+  return layout_problems();
+}
+
+// Target and source descriptors: cell budgets and the requested cell edge
+// (from the parameters and the tuning).  Uploading them resets the grid
+// fields the build writes on the device, so the grids are dirty afterwards.
+int Engine::layout_targets() {
   RTR_HIP(hipSetDevice(device_));
   RTR_HIP(hipStreamSynchronize(stream_));
-  for (const Problem& pr : probs_)
-    if (pr.src >= srcs_.size() || pr.tgt >= tgts_.size()) {
-      set_error("problem references a missing cloud (src %u of %zu, tgt %u of %zu)", pr.src, srcs_.size(), pr.tgt,
-                tgts_.size());
-      probs_.clear();
-      return RTR_ERR_INVALID_ARG;
-    }
   // target cell budgets and the requested cell edge
   const double rmax = prm_.max_correspondence_distance;
   float h_req = prm_.cell_size > 0 ? prm_.cell_size : 0.f;
@@ -279,6 +280,23 @@ int Engine::upload_problems() {
     RTR_HIP(hipMemcpy(tgt_desc_.p, tgts_.data(), tgts_.size() * sizeof(TargetDesc), hipMemcpyHostToDevice));
   if (!srcs_.empty())
     RTR_HIP(hipMemcpy(src_desc_.p, srcs_.data(), srcs_.size() * sizeof(SourceDesc), hipMemcpyHostToDevice));
+  grid_dirty_ = true;
+  graph_dirty_ = true;
+  results_valid_ = false;
+  return RTR_OK;
+}
+
+// Problems, items and per-problem state.  Leaves the grids as they are.
+int Engine::layout_problems() {
+  RTR_HIP(hipSetDevice(device_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  for (const Problem& pr : probs_)
+    if (pr.src >= srcs_.size() || pr.tgt >= tgts_.size()) {
+      set_error("problem references a missing cloud (src %u of %zu, tgt %u of %zu)", pr.src, srcs_.size(), pr.tgt,
+                tgts_.size());
+      probs_.clear();
+      return RTR_ERR_INVALID_ARG;
+    }
   // Items, grouped into one queue per XCD.  All problems of one target go to
   // the same XCD (its L2 then holds that target's grid); targets are placed
   // longest-first on the least-loaded XCD.
@@ -330,8 +348,20 @@ int Engine::upload_problems() {
   if (np) RTR_HIP(hipMemcpy(prob_desc_.p, probs_.data(), np * sizeof(Problem), hipMemcpyHostToDevice));
   if (!items_.empty())
     RTR_HIP(hipMemcpy(items_buf_.p, items_.data(), items_.size() * sizeof(uint2), hipMemcpyHostToDevice));
-  grid_dirty_ = true;
   graph_dirty_ = true;
+  results_valid_ = false;
+  return RTR_OK;
+}
+
+int Engine::build() {
+  RTR_HIP(hipSetDevice(device_));
+  if (srcs_.empty() || tgts_.empty()) {
+    set_error("no clouds set");
+    return RTR_ERR_NOT_READY;
+  }
+  RTR_TRY(enqueue_build());
+  RTR_HIP(hipStreamSynchronize(stream_));
+  grid_dirty_ = false;
   results_valid_ = false;
   return RTR_OK;
 }

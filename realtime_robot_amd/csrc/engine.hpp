@@ -59,7 +59,13 @@ class Engine {
   const rtr_params& params() const { return prm_; }
   void set_graph(bool on) { use_graph_ = on; }
   void set_profiling(bool on) { profiling_ = on; }
-  void set_rebuild_always(bool on) { rebuild_always_ = on; }
+  void set_rebuild_always(bool on) {
+    if (on != rebuild_always_) graph_dirty_ = true;
+    rebuild_always_ = on;
+  }
+  // Builds the grids and source orders now (aligns then reuse them until the
+  // clouds, parameters or tuning change, unless rebuild-always is on).
+  int build();
   TuneParams& tune() { return tune_; }
   void invalidate_graph() { graph_dirty_ = true; }
 
@@ -83,6 +89,8 @@ class Engine {
   int enqueue_align_body();
   DevEngine dev_view() const;
   int upload_problems();
+  int layout_targets();
+  int layout_problems();
   int grid_blocks() const;
 
   int device_;

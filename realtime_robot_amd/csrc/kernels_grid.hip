@@ -443,12 +443,10 @@ __device__ __forceinline__ uint32_t spread3_4(uint32_t v) {
   return r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_src_order_bins(DevEngine e, uint32_t total) {
-  __shared__ uint64_t s_key[kWaves][kBinCap];
-  __shared__ float4 s_pt[kWaves][kBinCap];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t b = blockIdx.x * kWaves + w;
-  if (b >= e.n_bins_total) return;
+// Waves stride over the bins (most bins of a sparse surface are empty, so a
+// wave per bin would launch mostly idle workgroups).
+__device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32_t total, int lane, int w,
+                                          uint64_t (*s_key)[kBinCap], float4 (*s_pt)[kBinCap]) {
   const uint32_t beg = e.bin_start[b];
   const uint32_t end = (b + 1 < e.n_bins_total) ? e.bin_start[b + 1] : total;
   const uint32_t n = end - beg;
@@ -512,6 +510,17 @@ __global__ __launch_bounds__(kBlock) void k_src_order_bins(DevEngine e, uint32_t
     e.sz[beg + r] = pt.z;
     e.perm[beg + r] = (uint32_t)__float_as_int(pt.w);
   }
+  __builtin_amdgcn_wave_barrier();  // the LDS slots are reused by the wave's next bin
+}
+
+constexpr uint32_t kOrderBlocks = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_src_order_bins(DevEngine e, uint32_t total) {
+  __shared__ uint64_t s_key[kWaves][kBinCap];
+  __shared__ float4 s_pt[kWaves][kBinCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t waves = gridDim.x * kWaves;
+  for (uint32_t b = blockIdx.x * kWaves + w; b < e.n_bins_total; b += waves) order_bin(e, b, total, lane, w, s_key, s_pt);
 }
 
 void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s) {
@@ -522,8 +531,9 @@ void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total,
   hipLaunchKernelGGL(k_src_count, g, dim3(kBlock), 0, s, e);
   launch_exclusive_scan(e.bin_count, e.bin_start, e.n_bins_total, e.scan_tmp, s);
   hipLaunchKernelGGL(k_src_scatter, g, dim3(kBlock), 0, s, e);
-  hipLaunchKernelGGL(k_src_order_bins, dim3((e.n_bins_total + kWaves - 1) / kWaves), dim3(kBlock), 0, s,
-                     e, src_total);
+  const uint32_t ob = (e.n_bins_total + kWaves - 1) / kWaves;
+  hipLaunchKernelGGL(k_src_order_bins, dim3(ob < kOrderBlocks ? ob : kOrderBlocks), dim3(kBlock), 0, s, e,
+                     src_total);
 }
 
 }  // namespace rtr

@@ -169,55 +169,77 @@ static void gemm(hipStream_t s, const uint16_t* A, int lda, int ta, const uint16
 // One workgroup per cloud.  The cloud is first subsampled with a fixed stride
 // to at most kFpsMax points; its centroid is removed; then kP points are
 // chosen greedily, starting at subsample index 0, each time the point with
-// the largest distance to the chosen set (lowest index on ties).
-__global__ __launch_bounds__(256) void k_fps(const float* raw, const uint32_t* off, const uint32_t* cnt,
-                                             float* pts_out /* cloud * kP * 3 */, uint16_t* x0 /* rows of kC0 */,
-                                             float* centroid /* cloud * 3 */, int32_t* sel_out /* cloud*kP */) {
+// the largest distance to the chosen set (lowest index on ties).  Each
+// thread keeps its kFpsPer subsample points (i = tid + k * kFpsThreads) and
+// their distances in registers; the chosen point is broadcast through LDS.
+constexpr int kFpsThreads = 512;
+constexpr int kFpsPer = kFpsMax / kFpsThreads;
+constexpr int kFpsWaves = kFpsThreads / 64;
+
+__global__ __launch_bounds__(kFpsThreads) void k_fps(const float* raw, const uint32_t* off, const uint32_t* cnt,
+                                                     float* pts_out /* cloud * kP * 3 */,
+                                                     uint16_t* x0 /* rows of kC0 */, float* centroid /* cloud * 3 */,
+                                                     int32_t* sel_out /* cloud*kP */) {
   const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t n = cnt[c];
   const float* p = raw + 3 * (size_t)off[c];
   const int m = (int)min((uint32_t)kFpsMax, n);
-  __shared__ float sx[kFpsMax], sy[kFpsMax], sz[kFpsMax];
-  __shared__ float red_v[4];
-  __shared__ int red_i[4];
-  __shared__ double cen[4][3];
+  __shared__ float s_q[3];
+  __shared__ float red_v[kFpsWaves];
+  __shared__ int red_i[kFpsWaves];
+  __shared__ double cen[kFpsWaves][3];
+  float px[kFpsPer], py[kFpsPer], pz[kFpsPer], dist[kFpsPer];
   double ax = 0, ay = 0, az = 0;
-  for (int i = tid; i < m; i += 256) {
-    const size_t src = (size_t)((uint64_t)i * n / (uint64_t)m);
-    sx[i] = p[3 * src];
-    sy[i] = p[3 * src + 1];
-    sz[i] = p[3 * src + 2];
-    ax += sx[i];
-    ay += sy[i];
-    az += sz[i];
+#pragma unroll
+  for (int k = 0; k < kFpsPer; ++k) {
+    const int i = tid + k * kFpsThreads;
+    px[k] = py[k] = pz[k] = 0.f;
+    dist[k] = -1.f;
+    if (i < m) {
+      const size_t src = (size_t)((uint64_t)i * n / (uint64_t)m);
+      px[k] = p[3 * src];
+      py[k] = p[3 * src + 1];
+      pz[k] = p[3 * src + 2];
+      dist[k] = 3.0e38f;
+      ax += px[k];
+      ay += py[k];
+      az += pz[k];
+    }
   }
 #pragma unroll
-  for (int off2 = 32; off2 > 0; off2 >>= 1) {
-    ax += __shfl_xor(ax, off2, 64);
-    ay += __shfl_xor(ay, off2, 64);
-    az += __shfl_xor(az, off2, 64);
+  for (int o = 32; o > 0; o >>= 1) {
+    ax += __shfl_xor(ax, o, 64);
+    ay += __shfl_xor(ay, o, 64);
+    az += __shfl_xor(az, o, 64);
   }
   if (lane == 0) {
     cen[w][0] = ax;
     cen[w][1] = ay;
     cen[w][2] = az;
   }
+  if (tid == 0) {  // subsample index 0 starts the selection
+    s_q[0] = px[0];
+    s_q[1] = py[0];
+    s_q[2] = pz[0];
+  }
   __syncthreads();
-  const float cx = m > 0 ? (float)((cen[0][0] + cen[1][0] + cen[2][0] + cen[3][0]) / m) : 0.f;
-  const float cy = m > 0 ? (float)((cen[0][1] + cen[1][1] + cen[2][1] + cen[3][1]) / m) : 0.f;
-  const float cz = m > 0 ? (float)((cen[0][2] + cen[1][2] + cen[2][2] + cen[3][2]) / m) : 0.f;
+  double sx = 0, sy = 0, sz = 0;
+  for (int k = 0; k < kFpsWaves; ++k) {
+    sx += cen[k][0];
+    sy += cen[k][1];
+    sz += cen[k][2];
+  }
+  const float cx = m > 0 ? (float)(sx / m) : 0.f;
+  const float cy = m > 0 ? (float)(sy / m) : 0.f;
+  const float cz = m > 0 ? (float)(sz / m) : 0.f;
   if (tid == 0) {
     centroid[3 * c] = cx;
     centroid[3 * c + 1] = cy;
     centroid[3 * c + 2] = cz;
   }
-  constexpr int PER = kFpsMax / 256;
-  float dist[PER];
-#pragma unroll
-  for (int k = 0; k < PER; ++k) dist[k] = 3.0e38f;
   int cur = 0;
   for (int s = 0; s < kP; ++s) {
-    const float qx = m > 0 ? sx[cur] : 0.f, qy = m > 0 ? sy[cur] : 0.f, qz = m > 0 ? sz[cur] : 0.f;
+    const float qx = s_q[0], qy = s_q[1], qz = s_q[2];
     if (tid == 0) {
       const size_t o = ((size_t)c * kP + s);
       pts_out[3 * o] = qx - cx;
@@ -233,24 +255,23 @@ __global__ __launch_bounds__(256) void k_fps(const float* raw, const uint32_t* o
     float bv = -1.f;
     int bi = 0x7fffffff;
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int i = tid + k * 256;
-      if (i < m) {
-        const float dx = sx[i] - qx, dy = sy[i] - qy, dz = sz[i] - qz;
+    for (int k = 0; k < kFpsPer; ++k) {
+      if (dist[k] >= 0.f) {  // a subsample point (i < m)
+        const float dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
         float d = dx * dx;
         d = d + dy * dy;
         d = d + dz * dz;
         dist[k] = fminf(dist[k], d);
-        if (dist[k] > bv || (dist[k] == bv && i < bi)) {
+        if (dist[k] > bv) {  // k ascending = index ascending: strict keeps the lowest
           bv = dist[k];
-          bi = i;
+          bi = tid + k * kFpsThreads;
         }
       }
     }
 #pragma unroll
-    for (int off2 = 32; off2 > 0; off2 >>= 1) {
-      const float ov = __shfl_xor(bv, off2, 64);
-      const int oi = __shfl_xor(bi, off2, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+      const float ov = __shfl_xor(bv, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
       if (ov > bv || (ov == bv && oi < bi)) {
         bv = ov;
         bi = oi;
@@ -260,16 +281,26 @@ __global__ __launch_bounds__(256) void k_fps(const float* raw, const uint32_t* o
       red_v[w] = bv;
       red_i[w] = bi;
     }
-    __syncthreads();
+    __syncthreads();  // s_q has been read by every thread; red_* written
     float gv = red_v[0];
     int gi = red_i[0];
-    for (int k = 1; k < 4; ++k)
+    for (int k = 1; k < kFpsWaves; ++k)
       if (red_v[k] > gv || (red_v[k] == gv && red_i[k] < gi)) {
         gv = red_v[k];
         gi = red_i[k];
       }
-    __syncthreads();
     cur = (gi >= 0 && gi < m) ? gi : 0;
+    if (tid == (cur % kFpsThreads)) {
+      const int kk = cur / kFpsThreads;
+#pragma unroll
+      for (int k = 0; k < kFpsPer; ++k)
+        if (k == kk) {
+          s_q[0] = px[k];
+          s_q[1] = py[k];
+          s_q[2] = pz[k];
+        }
+    }
+    __syncthreads();  // red_* read by every thread; s_q holds the next point
   }
 }
 
@@ -736,7 +767,7 @@ struct Proposer {
   // resident count).
   void enqueue_fps() {
     n_pairs = n_resident;
-    hipLaunchKernelGGL(k_fps, dim3(2 * n_pairs), dim3(256), 0, stream, raw.as<float>(), roff.as<uint32_t>(),
+    hipLaunchKernelGGL(k_fps, dim3(2 * n_pairs), dim3(kFpsThreads), 0, stream, raw.as<float>(), roff.as<uint32_t>(),
                        rcnt.as<uint32_t>(), pts.as<float>(), x0.as<uint16_t>(), cen.as<float>(),
                        sel.as<int32_t>());
   }

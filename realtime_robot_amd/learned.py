@@ -386,6 +386,9 @@ class LearnedRegistration:
         self.prop = Proposer(max_pairs=max_pairs * max(1, hypotheses), device=device, weights=weights)
         self.reg = BatchRegistration(device)
         self.reg.set_params(self.params)
+        # one grid build per align() call, shared by the refine and fallback launches
+        self.reg.set_rebuild(False)
+        self._problems = None
         self.n_starts = n_starts
         self.rmse_threshold = rmse_threshold
         self.min_inlier_frac = min_inlier_frac
@@ -398,25 +401,32 @@ class LearnedRegistration:
         self.n = len(sources)
         self.prop.set_clouds(sources, targets)
         self.reg.set_clouds(sources, targets)
+        self._problems = None
         self.cs = _centroids(sources) if src_centroids is None else np.asarray(src_centroids, np.float64)
         self.ct = _centroids(targets) if tgt_centroids is None else np.asarray(tgt_centroids, np.float64)
         self._ids = np.arange(self.n)
 
+    def _set_problems(self, key, ids):
+        if self._problems != key:
+            self.reg.set_problems(ids, ids)
+            self._problems = key
+
     def align(self):
         t0 = time.perf_counter()
+        self.reg.build()
         if self.hypotheses > 1:
             _, _, poses = self.prop.propose_multi(self.rots, self.tau)
         else:
             poses = self.prop.propose()
         t1 = time.perf_counter()
-        self.reg.set_problems(self._ids, self._ids)
+        self._set_problems("refine", self._ids)
         res = self.reg.align(poses)
         rmse, frac, pfrac = self.reg.judge(self.judge_k, self.judge_points, self.max_curvature)
         t2 = time.perf_counter()
         fail = np.where((rmse > self.rmse_threshold) | (frac < self.min_inlier_frac))[0]
         if len(fail):
             ids = np.repeat(fail, self.n_starts)
-            self.reg.set_problems(ids, ids)
+            self._set_problems(("fallback", tuple(fail.tolist())), ids)
             g = np.stack([T for i in fail for T in multistart.so3_start_poses(self.cs[i], self.ct[i], self.n_starts)])
             res8 = self.reg.align(g)
             for k, i in enumerate(fail):

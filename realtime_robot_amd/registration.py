@@ -230,6 +230,15 @@ class BatchRegistration:
     def set_graph(self, on: bool):
         L.check(L.lib.rtr_batch_set_graph(self._h, int(on)))
 
+    def set_rebuild(self, always: bool):
+        """True (default): every align rebuilds the grids; False: they are
+        built once (first align or build()) and reused, also across
+        set_problems."""
+        L.check(L.lib.rtr_batch_set_rebuild(self._h, int(always)))
+
+    def build(self):
+        L.check(L.lib.rtr_batch_build(self._h))
+
     def set_tuning(self, cell_factor=0.0, cells_per_point=0.0, item_pts=0, blocks_per_cu=-1, use_tile=-1):
         L.check(L.lib.rtr_batch_set_tuning(self._h, float(cell_factor), float(cells_per_point), int(item_pts),
                                            int(blocks_per_cu), int(use_tile)))

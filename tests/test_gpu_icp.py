@@ -195,3 +195,29 @@ def test_batch_resize_drops_stale_problems():
     for p in range(2):
         ref = oracle.icp(srcs[2 + p], tgts[2 + p], oracle.params(max_iter=10, r_max=0.05), guess=gts[2 + p])
         assert rot_err(r3[p]["T"], ref["T"]) < TOL_R and trans_err(r3[p]["T"], ref["T"]) < TOL_T
+
+
+def test_batch_reused_grids_match_rebuild_always():
+    """With the rebuild-always policy off, grids built once (build(), or the
+    first align after new clouds) serve later aligns across set_problems; the
+    results are bit-identical to an engine that rebuilds on every align."""
+    srcs, tgts, gts = synth.config3_batch(4, seed_offset=960, ns_range=(2000, 4000), nt_range=(5000, 9000))
+    p = R.make_params(max_iterations=15, max_correspondence_distance=0.05, transformation_epsilon=1e-8)
+    a, b = R.BatchRegistration(), R.BatchRegistration()
+    for e in (a, b):
+        e.set_clouds(srcs, tgts)
+        e.set_params(p)
+    b.set_rebuild(False)
+    b.build()
+    for ids, guesses in [([0, 1, 2, 3], gts), ([2, 2, 3], [gts[2], np.eye(4), gts[3]]), ([1], [gts[1]])]:
+        for e in (a, b):
+            e.set_problems(ids, ids)
+        ra, rb = a.align(guesses), b.align(guesses)
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x["T"], y["T"]) and x["iterations"] == y["iterations"]
+    # new clouds: the next align rebuilds by itself
+    b.set_clouds(srcs[::-1], tgts[::-1])
+    b.set_problems([0], [0])
+    r = b.align([gts[3]])[0]
+    ref = oracle.icp(srcs[3], tgts[3], oracle.params(max_iter=15, r_max=0.05, eps_T=1e-8), guess=gts[3])
+    assert rot_err(r["T"], ref["T"]) < TOL_R and trans_err(r["T"], ref["T"]) < TOL_T
