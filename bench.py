@@ -275,12 +275,14 @@ def main():
                    "tuning": list(tuned) if tuned else None,
                    "tuning_refine": list(tuned_refine) if tuned_refine else None,
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],
-                   "learned_ms_per_step": t_l * 1e3, "learned_ms_propose": st["ms_propose"],
-                   "learned_ms_propose_device": st["ms_propose_device"], "learned_ms_refine_judge": st["ms_refine"],
+                   "learned_ms_per_step": t_l * 1e3, "learned_ms_build": st["ms_build"],
+                   "learned_ms_propose": st["ms_propose"], "learned_ms_propose_device": st["ms_propose_device"],
+                   "learned_ms_refine": st["ms_refine"], "learned_ms_judge": st["ms_judge"],
                    "learned_ms_fallback": st["ms_fallback"],
                    "proposal_rot_err_deg_median": float(np.median(prop_rot)),
                    "proposal_rot_err_deg_p90": float(np.percentile(prop_rot, 90)),
                    "plain_value": float(plain_value), "plain_ok": int(ok_p.sum()), "plain_ms_per_step": t_p * 1e3,
+                   "plain_iterations_total": ms.iterations_total,
                    "speedup_vs_plain": float(value / plain_value) if plain_value > 0 else None,
                    "weights": {k: meta[k] for k in ("base_pairs", "pairs_seen", "steps")}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -296,8 +298,7 @@ def main():
     if not args.no_cpu_baseline:
         starts = [multistart.so3_start_poses(cs[i], ct[i], args.starts) for i in range(n)]
         out["cpu_baseline"] = cpu_baseline(srcs, tgts, starts, args.starts, args.cpu_budget)
-        rejected = set(np.where((st["judge_rmse"] > lr.rmse_threshold) |
-                                (st["judge_frac"] < lr.min_inlier_frac))[0].tolist())
+        rejected = set(st["fallback_pairs"].tolist())
         # the init the learned path's final ICP ran from (fallback pairs are left out)
         inits = [None if i in rejected else st["proposals"][i] for i in range(n)]
         chk = [i for i in range(n) if inits[i] is not None][:args.fp64_sample]

@@ -433,8 +433,9 @@ __global__ void k_src_scatter(DevEngine e) {
 // summation order of the moments, is identical from run to run, and
 // consecutive points stay spatially compact below the bin size.  Ranks are
 // found by counting (O(n^2 / 64) per bin, n ~ tens); a bin larger than the
-// LDS tile is ordered by original index alone by one lane.
-constexpr int kBinCap = 256;
+// wave's LDS tile (1024 points) is ordered by original index alone by one
+// lane.
+constexpr int kBinCap = 1024;
 
 __device__ __forceinline__ uint32_t spread3_4(uint32_t v) {
   uint32_t r = 0;
@@ -513,14 +514,14 @@ __device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32
   __builtin_amdgcn_wave_barrier();  // the LDS slots are reused by the wave's next bin
 }
 
-constexpr uint32_t kOrderBlocks = 4096;
+constexpr uint32_t kOrderBlocks = 16384;
 
-__global__ __launch_bounds__(kBlock) void k_src_order_bins(DevEngine e, uint32_t total) {
-  __shared__ uint64_t s_key[kWaves][kBinCap];
-  __shared__ float4 s_pt[kWaves][kBinCap];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t waves = gridDim.x * kWaves;
-  for (uint32_t b = blockIdx.x * kWaves + w; b < e.n_bins_total; b += waves) order_bin(e, b, total, lane, w, s_key, s_pt);
+// one wave per workgroup (24 KB of LDS each)
+__global__ __launch_bounds__(64) void k_src_order_bins(DevEngine e, uint32_t total) {
+  __shared__ uint64_t s_key[1][kBinCap];
+  __shared__ float4 s_pt[1][kBinCap];
+  const int lane = threadIdx.x & 63;
+  for (uint32_t b = blockIdx.x; b < e.n_bins_total; b += gridDim.x) order_bin(e, b, total, lane, 0, s_key, s_pt);
 }
 
 void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s) {
@@ -531,9 +532,8 @@ void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total,
   hipLaunchKernelGGL(k_src_count, g, dim3(kBlock), 0, s, e);
   launch_exclusive_scan(e.bin_count, e.bin_start, e.n_bins_total, e.scan_tmp, s);
   hipLaunchKernelGGL(k_src_scatter, g, dim3(kBlock), 0, s, e);
-  const uint32_t ob = (e.n_bins_total + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL(k_src_order_bins, dim3(ob < kOrderBlocks ? ob : kOrderBlocks), dim3(kBlock), 0, s, e,
-                     src_total);
+  const uint32_t ob = e.n_bins_total;
+  hipLaunchKernelGGL(k_src_order_bins, dim3(ob < kOrderBlocks ? ob : kOrderBlocks), dim3(64), 0, s, e, src_total);
 }
 
 }  // namespace rtr

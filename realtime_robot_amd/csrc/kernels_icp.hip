@@ -336,7 +336,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
     vis.where = 0u;
     bool staged = false;
     int X0 = 0, Y0 = 0, Z0 = 0, NY = 1, W1 = 1, nrows = 0;
-    if (__ballot(blk) != 0ull) {
+    if (TILE && e.use_tile && __ballot(blk) != 0ull) {
       X0 = max(wave_min_i(blk ? cx : 0x7fffffff) - 1, 0);
       Y0 = max(wave_min_i(blk ? cy : 0x7fffffff) - 1, 0);
       Z0 = max(wave_min_i(blk ? cz : 0x7fffffff) - 1, 0);

@@ -352,6 +352,7 @@ class MultiStartRegistration:
 
     def align(self):
         res = self.reg.align(self.guesses)
+        self.iterations_total = int(sum(r["iterations"] for r in res))
         out = []
         for i in range(self.n):
             r = res[i * self.n_starts:(i + 1) * self.n_starts]
@@ -414,6 +415,7 @@ class LearnedRegistration:
     def align(self):
         t0 = time.perf_counter()
         self.reg.build()
+        tb = time.perf_counter()
         if self.hypotheses > 1:
             _, _, poses = self.prop.propose_multi(self.rots, self.tau)
         else:
@@ -421,8 +423,9 @@ class LearnedRegistration:
         t1 = time.perf_counter()
         self._set_problems("refine", self._ids)
         res = self.reg.align(poses)
-        rmse, frac, pfrac = self.reg.judge(self.judge_k, self.judge_points, self.max_curvature)
         t2 = time.perf_counter()
+        rmse, frac, pfrac = self.reg.judge(self.judge_k, self.judge_points, self.max_curvature)
+        t3 = time.perf_counter()
         fail = np.where((rmse > self.rmse_threshold) | (frac < self.min_inlier_frac))[0]
         if len(fail):
             ids = np.repeat(fail, self.n_starts)
@@ -432,9 +435,9 @@ class LearnedRegistration:
             for k, i in enumerate(fail):
                 r = res8[k * self.n_starts:(k + 1) * self.n_starts]
                 res[i] = r[multistart.select_best(r)]
-        t3 = time.perf_counter()
-        self.stats = {"n_fallback": int(len(fail)), "proposals": poses, "judge_rmse": rmse, "judge_frac": frac,
-                      "judge_planar_frac": pfrac,
-                      "ms_propose": (t1 - t0) * 1e3, "ms_refine": (t2 - t1) * 1e3, "ms_fallback": (t3 - t2) * 1e3,
-                      "ms_propose_device": self.prop.last_ms()}
+        t4 = time.perf_counter()
+        self.stats = {"n_fallback": int(len(fail)), "fallback_pairs": fail, "proposals": poses, "judge_rmse": rmse,
+                      "judge_frac": frac, "judge_planar_frac": pfrac, "ms_build": (tb - t0) * 1e3,
+                      "ms_propose": (t1 - tb) * 1e3, "ms_refine": (t2 - t1) * 1e3, "ms_judge": (t3 - t2) * 1e3,
+                      "ms_fallback": (t4 - t3) * 1e3, "ms_propose_device": self.prop.last_ms()}
         return res
