@@ -647,15 +647,23 @@ int Engine::judge(int k, int max_pts, double max_curv, double* plane_rmse, doubl
   }
   const size_t np = probs_.size();
   if (np == 0) return RTR_OK;
-  RTR_TRY(out_f32_.ensure(np * 4 * sizeof(double)));
+  const size_t words = np * kJudgeSplit * 4;
+  RTR_TRY(out_f32_.ensure(words * sizeof(double)));
   DevEngine e = dev_view();
   launch_judge(e, k, max_pts, e.prm.r2f, max_curv, reinterpret_cast<double*>(out_f32_.p), stream_);
   RTR_HIP(hipGetLastError());
-  std::vector<double> h(np * 4);
-  RTR_HIP(hipMemcpyAsync(h.data(), out_f32_.p, np * 4 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  std::vector<double> h(words);
+  RTR_HIP(hipMemcpyAsync(h.data(), out_f32_.p, words * sizeof(double), hipMemcpyDeviceToHost, stream_));
   RTR_HIP(hipStreamSynchronize(stream_));
   for (size_t p = 0; p < np; ++p) {
-    const double s = h[4 * p], pc = h[4 * p + 1], c = h[4 * p + 2], m = h[4 * p + 3];
+    double s = 0.0, pc = 0.0, c = 0.0, m = 0.0;
+    for (int b = 0; b < kJudgeSplit; ++b) {
+      const double* o = &h[4 * (p * kJudgeSplit + b)];
+      s += o[0];
+      pc += o[1];
+      c += o[2];
+      m += o[3];
+    }
     if (plane_rmse) plane_rmse[p] = pc > 0 ? std::sqrt(s / pc) : DBL_MAX;
     if (inlier_frac) inlier_frac[p] = m > 0 ? c / m : 0.0;
     if (planar_frac) planar_frac[p] = c > 0 ? pc / c : 0.0;

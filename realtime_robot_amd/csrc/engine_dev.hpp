@@ -74,6 +74,7 @@ void launch_correspondences(const DevEngine& e, int prob, const double* T, float
                             float* d2, hipStream_t s);
 void launch_transform_out(const DevEngine& e, int prob, float* out, hipStream_t s);
 void launch_fitness_d2(const DevEngine& e, float* d2_all, hipStream_t s);
+constexpr int kJudgeSplit = 8;  // workgroups per problem in the registration judge
 void launch_judge(const DevEngine& e, int k, int max_pts, float r2f, double max_curv, double* out,
                   hipStream_t s);
 void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, float vpz,

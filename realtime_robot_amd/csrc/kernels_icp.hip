@@ -820,8 +820,11 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
 // surface.  Unlike the point-to-point MSE, this residual does not grow with
 // the target's sampling spacing, so one threshold serves sparse and dense
 // targets.
-// out[4p..4p+3] = {sum of squared plane distances, planar inliers, inliers,
-// sampled points}.
+// The sample of problem p is split over gridDim.y workgroups (sample j goes
+// to workgroup j % gridDim.y); workgroup s writes its partial sums to
+// out[4 (p * gridDim.y + s) + {0, 1, 2, 3}] = {sum of squared plane
+// distances, planar inliers, inliers, sampled points}, which the host adds
+// in fixed order.
 __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max_pts, float r2f, double max_curv, double* out) {
   __shared__ uint64_t s_key[kMaxK * kNrmBlock];
   __shared__ uint32_t s_slot[kMaxK * kNrmBlock];
@@ -836,7 +839,8 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
   const uint32_t n = sd.n;
   const uint32_t ms = n < (uint32_t)max_pts ? n : (uint32_t)max_pts;
   double ssum = 0.0, cnt = 0.0, pcnt = 0.0;
-  for (uint32_t j = threadIdx.x; j < ms; j += kNrmBlock) {
+  const uint32_t split = gridDim.y;
+  for (uint32_t j = blockIdx.y + split * threadIdx.x; j < ms; j += split * kNrmBlock) {
     const uint32_t sj = sd.off + (uint32_t)(((uint64_t)j * n) / ms);
     float qx, qy, qz;
     apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
@@ -900,10 +904,11 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
       pc += s_red[1][i];
       c += s_red[2][i];
     }
-    out[4 * prob] = a;
-    out[4 * prob + 1] = pc;
-    out[4 * prob + 2] = c;
-    out[4 * prob + 3] = (double)ms;
+    double* o = out + 4 * ((size_t)prob * split + blockIdx.y);
+    o[0] = a;
+    o[1] = pc;
+    o[2] = c;
+    o[3] = blockIdx.y == 0 ? (double)ms : 0.0;
   }
 }
 
@@ -911,7 +916,8 @@ void launch_judge(const DevEngine& e, int k, int max_pts, float r2f, double max_
                   hipStream_t s) {
   if (e.n_prob == 0) return;
   if (k > kMaxK) k = kMaxK;
-  hipLaunchKernelGGL(k_judge, dim3(e.n_prob), dim3(kNrmBlock), 0, s, e, k, max_pts, r2f, max_curv, out);
+  hipLaunchKernelGGL(k_judge, dim3(e.n_prob, kJudgeSplit), dim3(kNrmBlock), 0, s, e, k, max_pts, r2f, max_curv,
+                     out);
 }
 
 }  // namespace rtr
