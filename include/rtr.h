@@ -292,6 +292,9 @@ int rtr_prop_propose(rtr_prop* h, double* poses);
  * (lowest j on ties). */
 int rtr_prop_propose_multi(rtr_prop* h, int k, const double* rots, float tau, double* poses,
                            double* scores, double* best);
+/* Inference through the fused point-MLP + max-pool kernel (on = 1, default)
+ * or through the layer-by-layer products (0); both give identical features. */
+int rtr_prop_set_fused(rtr_prop* h, int on);
 /* Device time of the last rtr_prop_propose / rtr_prop_propose_multi, ms. */
 int rtr_prop_last_ms(rtr_prop* h, float* ms);
 /* Network on given centred FPS points (n*2*1024*3, clouds src0, tgt0, ...).

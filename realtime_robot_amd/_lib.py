@@ -123,6 +123,7 @@ _SIGS = {
     "rtr_prop_fps": (_i, [_vp, _vp, _vp, _vp]),
     "rtr_prop_propose": (_i, [_vp, _vp]),
     "rtr_prop_propose_multi": (_i, [_vp, _i, _vp, _f, _vp, _vp, _vp]),
+    "rtr_prop_set_fused": (_i, [_vp, _i]),
     "rtr_prop_last_ms": (_i, [_vp, C.POINTER(_f)]),
     "rtr_prop_forward_backward": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rtr_prop_train_step": (_i, [_vp, _i, _vp, _vp, _f, C.POINTER(_f)]),

@@ -487,7 +487,11 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
 }
 
 template <int MODE, bool TILE>
-__global__ __launch_bounds__(kBlock) void k_icp(DevEngine e) {
+// Point-to-point asks for at least 3 waves per SIMD (<= 168 VGPRs, no
+// spills): the gathers of the ring walk are latency-bound, so resident waves
+// are what hides them.  Point-to-plane carries 29 compensated sums and keeps
+// 2 waves per SIMD rather than spill.
+__global__ __launch_bounds__(kBlock, MODE == RTR_MODE_POINT_TO_POINT ? 3 : 2) void k_icp(DevEngine e) {
   constexpr int NS = MODE == RTR_MODE_POINT_TO_POINT ? SUM_P2P_COUNT : SUM_P2L_COUNT;
   using TileT = typename std::conditional<TILE, WaveTile, WaveTileNone>::type;
   __shared__ IcpShared<NS, TileT> sh;

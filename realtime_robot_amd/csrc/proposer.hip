@@ -46,6 +46,25 @@ constexpr int kC1 = 64, kC2 = 128, kC3 = 256, kH = 256, kOut = 9;
 constexpr int kFpsMax = 8192; // stride pre-subsample before FPS
 constexpr int kMaxHyp = 64;   // hypothesis rotations per proposal call
 
+// parameter layout (FP32 master and BF16 copy share offsets)
+struct ParamLayout {
+  size_t W1, b1, W2, b2, W3, b3, W4, b4, W5, b5, total;
+  ParamLayout() {
+    size_t o = 0;
+    W1 = o; o += (size_t)kC0 * kC1;
+    b1 = o; o += kC1;
+    W2 = o; o += (size_t)kC1 * kC2;
+    b2 = o; o += kC2;
+    W3 = o; o += (size_t)kC2 * kC3;
+    b3 = o; o += kC3;
+    W4 = o; o += (size_t)2 * kC3 * kH;
+    b4 = o; o += kH;
+    W5 = o; o += (size_t)kH * kOut;
+    b5 = o; o += kOut;
+    total = o;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N] (+ bias[N]) (ReLU)
 //   A(i,k) = ta ? A[k*lda + i] : A[i*lda + k]   (bf16)
@@ -160,6 +179,145 @@ static void gemm(hipStream_t s, const uint16_t* A, int lda, int ta, const uint16
   g.k_split = k_split;
   dim3 grid((M + kTM - 1) / kTM, (N + kTN - 1) / kTN, k_split);
   hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
+}
+
+// ---------------------------------------------------------------------------
+// Fused point MLP + max-pool (inference)
+//
+// Rows of a cloud go through 16 -> 64 -> 128 -> 256 with bias + ReLU +
+// BF16 rounding after every layer and only the per-channel maxima leave the
+// chip.  Each layer is the same sequence of v_mfma_f32_32x32x8bf16_1k steps
+// (k ascending) as k_gemm_bf16 and rounds where the layer-by-layer path
+// stores BF16, so the pooled features are bit-identical to it.  The
+// transposed weights sit in LDS for the workgroup's lifetime; each wave owns
+// a 32-row activation tile in LDS; workgroups stride over clouds.
+// ---------------------------------------------------------------------------
+constexpr int kFWaves = 8;
+constexpr int kFThreads = kFWaves * 64;
+constexpr int kFPad = 4;  // bf16 elements of row padding (8-byte aligned, spreads LDS banks)
+
+struct FusedShared {
+  uint16_t w1t[kC1][kC0 + kFPad];
+  uint16_t w2t[kC2][kC1 + kFPad];
+  uint16_t w3t[kC3][kC2 + kFPad];
+  uint16_t act[kFWaves][32][kC2 + kFPad];
+  float b1[kC1], b2[kC2], b3[kC3];
+  uint32_t cmax[kC3];
+};
+
+__device__ __forceinline__ bf16x4 lds4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+// acc row of register j for this lane (32x32 MFMA output layout)
+__device__ __forceinline__ int mfma_row(int lane, int j) { return (j >> 2) * 8 + (lane >> 5) * 4 + (j & 3); }
+
+__global__ __launch_bounds__(kFThreads) void k_mlp_pool(const uint16_t* x0, int n_clouds, const uint16_t* wb,
+                                                       const float* w, ParamLayout L, float* G) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  FusedShared& sh = *reinterpret_cast<FusedShared*>(smem_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // transposed weights (B operands read 4 consecutive k per lane) and biases
+  for (int i = tid; i < kC0 * kC1; i += kFThreads) sh.w1t[i % kC1][i / kC1] = wb[L.W1 + i];
+  for (int i = tid; i < kC1 * kC2; i += kFThreads) sh.w2t[i % kC2][i / kC2] = wb[L.W2 + i];
+  for (int i = tid; i < kC2 * kC3; i += kFThreads) sh.w3t[i % kC3][i / kC3] = wb[L.W3 + i];
+  for (int i = tid; i < kC1; i += kFThreads) sh.b1[i] = w[L.b1 + i];
+  for (int i = tid; i < kC2; i += kFThreads) sh.b2[i] = w[L.b2 + i];
+  for (int i = tid; i < kC3; i += kFThreads) sh.b3[i] = w[L.b3 + i];
+  uint16_t(*act)[kC2 + kFPad] = sh.act[wv];
+  const int lr = lane & 31, kh = (lane >> 5) * 4;
+  for (int c = blockIdx.x; c < n_clouds; c += gridDim.x) {
+    for (int i = tid; i < kC3; i += kFThreads) sh.cmax[i] = 0u;
+    __syncthreads();
+    float vmax[kC3 / 32];
+#pragma unroll
+    for (int t = 0; t < kC3 / 32; ++t) vmax[t] = 0.f;
+    for (int r0 = wv * 32; r0 < kP; r0 += kFWaves * 32) {
+      const uint16_t* xr = x0 + ((size_t)c * kP + r0 + lr) * kC0;
+      // layer 1: K = 16 (two k-steps), N = 64 (two tiles)
+      const bf16x4 a0 = *reinterpret_cast<const bf16x4*>(xr + kh);
+      const bf16x4 a1 = *reinterpret_cast<const bf16x4*>(xr + 8 + kh);
+#pragma unroll
+      for (int nt = 0; nt < kC1 / 32; ++nt) {
+        f32x16 acc;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, lds4(&sh.w1t[nt * 32 + lr][kh]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, lds4(&sh.w1t[nt * 32 + lr][8 + kh]), acc, 0, 0, 0);
+        const int col = nt * 32 + lr;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const float v = acc[j] + sh.b1[col];
+          act[mfma_row(lane, j)][col] = f2bf(v > 0.f ? v : 0.f);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      // layer 2: K = 64 (eight k-steps), N = 128 (four tiles)
+      {
+        f32x16 acc[kC2 / 32];
+#pragma unroll
+        for (int nt = 0; nt < kC2 / 32; ++nt)
+#pragma unroll
+          for (int j = 0; j < 16; ++j) acc[nt][j] = 0.f;
+#pragma unroll
+        for (int ks = 0; ks < kC1 / 8; ++ks) {
+          const bf16x4 a = lds4(&act[lr][ks * 8 + kh]);
+#pragma unroll
+          for (int nt = 0; nt < kC2 / 32; ++nt)
+            acc[nt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, lds4(&sh.w2t[nt * 32 + lr][ks * 8 + kh]), acc[nt], 0,
+                                                               0, 0);
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int nt = 0; nt < kC2 / 32; ++nt) {
+          const int col = nt * 32 + lr;
+#pragma unroll
+          for (int j = 0; j < 16; ++j) {
+            const float v = acc[nt][j] + sh.b2[col];
+            act[mfma_row(lane, j)][col] = f2bf(v > 0.f ? v : 0.f);
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      // layer 3: K = 128 (sixteen k-steps), N = 256 in two halves of four tiles
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        f32x16 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int j = 0; j < 16; ++j) acc[t][j] = 0.f;
+#pragma unroll 4
+        for (int ks = 0; ks < kC2 / 8; ++ks) {
+          const bf16x4 a = lds4(&act[lr][ks * 8 + kh]);
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, lds4(&sh.w3t[(half * 4 + t) * 32 + lr][ks * 8 + kh]),
+                                                              acc[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int col = (half * 4 + t) * 32 + lr;
+          float m = vmax[half * 4 + t];
+#pragma unroll
+          for (int j = 0; j < 16; ++j) {
+            const float v = acc[t][j] + sh.b3[col];
+            m = fmaxf(m, bf2f(f2bf(v > 0.f ? v : 0.f)));
+          }
+          vmax[half * 4 + t] = m;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();  // act is rewritten by the next rows
+    }
+    // lanes l and l + 32 hold different rows of the same channels
+#pragma unroll
+    for (int t = 0; t < kC3 / 32; ++t) {
+      const float o = __shfl_xor(vmax[t], 32, 64);
+      const float m = fmaxf(vmax[t], o);
+      if (lane < 32) atomicMax(&sh.cmax[t * 32 + lane], __float_as_uint(m));  // m >= 0: bits order as floats
+    }
+    __syncthreads();
+    for (int i = tid; i < kC3; i += kFThreads) G[(size_t)c * kC3 + i] = __uint_as_float(sh.cmax[i]);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -602,24 +760,6 @@ __global__ void k_adam(float* w, const float* g, float* m, float* v, size_t n, f
 // host side
 // ---------------------------------------------------------------------------
 
-// parameter layout (FP32 master and BF16 copy share offsets)
-struct ParamLayout {
-  size_t W1, b1, W2, b2, W3, b3, W4, b4, W5, b5, total;
-  ParamLayout() {
-    size_t o = 0;
-    W1 = o; o += (size_t)kC0 * kC1;
-    b1 = o; o += kC1;
-    W2 = o; o += (size_t)kC1 * kC2;
-    b2 = o; o += kC2;
-    W3 = o; o += (size_t)kC2 * kC3;
-    b3 = o; o += kC3;
-    W4 = o; o += (size_t)2 * kC3 * kH;
-    b4 = o; o += kH;
-    W5 = o; o += (size_t)kH * kOut;
-    b5 = o; o += kOut;
-    total = o;
-  }
-};
 
 struct Proposer {
   int device = 0, max_pairs = 0;
@@ -636,6 +776,8 @@ struct Proposer {
   DevBuf dtmp, dbf, dG, dF, dH1, dH1b, dOb;
   DevBuf rots, score;               // hypothesis rotations, candidate scores
   bool train_ready = false;
+  int fused_blocks = 256;           // workgroups of the fused point MLP (one per CU)
+  bool fused_off = false;           // inference through the layer-by-layer products
   std::vector<uint32_t> h_off, h_cnt;
   float last_ms = 0.f;
 
@@ -652,6 +794,11 @@ struct Proposer {
     RTR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     RTR_HIP(hipEventCreate(&ev0));
     RTR_HIP(hipEventCreate(&ev1));
+    RTR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_pool),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(FusedShared)));
+    int cus = 0;
+    RTR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    fused_blocks = cus > 0 ? cus : 256;
     const size_t P = L.total;
     const size_t clouds = 2 * (size_t)maxp, rows = clouds * kP;
     int rc = w.ensure(P * 4);
@@ -790,14 +937,26 @@ struct Proposer {
   const uint16_t* wbp(size_t off) const { return wb.as<uint16_t>() + off; }
   const float* wp(size_t off) const { return w.as<float>() + off; }
 
-  void enqueue_forward(bool with_labels) {
-    const int B = n_pairs, C = 2 * B;
+  // pooled features G of C clouds from x0: fused for inference, layer by
+  // layer (activations and argmax kept for the backward) for training
+  void enqueue_point_mlp(int C, bool keep_activations) {
+    if (!keep_activations) {
+      const int grid = C < fused_blocks ? C : fused_blocks;
+      hipLaunchKernelGGL(k_mlp_pool, dim3(grid), dim3(kFThreads), sizeof(FusedShared), stream, x0.as<uint16_t>(), C,
+                         wb.as<uint16_t>(), w.as<float>(), L, G.as<float>());
+      return;
+    }
     const int rows = C * kP;
     gemm(stream, x0.as<uint16_t>(), kC0, 0, wbp(L.W1), kC1, 0, y1.p, kC1, rows, kC1, kC0, wp(L.b1), 1, 1);
     gemm(stream, y1.as<uint16_t>(), kC1, 0, wbp(L.W2), kC2, 0, y2.p, kC2, rows, kC2, kC1, wp(L.b2), 1, 1);
     gemm(stream, y2.as<uint16_t>(), kC2, 0, wbp(L.W3), kC3, 0, y3.p, kC3, rows, kC3, kC2, wp(L.b3), 1, 1);
     hipLaunchKernelGGL(k_maxpool, dim3(C, 1), dim3(kC3), 0, stream, y3.as<uint16_t>(), kC3, C, G.as<float>(),
                        arg.as<int32_t>());
+  }
+
+  void enqueue_forward(bool with_labels, bool keep_activations) {
+    const int B = n_pairs, C = 2 * B;
+    enqueue_point_mlp(C, keep_activations);
     hipLaunchKernelGGL(k_concat, dim3(B), dim3(2 * kC3), 0, stream, G.as<float>(), B, F.as<uint16_t>());
     gemm(stream, F.as<uint16_t>(), 2 * kC3, 0, wbp(L.W4), kH, 0, h1.p, kH, B, kH, 2 * kC3, wp(L.b4), 1, 1);
     gemm(stream, h1.as<uint16_t>(), kH, 0, wbp(L.W5), kOut, 0, O.p, kOut, B, kOut, kH, wp(L.b5), 0, 0);
@@ -810,14 +969,9 @@ struct Proposer {
   // MLP over (k+1)*n clouds, head over k*n rows, then candidate scores.
   void enqueue_forward_multi(int k, float tau2) {
     const int n = n_pairs, C = (k + 1) * n, B = k * n;
-    const int rows = C * kP;
     hipLaunchKernelGGL(k_pack_multi, dim3(2048), dim3(256), 0, stream, pts.as<float>(), n, k, rots.as<float>(),
                        x0.as<uint16_t>());
-    gemm(stream, x0.as<uint16_t>(), kC0, 0, wbp(L.W1), kC1, 0, y1.p, kC1, rows, kC1, kC0, wp(L.b1), 1, 1);
-    gemm(stream, y1.as<uint16_t>(), kC1, 0, wbp(L.W2), kC2, 0, y2.p, kC2, rows, kC2, kC1, wp(L.b2), 1, 1);
-    gemm(stream, y2.as<uint16_t>(), kC2, 0, wbp(L.W3), kC3, 0, y3.p, kC3, rows, kC3, kC2, wp(L.b3), 1, 1);
-    hipLaunchKernelGGL(k_maxpool, dim3(C, 1), dim3(kC3), 0, stream, y3.as<uint16_t>(), kC3, C, G.as<float>(),
-                       arg.as<int32_t>());
+    enqueue_point_mlp(C, fused_off);
     hipLaunchKernelGGL(k_concat_multi, dim3(B), dim3(2 * kC3), 0, stream, G.as<float>(), n, k, F.as<uint16_t>());
     gemm(stream, F.as<uint16_t>(), 2 * kC3, 0, wbp(L.W4), kH, 0, h1.p, kH, B, kH, 2 * kC3, wp(L.b4), 1, 1);
     gemm(stream, h1.as<uint16_t>(), kH, 0, wbp(L.W5), kOut, 0, O.p, kOut, B, kOut, kH, wp(L.b5), 0, 0);
@@ -946,7 +1100,7 @@ int rtr_prop_propose(rtr_prop* h, double* poses) {
   }
   RTR_HIP(hipEventRecord(P.ev0, P.stream));
   P.enqueue_fps();
-  P.enqueue_forward(false);
+  P.enqueue_forward(false, P.fused_off);
   RTR_HIP(hipGetLastError());
   RTR_HIP(hipEventRecord(P.ev1, P.stream));
   std::vector<double> pose((size_t)P.n_pairs * 12);
@@ -969,6 +1123,12 @@ int rtr_prop_propose(rtr_prop* h, double* poses) {
       T[12 + r] = t[r] + ct[r] - (R[r * 3] * cs[0] + R[r * 3 + 1] * cs[1] + R[r * 3 + 2] * cs[2]);
     }
   }
+  return RTR_OK;
+}
+
+int rtr_prop_set_fused(rtr_prop* h, int on) {
+  if (!h) return RTR_ERR_INVALID_ARG;
+  h->p.fused_off = (on == 0);
   return RTR_OK;
 }
 
@@ -1081,7 +1241,7 @@ int rtr_prop_forward_backward(rtr_prop* h, int n_pairs, const float* xyz, const 
   if (!rc) rc = P.set_points(n_pairs, xyz);
   if (rc) return rc;
   if (labels) RTR_HIP(hipMemcpyAsync(P.lab.p, labels, (size_t)n_pairs * 12 * 8, hipMemcpyHostToDevice, P.stream));
-  P.enqueue_forward(labels != nullptr);
+  P.enqueue_forward(labels != nullptr, labels != nullptr || P.fused_off);
   if (labels && grad_out) P.enqueue_backward();
   RTR_HIP(hipGetLastError());
   if (O_out) RTR_HIP(hipMemcpyAsync(O_out, P.O.p, (size_t)n_pairs * kOut * 4, hipMemcpyDeviceToHost, P.stream));
@@ -1107,7 +1267,7 @@ int rtr_prop_train_step(rtr_prop* h, int n_pairs, const float* xyz, const double
   if (!rc) rc = P.set_points(n_pairs, xyz);
   if (rc) return rc;
   RTR_HIP(hipMemcpyAsync(P.lab.p, labels, (size_t)n_pairs * 12 * 8, hipMemcpyHostToDevice, P.stream));
-  P.enqueue_forward(true);
+  P.enqueue_forward(true, true);
   P.enqueue_backward();
   P.step += 1;
   const float b1 = 0.9f, b2 = 0.999f;

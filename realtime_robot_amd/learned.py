@@ -154,6 +154,11 @@ class Proposer:
         poses = np.ascontiguousarray(np.transpose(cm.reshape(n, k, 4, 4), (0, 1, 3, 2)))
         return poses, sc, np.ascontiguousarray(np.transpose(best.reshape(n, 4, 4), (0, 2, 1)))
 
+    def set_fused(self, on: bool):
+        """Inference through the fused point-MLP + max-pool kernel (default)
+        or the layer-by-layer products; the features are identical."""
+        L.check(L.lib.rtr_prop_set_fused(self._h, int(on)))
+
     def last_ms(self) -> float:
         ms = C.c_float()
         L.check(L.lib.rtr_prop_last_ms(self._h, C.byref(ms)))

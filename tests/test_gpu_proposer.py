@@ -243,3 +243,27 @@ def test_multi_hypothesis_proposals_and_scores():
             ref = np.minimum(d2, 0.01).mean()
             assert abs(scores[p, j] - ref) <= 1e-4 * ref + 1e-7, (p, j, scores[p, j], ref)
         assert np.allclose(best[p], poses[p, int(np.argmin(scores[p]))])
+
+
+def test_fused_point_mlp_is_bit_identical():
+    """The fused point-MLP + max-pool inference kernel and the layer-by-layer
+    BF16 products give bit-identical head outputs and proposals."""
+    n, k = 3, 4
+    srcs, tgts, _ = synth.config4_batch(n, seed_offset=1300, ns_range=(3000, 9000), nt_range=(5000, 20000))
+    rng = np.random.default_rng(21)
+    w = learned.init_weights(9)
+    w += rng.normal(0, 0.02, w.shape).astype(np.float32)
+    prop = learned.Proposer(max_pairs=k * n, weights=w)
+    prop.set_clouds(srcs, tgts)
+    pts, _, _ = prop.fps()
+    xyz = pts.reshape(-1, 3)
+    fused = prop.forward_backward(xyz, None)["O"]
+    lab = _labels(rng, n)
+    layered = prop.forward_backward(xyz, lab, grad=False)["O"]   # with labels: layer by layer
+    assert np.array_equal(fused, layered)
+    Q = multistart.so3_rotations(k)
+    p1, s1, b1 = prop.propose_multi(Q)
+    prop.set_fused(False)
+    p2, s2, b2 = prop.propose_multi(Q)
+    assert np.array_equal(p1, p2) and np.array_equal(s1, s2) and np.array_equal(b1, b2)
+    assert np.array_equal(prop.propose(), p2[:, 0])
