@@ -35,11 +35,18 @@ CANDIDATES = [
 # Few problems per launch (one refine per pair, a small fallback set): the
 # launch is bound by each problem's per-iteration latency, so smaller work
 # items (more workgroups per problem per iteration) are tried as well.
+# Fewer persistent workgroups per CU than the occupancy limit are tried too:
+# extra co-resident workgroups raise throughput but stretch each problem's
+# per-iteration latency.
 LATENCY_CANDIDATES = CANDIDATES + [
     (0.25, 2.0, 256, 0, 0),
     (0.35, 2.0, 256, 0, 0),
     (0.2, 3.0, 256, 0, 0),
     (0.35, 2.0, 512, 0, 0),
+    (0.25, 2.0, 512, 2, 0),
+    (0.25, 2.0, 256, 2, 0),
+    (0.35, 2.0, 512, 2, 0),
+    (0.2, 3.0, 512, 2, 0),
 ]
 
 
