@@ -259,3 +259,176 @@ def training_base(n_pairs: int, seed: int, ns_range=(2000, 6000), nt_range=(4000
         clouds = sample_many(pieces + pieces, nt + ns, rng, noise)
         yield clouds[m:], clouds[:m]
         done += m
+
+
+def room_faces(size=(5.0, 5.0, 2.5)):
+    """Floor (z = 0) and four walls of a room [0, sx] x [0, sy] x [0, sz] as
+    (origin, edge u, edge v, area) faces."""
+    sx, sy, sz = size
+    o = np.zeros(3)
+    return [(o, np.array([sx, 0, 0.0]), np.array([0, sy, 0.0]), sx * sy),
+            (o, np.array([sx, 0, 0.0]), np.array([0, 0, sz]), sx * sz),
+            (np.array([0, sy, 0.0]), np.array([sx, 0, 0.0]), np.array([0, 0, sz]), sx * sz),
+            (o, np.array([0, sy, 0.0]), np.array([0, 0, sz]), sy * sz),
+            (np.array([sx, 0, 0.0]), np.array([0, sy, 0.0]), np.array([0, 0, sz]), sy * sz)]
+
+
+def sample_faces(faces, n, rng, noise=0.0):
+    areas = np.array([f[3] for f in faces])
+    which = rng.choice(len(faces), size=n, p=areas / areas.sum())
+    uv = rng.random((n, 2))
+    O = np.stack([f[0] for f in faces])[which]
+    U = np.stack([f[1] for f in faces])[which]
+    V = np.stack([f[2] for f in faces])[which]
+    p = O + U * uv[:, :1] + V * uv[:, 1:2]
+    if noise > 0:
+        p = p + rng.normal(0.0, noise, size=p.shape)
+    return p.astype(np.float32)
+
+
+def floor_pose(rng, size=(5.0, 5.0, 2.5), margin=1.0):
+    """A furniture pose on the floor: yaw U[0, 360 deg], position inside the room."""
+    T = np.eye(4)
+    yaw = rng.uniform(0, 2 * np.pi)
+    c, s = np.cos(yaw), np.sin(yaw)
+    T[:2, :2] = [[c, -s], [s, c]]
+    T[0, 3] = rng.uniform(margin, size[0] - margin)
+    T[1, 3] = rng.uniform(margin, size[1] - margin)
+    return T
+
+
+def config2_scene(seed: int = 0, n_room: int = 300000, n_model: int = 30000, density: float = 4000.0,
+                  noise: float = 0.003, max_deg: float = 20.0, max_t: float = 0.2):
+    """Model-to-scene point-to-plane workload: a 5 x 5 x 2.5 m room (floor +
+    4 walls) sampled at n_room points plus 6 box-composite pieces standing on
+    the floor (the first is the chair that the model is, the others random
+    furniture, 0.4-2.0 m) sampled at the same surface density, noise
+    N(0, noise) per axis; the model is an independent n_model-point sample
+    of the chair in its own frame.  Returns (model, scene, T_gt, T_init):
+    T_gt maps the model onto its instance in the scene, T_init is T_gt
+    perturbed by a rotation U[0, max_deg] about the model centroid and a
+    translation U[-max_t, max_t]^3."""
+    rng = rng_for(2, seed)
+    parts = [sample_faces(room_faces(), n_room, rng, noise)]
+    T_gt = floor_pose(rng)
+    poses = [T_gt] + [floor_pose(rng) for _ in range(5)]
+    pieces = [CHAIR] + [random_composite(rng) for _ in range(5)]
+    for T, boxes in zip(poses, pieces):
+        area = sum(f[3] for f in box_faces(boxes))
+        pts = sample_boxes(boxes, int(area * density), rng, noise)
+        parts.append(apply(T, pts))
+    scene = np.concatenate(parts)
+    model = sample_boxes(CHAIR, n_model, rng, 0.001)
+    D = random_rigid(rng, max_deg, 0.0)
+    c = model.astype(np.float64).mean(0)
+    Dm = np.eye(4)
+    Dm[:3, :3] = D[:3, :3]
+    Dm[:3, 3] = c - D[:3, :3] @ c + rng.uniform(-max_t, max_t, 3)
+    return model, scene, T_gt, T_gt @ Dm
+
+
+# ---------------------------------------------------------------------------
+# depth rendering of the room scene (streaming reconstruction workload)
+# ---------------------------------------------------------------------------
+
+def room_boxes(seed: int = 0):
+    """The config-2 room's furniture as world-frame oriented boxes
+    (centre, half extents, 3x3 rotation), same random draws as config2_scene."""
+    rng = rng_for(2, seed)
+    sample_faces(room_faces(), 1, rng, 0.0)  # keep the generator's draw order
+    T_gt = floor_pose(rng)
+    poses = [T_gt] + [floor_pose(rng) for _ in range(5)]
+    pieces = [CHAIR] + [random_composite(rng) for _ in range(5)]
+    out = []
+    for T, boxes in zip(poses, pieces):
+        for c, s, yaw in boxes:
+            cy, sy = np.cos(yaw), np.sin(yaw)
+            Rb = np.array([[cy, -sy, 0], [sy, cy, 0], [0, 0, 1.0]])
+            out.append((T[:3, :3] @ np.asarray(c, np.float64) + T[:3, 3], np.asarray(s, np.float64) / 2,
+                        T[:3, :3] @ Rb))
+    return out
+
+
+def camera_path(n_frames: int, seed: int = 0, size=(5.0, 5.0, 2.5)):
+    """A smooth 6-DoF camera trajectory inside the room: the camera moves on
+    an ellipse at 1.2-1.6 m height and looks towards the room centre with a
+    gentle roll/pitch wobble.  Returns camera-to-world poses (n, 4, 4); the
+    camera looks along +z, x right, y down."""
+    rng = rng_for(5, seed)
+    ph = rng.uniform(0, 2 * np.pi)
+    poses = []
+    for k in range(n_frames):
+        a = ph + 0.9 * np.pi * k / max(n_frames - 1, 1)
+        pos = np.array([size[0] / 2 + 1.3 * np.cos(a), size[1] / 2 + 1.1 * np.sin(a),
+                        1.4 + 0.2 * np.sin(3.1 * a)])
+        target = np.array([size[0] / 2 + 0.6 * np.cos(2 * a + 1.0), size[1] / 2 + 0.6 * np.sin(1.5 * a), 0.6])
+        z = target - pos
+        z /= np.linalg.norm(z)
+        up = np.array([0, 0, 1.0])
+        x = np.cross(z, up)
+        x /= np.linalg.norm(x)
+        y = np.cross(z, x)
+        R = np.stack([x, y, z], axis=1)
+        roll = 0.08 * np.sin(2.3 * a)
+        cr, sr = np.cos(roll), np.sin(roll)
+        R = R @ np.array([[cr, -sr, 0], [sr, cr, 0], [0, 0, 1.0]])
+        T = np.eye(4)
+        T[:3, :3] = R
+        T[:3, 3] = pos
+        poses.append(T)
+    return np.stack(poses)
+
+
+def render_depth(T_cw, boxes, width=640, height=480, fx=525.0, fy=525.0, cx=319.5, cy=239.5,
+                 size=(5.0, 5.0, 2.5), noise_rng=None, noise_k: float = 0.0015, max_depth: float = 8.0):
+    """Depth image (metres along the optical axis, 0 = no hit) of the room
+    (floor + 4 walls, open ceiling) and the boxes seen from camera-to-world
+    pose T_cw, with depth noise N(0, noise_k * z^2) when noise_rng is given."""
+    u, v = np.meshgrid(np.arange(width, dtype=np.float64), np.arange(height, dtype=np.float64))
+    d_cam = np.stack([(u - cx) / fx, (v - cy) / fy, np.ones_like(u)], axis=-1).reshape(-1, 3)
+    R, o = T_cw[:3, :3], T_cw[:3, 3]
+    d = d_cam @ R.T
+    t_best = np.full(len(d), np.inf)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # room: floor z = 0 and the four walls, hit from inside
+        for ax, val in ((2, 0.0), (0, 0.0), (0, size[0]), (1, 0.0), (1, size[1])):
+            t = (val - o[ax]) / d[:, ax]
+            p = o + t[:, None] * d
+            ok = (t > 1e-6) & np.all((p >= -1e-9) & (p <= np.array(size) + 1e-9), axis=1)
+            t_best = np.where(ok & (t < t_best), t, t_best)
+        for c, h, Rb in boxes:
+            ol = (o - c) @ Rb
+            dl = d @ Rb
+            t1 = (-h - ol) / dl
+            t2 = (h - ol) / dl
+            tmin = np.nanmax(np.minimum(t1, t2), axis=1)
+            tmax = np.nanmin(np.maximum(t1, t2), axis=1)
+            ok = (tmax >= np.maximum(tmin, 0.0)) & (tmin > 1e-6)
+            t_best = np.where(ok & (tmin < t_best), tmin, t_best)
+    z = t_best  # d_cam has unit z, so the ray parameter is the depth
+    z[~np.isfinite(z) | (z > max_depth)] = 0.0
+    if noise_rng is not None:
+        hit = z > 0
+        z[hit] += noise_rng.normal(0.0, 1.0, hit.sum()) * noise_k * z[hit] ** 2
+    return z.reshape(height, width).astype(np.float32)
+
+
+def backproject(depth, fx=525.0, fy=525.0, cx=319.5, cy=239.5):
+    """Camera-frame points (n, 3) float32 of the valid pixels."""
+    h, w = depth.shape
+    v, u = np.nonzero(depth > 0)
+    z = depth[v, u].astype(np.float64)
+    return np.stack([(u - cx) / fx * z, (v - cy) / fy * z, z], axis=1).astype(np.float32)
+
+
+def config5_frames(n_frames: int = 64, seed: int = 0, width: int = 640, height: int = 480):
+    """Streaming workload: depth frames of the config-2 room along camera_path,
+    depth noise N(0, 0.0015 z^2).  Returns (depth images, camera poses,
+    intrinsics scale) with intrinsics scaled to the resolution."""
+    boxes = room_boxes(seed)
+    poses = camera_path(n_frames, seed)
+    s = width / 640.0
+    rng = rng_for(5, 1000 + seed)
+    depths = [render_depth(T, boxes, width, height, 525.0 * s, 525.0 * s, (width - 1) / 2.0, (height - 1) / 2.0,
+                           noise_rng=rng) for T in poses]
+    return np.stack(depths), poses, s
