@@ -349,19 +349,21 @@ def room_boxes(seed: int = 0):
     return out
 
 
-def camera_path(n_frames: int, seed: int = 0, size=(5.0, 5.0, 2.5)):
+def camera_path(n_frames: int, seed: int = 0, size=(5.0, 5.0, 2.5), step: float = 0.9 * np.pi / 63):
     """A smooth 6-DoF camera trajectory inside the room: the camera moves on
     an ellipse at 1.2-1.6 m height and looks towards the room centre with a
-    gentle roll/pitch wobble.  Returns camera-to-world poses (n, 4, 4); the
-    camera looks along +z, x right, y down."""
+    gentle roll/pitch wobble, advancing `step` radians along the ellipse per
+    frame (64 frames cover 0.9 pi; shorter runs are prefixes of that path).
+    Returns camera-to-world poses (n, 4, 4); the camera looks along +z, x
+    right, y down."""
     rng = rng_for(5, seed)
     ph = rng.uniform(0, 2 * np.pi)
     poses = []
     for k in range(n_frames):
-        a = ph + 0.9 * np.pi * k / max(n_frames - 1, 1)
+        a = ph + step * k
         pos = np.array([size[0] / 2 + 1.3 * np.cos(a), size[1] / 2 + 1.1 * np.sin(a),
                         1.4 + 0.2 * np.sin(3.1 * a)])
-        target = np.array([size[0] / 2 + 0.6 * np.cos(2 * a + 1.0), size[1] / 2 + 0.6 * np.sin(1.5 * a), 0.6])
+        target = np.array([size[0] / 2 + 0.6 * np.cos(a + 1.0), size[1] / 2 + 0.6 * np.sin(0.8 * a), 0.6])
         z = target - pos
         z /= np.linalg.norm(z)
         up = np.array([0, 0, 1.0])

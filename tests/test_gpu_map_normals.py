@@ -128,8 +128,12 @@ def test_batch_fitness_matches_oracle():
 def test_config2_model_to_scene_point_to_plane(seed):
     """Model-to-scene workload: 1 cm downsample of model and room scene,
     k = 20 PCA normals of the scene on the device, point-to-plane ICP from a
-    20 deg / 0.2 m perturbation; the pose equals the FP64 oracle's from the
-    same init (same normals and anchor) and lands within 2 mm of the truth."""
+    20 deg / 0.2 m perturbation.  Fifteen iterations (no early stop) equal
+    the FP64 oracle's from the same init (same normals and anchor); run to
+    convergence the pose lands within 2 mm of the truth.  (Runs to
+    convergence may stop an iteration or two apart when the increments hover
+    at the transformation epsilon, so the arithmetic is compared at a fixed
+    iteration count.)"""
     model, scene, T_gt, T0 = synth.config2_scene(seed)
     md = R.voxel_downsample(model, 0.01)
     sd = R.voxel_downsample(scene, 0.01)
@@ -138,18 +142,23 @@ def test_config2_model_to_scene_point_to_plane(seed):
     icp.setInputTarget(sd)
     icp.computeTargetNormals(20, (2.5, 2.5, 1.5))
     icp.setMaxCorrespondenceDistance(0.1)
+    icp.setMaximumIterations(15)
+    icp.setTransformationEpsilon(0.0)
+    icp.align(T0)
+    T = icp.getFinalTransformation()
+    nrm = icp.getTargetNormals()
+    ref = oracle.icp(md, sd, oracle.params(max_iter=15, r_max=0.1, eps_T=0.0, mode=1, anchor=icp.getAnchor()),
+                     guess=T0, tgt_normals=nrm)
+    print(f"config 2 seed {seed}: {len(md)} model / {len(sd)} scene points, 15 iterations, vs oracle "
+          f"{rot_err(T, ref['T']):.1e} / {trans_err(T, ref['T']):.1e}")
+    assert icp.getResult()["iterations"] == ref["iterations"] == 15
+    assert rot_err(T, ref["T"]) < 1e-5 and trans_err(T, ref["T"]) < 1e-5
     icp.setMaximumIterations(50)
     icp.setTransformationEpsilon(1e-8)
     icp.align(T0)
     T = icp.getFinalTransformation()
-    nrm = icp.getTargetNormals()
-    ref = oracle.icp(md, sd, oracle.params(max_iter=50, r_max=0.1, eps_T=1e-8, mode=1, anchor=icp.getAnchor()),
-                     guess=T0, tgt_normals=nrm)
-    res = icp.getResult()
-    print(f"config 2 seed {seed}: {len(md)} model / {len(sd)} scene points, iterations {res['iterations']} "
-          f"(oracle {ref['iterations']}), vs oracle {rot_err(T, ref['T']):.1e} / {trans_err(T, ref['T']):.1e}, "
-          f"vs truth {synth.rmse_between(T, T_gt, model) * 1e3:.2f} mm")
-    assert rot_err(T, ref["T"]) < 1e-5 and trans_err(T, ref["T"]) < 1e-5
+    print(f"  converged in {icp.getResult()['iterations']} iterations, {synth.rmse_between(T, T_gt, model) * 1e3:.2f} "
+          f"mm from the truth")
     assert synth.rmse_between(T, T_gt, model) < 2e-3
 
 
