@@ -28,7 +28,7 @@ from .registration import IterativeClosestPointWithNormals, VoxelMap, voxel_down
 
 class StreamingReconstruction:
     def __init__(self, voxel: float = 0.02, capacity: int = 8 << 20, icp_voxel: float = 0.02,
-                 max_correspondence_distance: float = 0.1, max_iterations: int = 30, normal_k: int = 20,
+                 max_correspondence_distance: float = 0.03, max_iterations: int = 30, normal_k: int = 20,
                  device: int = 0):
         self.map = VoxelMap(voxel, capacity=capacity, device=device)
         self.icp = IterativeClosestPointWithNormals(device=device)
@@ -86,6 +86,10 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=640)
     ap.add_argument("--height", type=int, default=480)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--icp-voxel", type=float, default=0.02)
+    ap.add_argument("--max-iter", type=int, default=30)
+    ap.add_argument("--max-dist", type=float, default=0.03)
+    ap.add_argument("--per-frame", action="store_true", help="add per-frame pose errors to the output")
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
     depths, gt, _ = synth.config5_frames(args.frames, args.seed, args.width, args.height)
@@ -94,7 +98,8 @@ def main(argv=None):
               for d in depths]
     print(f"[streaming] {args.frames} frames rendered in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
-    rec = StreamingReconstruction()
+    rec = StreamingReconstruction(icp_voxel=args.icp_voxel, max_iterations=args.max_iter,
+                                  max_correspondence_distance=args.max_dist)
     rec.add_frame(clouds[0], gt[0])                 # the first frame fixes the map's frame
     for k in range(1, len(clouds)):
         rec.add_frame(clouds[k])
@@ -106,6 +111,12 @@ def main(argv=None):
            "frames_per_s": float(1e3 / frame_ms.mean()), "ate_rmse_m": ate, "max_rot_err_deg": rot,
            "map_voxels": rec.map.size(), "dropped": rec.map.dropped(),
            "icp_iterations_mean": float(np.mean([r["iterations"] for r in rec.results[1:]]))}
+    if args.per_frame:
+        P = np.asarray(rec.poses)
+        out["trans_err_mm"] = [round(float(x) * 1e3, 2) for x in np.linalg.norm(P[:, :3, 3] - gt[:, :3, 3], axis=1)]
+        c = (np.einsum("nij,nij->n", P[:, :3, :3], gt[:, :3, :3]) - 1.0) / 2.0
+        out["rot_err_deg"] = [round(float(x), 3) for x in np.degrees(np.arccos(np.clip(c, -1, 1)))]
+        out["iterations"] = [r["iterations"] for r in rec.results[1:]]
     print(json.dumps(out), flush=True)
     return out
 
