@@ -170,12 +170,15 @@ RTR_HD float slab_gap(float q, float o, float h, float margin, int a, int b) {
 // x0..x1 of row (y, z): the Euclidean distance to the box of those cells with
 // each axis gap shortened by g.margin (float rounding of the cell assignment,
 // as in ring_done), squared and scaled by (1 - 2e-6), which undercuts the
-// float d2 of any point in the box.
+// float d2 of any point in the box.  The three squared gaps are separable, so
+// walks compute each axis once and combine them with lb2_of.
+RTR_HD float lb2_of(float gx2, float gy2, float gz2) { return ((gx2 + gy2) + gz2) * 0.999998f; }
+
 RTR_HD float row_lb2(const GridView& g, float qx, float qy, float qz, int x0, int x1, int y, int z) {
   const float ax = slab_gap(qx, g.ox, g.h, g.margin, x0, x1);
   const float ay = slab_gap(qy, g.oy, g.h, g.margin, y, y);
   const float az = slab_gap(qz, g.oz, g.h, g.margin, z, z);
-  return ((ax * ax + ay * ay) + az * az) * 0.999998f;
+  return lb2_of(ax * ax, ay * ay, az * az);
 }
 
 // A row segment can be skipped when its lower bound exceeds both the best
@@ -196,10 +199,21 @@ RTR_HD void walk_block(const GridView& g, float qx, float qy, float qz, int cx, 
                        V& vis) {
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
+  const float ax = slab_gap(qx, g.ox, g.h, g.margin, x0, x1);
+  float gy2[3], gz2[3];
+#pragma unroll
+  for (int d = -1; d <= 1; ++d) {
+    const float ay = slab_gap(qy, g.oy, g.h, g.margin, cy + d, cy + d);
+    const float az = slab_gap(qz, g.oz, g.h, g.margin, cz + d, cz + d);
+    gy2[d + 1] = ay * ay;
+    gz2[d + 1] = az * az;
+  }
+  const float gx2 = ax * ax;
   for (int k = 0; k < 9; ++k) {
-    const int y = cy + kBlockRowOrder[k][0], z = cz + kBlockRowOrder[k][1];
+    const int dy = kBlockRowOrder[k][0], dz = kBlockRowOrder[k][1];
+    const int y = cy + dy, z = cz + dz;
     if (z < 0 || z >= g.dz || y < 0 || y >= g.dy) continue;
-    if (k > 0 && row_skip(row_lb2(g, qx, qy, qz, x0, x1, y, z), vis.bound(), r2f)) continue;
+    if (k > 0 && row_skip(lb2_of(gx2, gy2[dy + 1], gz2[dz + 1]), vis.bound(), r2f)) continue;
     const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
     vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
   }
@@ -210,23 +224,32 @@ RTR_HD void walk_block(const GridView& g, float qx, float qy, float qz, int cx, 
 template <class V>
 RTR_HD void walk_shell(const GridView& g, float qx, float qy, float qz, int cx, int cy, int cz, int k,
                        float r2f, V& vis) {
+  const int fx0 = clampi(cx - k, 0, g.dx - 1), fx1 = clampi(cx + k, 0, g.dx - 1);
+  const float afx = slab_gap(qx, g.ox, g.h, g.margin, fx0, fx1);
+  const float ae0 = slab_gap(qx, g.ox, g.h, g.margin, cx - k, cx - k);
+  const float ae1 = slab_gap(qx, g.ox, g.h, g.margin, cx + k, cx + k);
+  const float gfx2 = afx * afx, ge02 = ae0 * ae0, ge12 = ae1 * ae1;
   for (int dz = -k; dz <= k; ++dz) {
     const int z = cz + dz;
     if (z < 0 || z >= g.dz) continue;
+    const float az = slab_gap(qz, g.oz, g.h, g.margin, z, z);
+    const float gz2 = az * az;
+    // the whole plane is beyond the bound: every row in it is
+    if (row_skip(lb2_of(gfx2, 0.f, gz2), vis.bound(), r2f)) continue;
     const bool zface = (dz == -k || dz == k);
     for (int dy = -k; dy <= k; ++dy) {
       const int y = cy + dy;
       if (y < 0 || y >= g.dy) continue;
+      const float ay = slab_gap(qy, g.oy, g.h, g.margin, y, y);
+      const float gy2 = ay * ay;
       const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
       if (zface || dy == -k || dy == k) {
-        const int x0 = clampi(cx - k, 0, g.dx - 1);
-        const int x1 = clampi(cx + k, 0, g.dx - 1);
-        if (row_skip(row_lb2(g, qx, qy, qz, x0, x1, y, z), vis.bound(), r2f)) continue;
-        vis.scan(g.cell_start[row + x0], g.cell_start[row + x1 + 1]);
+        if (row_skip(lb2_of(gfx2, gy2, gz2), vis.bound(), r2f)) continue;
+        vis.scan(g.cell_start[row + fx0], g.cell_start[row + fx1 + 1]);
       } else {
-        if (cx - k >= 0 && !row_skip(row_lb2(g, qx, qy, qz, cx - k, cx - k, y, z), vis.bound(), r2f))
+        if (cx - k >= 0 && !row_skip(lb2_of(ge02, gy2, gz2), vis.bound(), r2f))
           vis.scan(g.cell_start[row + cx - k], g.cell_start[row + cx - k + 1]);
-        if (cx + k < g.dx && !row_skip(row_lb2(g, qx, qy, qz, cx + k, cx + k, y, z), vis.bound(), r2f))
+        if (cx + k < g.dx && !row_skip(lb2_of(ge12, gy2, gz2), vis.bound(), r2f))
           vis.scan(g.cell_start[row + cx + k], g.cell_start[row + cx + k + 1]);
       }
     }

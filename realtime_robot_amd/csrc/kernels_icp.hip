@@ -209,12 +209,21 @@ __device__ __forceinline__ void walk_block_global(const GridView& g, float qx, f
       rb[k] = re[k] = 0u;
     }
   }
+  // separable squared gaps of the side rows (the x extent is shared)
+  const float ax = slab_gap(qx, g.ox, g.h, g.margin, x0, x1);
+  const float ay0 = slab_gap(qy, g.oy, g.h, g.margin, cy - 1, cy - 1);
+  const float ay1 = slab_gap(qy, g.oy, g.h, g.margin, cy + 1, cy + 1);
+  const float az0 = slab_gap(qz, g.oz, g.h, g.margin, cz - 1, cz - 1);
+  const float az1 = slab_gap(qz, g.oz, g.h, g.margin, cz + 1, cz + 1);
+  const float gx2 = ax * ax;
+  const float gy2[3] = {ay0 * ay0, 0.f, ay1 * ay1};
+  const float gz2[3] = {az0 * az0, 0.f, az1 * az1};
   vis.scan(rb[0], re[0]);
 #pragma unroll
   for (int k = 1; k < 9; ++k) {
     if (rb[k] == re[k]) continue;
-    const int y = cy + kBlockRowOrder[k][0], z = cz + kBlockRowOrder[k][1];
-    if (row_skip(row_lb2(g, qx, qy, qz, x0, x1, y, z), vis.bound(), r2f)) continue;
+    const float lb2 = lb2_of(gx2, gy2[kBlockRowOrder[k][0] + 1], gz2[kBlockRowOrder[k][1] + 1]);
+    if (row_skip(lb2, vis.bound(), r2f)) continue;
     vis.scan(rb[k], re[k]);
   }
 }
