@@ -250,6 +250,22 @@ int rtr_pcd_write_ascii(const char* path, const float* xyz, size_t n, size_t str
 int rtr_copy_probe(int device, size_t bytes, int reps, float* ms_avg);
 
 /* ------------------------------------------------------------------ */
+/* Truncated distance field (TDF) of an occupancy grid: replaces        */
+/* ComputeTDFWithCuda (kernel.cu:34-35, declared key_point.h:35-36).     */
+/* out[x + y*dims[0] + z*dims[0]*dims[1]] = min(trunc2, squared distance */
+/* in voxels to the nearest occupied voxel) -- the reference's values    */
+/* (kernel.cu:9-31; trunc2 = 900 there) for any grid size, computed by   */
+/* an exact separable distance transform.  Occupied coordinates outside  */
+/* the grid are ignored (the reference indexes without a bounds check,   */
+/* matching.h:181-187).                                                  */
+/* ------------------------------------------------------------------ */
+int rtr_tdf(int device, const int32_t* occ_xyz, size_t n_occ, const int dims[3], float trunc2, float* out);
+/* Same from points: voxel = floor((p - origin) / resolution) inside the
+ * grid (the occupancy of key_point.h:287-304 and matching.h:169-186). */
+int rtr_tdf_points(int device, const float* xyz, size_t n, size_t stride_bytes, const float origin[3],
+                   float resolution, const int dims[3], float trunc2, float* out);
+
+/* ------------------------------------------------------------------ */
 /* Learned pose proposer: farthest-point sampling (1024 points per      */
 /* cloud), shared point MLP 3-64-128-256 + max-pool per cloud, head     */
 /* 512-256-9 (6D rotation + translation), BF16 MFMA with FP32           */

@@ -76,6 +76,7 @@ _vox = _sig("orc_voxel_accumulate", C.c_int64, [_f32p, C.c_int64, C.c_void_p, C.
                                                 _i64p, C.c_int64, _i64p])
 _normals = _sig("orc_normals", None, [_f32p, C.c_int64, C.c_int, _f32p, _f32p, _f32p])
 _downsample = _sig("orc_voxel_downsample", C.c_int64, [_f32p, C.c_int64, C.c_float, _f32p, C.c_int64, _i64p])
+_tdf = _sig("orc_tdf", None, [_i32p, C.c_int64, _i32p, C.c_float, _f32p])
 _fps = _sig("orc_fps", None, [_f32p, C.c_int64, C.c_int64, C.c_int, _i32p, _f64p])
 _pcl_icp = _sig("orc_pcl_icp", C.c_int, [_f32p, C.c_int64, _f32p, C.c_int64, C.POINTER(IcpParams), _f64p,
                                          C.POINTER(IcpResult)])
@@ -243,6 +244,15 @@ def voxel_downsample(pts, leaf):
     counts = np.empty(max(len(p), 1), np.int64)
     nv = _downsample(p, len(p), np.float32(leaf), out, len(p), counts)
     return out[:nv].copy(), counts[:nv].copy()
+
+
+def tdf(occupied, dims, trunc2=900.0):
+    """Brute-force truncated squared distance field, (dz, dy, dx) float32."""
+    occ = np.ascontiguousarray(occupied, np.int32).reshape(-1, 3)
+    d = np.ascontiguousarray(dims, np.int32).reshape(3)
+    out = np.empty((int(d[2]), int(d[1]), int(d[0])), np.float32)
+    _tdf(occ, len(occ), d, np.float32(trunc2), out)
+    return out
 
 
 def fps(pts, k=1024, max_sub=8192):

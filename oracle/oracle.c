@@ -1139,3 +1139,21 @@ void orc_fps(const float* pts, int64_t n, int64_t max_sub, int k, int32_t* sel, 
   free(sp);
   free(dist);
 }
+
+/* ---- truncated distance field (restates kernel.cu:9-31, bounds fixed) ---- */
+void orc_tdf(const int32_t* occ, int64_t n_occ, const int dims[3], float trunc2, float* out) {
+  const int dx = dims[0], dy = dims[1], dz = dims[2];
+  for (int z = 0; z < dz; ++z)
+    for (int y = 0; y < dy; ++y)
+      for (int x = 0; x < dx; ++x) {
+        float d = trunc2;
+        for (int64_t i = 0; i < n_occ; ++i) {
+          const int ox = occ[3 * i], oy = occ[3 * i + 1], oz = occ[3 * i + 2];
+          if (ox < 0 || oy < 0 || oz < 0 || ox >= dx || oy >= dy || oz >= dz) continue;
+          const int a = x - ox, b = y - oy, c = z - oz;
+          const float t = (float)(a * a + b * b + c * c);
+          if (t < d) d = t;
+        }
+        out[x + (int64_t)dx * (y + (int64_t)dy * z)] = d;
+      }
+}

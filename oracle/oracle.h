@@ -142,6 +142,12 @@ int64_t orc_voxel_downsample(const float* pts, int64_t n, float leaf, float* out
  * of the subsample). */
 void orc_fps(const float* pts, int64_t n, int64_t max_sub, int k, int32_t* sel, double* centroid);
 
+/* Truncated distance field by brute force, as kernel.cu:9-31 computes it:
+ * out[x + y*dx + z*dx*dy] = min(trunc2, min over occupied voxels of the
+ * integer squared distance), with occupied coordinates outside the grid
+ * ignored (bounds fixed). */
+void orc_tdf(const int32_t* occ, int64_t n_occ, const int dims[3], float trunc2, float* out);
+
 /* ---- PCL-semantics CPU baseline (float, cumulative, kd-tree leaf 15) ---- */
 int orc_pcl_icp(const float* src, int64_t ns, const float* tgt, int64_t nt, const orc_icp_params* p,
                 const double guess[16], orc_icp_result* out);

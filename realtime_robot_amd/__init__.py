@@ -8,9 +8,9 @@ Python host mirror.
 """
 from ._lib import RtrError, Params, Result, MODE_POINT_TO_POINT, MODE_POINT_TO_PLANE  # noqa: F401
 from .registration import (BatchRegistration, IterativeClosestPoint, IterativeClosestPointWithNormals,  # noqa: F401
-                           VoxelMap, load_pcd, make_params, save_pcd_ascii, selfcheck_nn, selfcheck_solve,
-                           voxel_downsample)
+                           VoxelMap, load_pcd, make_params, save_pcd_ascii, selfcheck_nn, selfcheck_solve, tdf,
+                           tdf_points, voxel_downsample)
 
 __all__ = ["IterativeClosestPoint", "IterativeClosestPointWithNormals", "BatchRegistration", "VoxelMap",
-           "voxel_downsample", "load_pcd", "save_pcd_ascii", "make_params", "selfcheck_nn", "selfcheck_solve",
+           "voxel_downsample", "tdf", "tdf_points", "load_pcd", "save_pcd_ascii", "make_params", "selfcheck_nn", "selfcheck_solve",
            "RtrError", "Params", "Result"]

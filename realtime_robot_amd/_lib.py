@@ -114,6 +114,8 @@ _SIGS = {
     "rtr_pcd_read": (_i, [C.c_char_p, _vp, _vp, _sz, C.POINTER(_sz)]),
     "rtr_pcd_write_ascii": (_i, [C.c_char_p, _vp, _sz, _sz]),
     "rtr_copy_probe": (_i, [_i, _sz, _i, C.POINTER(_f)]),
+    "rtr_tdf": (_i, [_i, _vp, _sz, _vp, _f, _vp]),
+    "rtr_tdf_points": (_i, [_i, _vp, _sz, _sz, _vp, _f, _vp, _f, _vp]),
     "rtr_prop_create": (_i, [C.POINTER(_vp), _i, _i]),
     "rtr_prop_destroy": (_i, [_vp]),
     "rtr_prop_num_params": (_i, [C.POINTER(_sz)]),

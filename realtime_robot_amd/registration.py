@@ -390,3 +390,26 @@ def selfcheck_solve(mode, a, b, normals=None, anchor=(0.0, 0.0, 0.0)):
     T = np.empty(16)
     L.check(L.lib.rtr_selfcheck_solve(mode, L.ptr(a), L.ptr(b), L.ptr(n), len(a), L.ptr(an), L.ptr(T)))
     return L.rowmajor(T)
+
+
+def tdf(occupied, dims, trunc2: float = 900.0, device: int = 0) -> np.ndarray:
+    """Truncated squared distance field (voxel units) of an occupancy grid:
+    occupied (n, 3) int voxel coordinates, dims (dx, dy, dz); returns a
+    (dz, dy, dx) float32 array (x fastest in memory), the values of the
+    reference's ComputeTDF (kernel.cu:9-31) for any grid size."""
+    occ = np.ascontiguousarray(occupied, np.int32).reshape(-1, 3)
+    d = np.ascontiguousarray(dims, np.int32).reshape(3)
+    out = np.empty((int(d[2]), int(d[1]), int(d[0])), np.float32)
+    L.check(L.lib.rtr_tdf(device, L.ptr(occ), len(occ), L.ptr(d), float(trunc2), L.ptr(out)))
+    return out
+
+
+def tdf_points(points, origin, resolution: float, dims, trunc2: float = 900.0, device: int = 0) -> np.ndarray:
+    """tdf() of the voxels floor((p - origin) / resolution) the points occupy."""
+    p = L.pts3(points)
+    o = np.ascontiguousarray(origin, np.float32).reshape(3)
+    d = np.ascontiguousarray(dims, np.int32).reshape(3)
+    out = np.empty((int(d[2]), int(d[1]), int(d[0])), np.float32)
+    L.check(L.lib.rtr_tdf_points(device, L.ptr(p), len(p), 12, L.ptr(o), float(resolution), L.ptr(d),
+                                 float(trunc2), L.ptr(out)))
+    return out

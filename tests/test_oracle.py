@@ -358,3 +358,21 @@ def test_pcl_semantics_baseline_agrees_with_fp64_oracle():
     assert synth.rmse_between(a["T"], b["T"], src) < 1e-3
     batch = oracle.pcl_icp_batch([src, src], [tgt, tgt], p, nthreads=2)
     assert np.array_equal(batch[0]["T"], a["T"]) and np.array_equal(batch[1]["T"], a["T"])
+
+
+def test_tdf_oracle_matches_scipy_edt():
+    """The brute-force TDF (kernel.cu:9-31 semantics, bounds fixed) equals
+    scipy's exact Euclidean distance transform, squared and capped."""
+    from scipy import ndimage
+    rng = np.random.default_rng(31)
+    dims = (13, 9, 11)
+    occ = rng.integers(0, [13, 9, 11], size=(25, 3))
+    out = oracle.tdf(occ, dims, 30.0)
+    grid = np.ones((dims[2], dims[1], dims[0]), bool)
+    grid[occ[:, 2], occ[:, 1], occ[:, 0]] = False
+    ref = np.minimum(ndimage.distance_transform_edt(grid) ** 2, 30.0)
+    assert np.array_equal(out, ref.astype(np.float32))
+    # out-of-grid voxels are ignored; an empty grid is the cap everywhere
+    assert np.array_equal(oracle.tdf([[-1, 0, 0], [13, 0, 0]], dims, 900.0), np.full(out.shape, 900.0, np.float32))
+    one = oracle.tdf([[2, 3, 4]], (5, 5, 5), 900.0)
+    assert one[4, 3, 2] == 0 and one[0, 0, 0] == 4 + 9 + 16
