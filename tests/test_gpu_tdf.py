@@ -28,7 +28,7 @@ def test_tdf_of_key_point_box_like_the_reference():
     center = tgt[1234].astype(np.float64)
     lo = (center - 0.15).astype(np.float32)
     got = R.tdf_points(tgt, lo, 0.01, (30, 30, 30), 900.0)
-    f = (tgt - lo) * np.float32(1.0 / np.float32(0.01))
+    f = (tgt - lo) * (np.float32(1.0) / np.float32(0.01))
     inside = np.all((f >= 0) & (f < 30), axis=1)
     occ = np.unique(f[inside].astype(np.int32), axis=0)
     ref = oracle.tdf(occ, (30, 30, 30), 900.0)
