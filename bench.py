@@ -11,8 +11,9 @@ before timing starts.
 
 One step of the learned path = farthest-point sampling + the BF16 network on
 8 orientation hypotheses per pair + candidate scoring + one batched ICP
-refine of every pair's best proposal + the registration judge + 8-start ICP
-for the pairs the judge rejects + best-start selection.  One step of the
+refine of every pair's best proposal (Anderson-accelerated, memory 5, unless
+--refine-anderson 0) + the registration judge + plain 8-start ICP for the
+pairs the judge rejects + best-start selection.  One step of the
 plain path = 8-start ICP of every pair (starts spread over SO(3)) + best-start
 selection.  A registration counts when its pose is within 1 mm RMSE (over
 the source points) of the reference pose: the GPU ICP started at the
@@ -92,6 +93,50 @@ def fp64_check(srcs, tgts, gts, T_ref, inits, T_gpu, n_chk):
             "within_1mm": int(sum(x <= OK_RMSE for x in rb))}
 
 
+def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample):
+    """Algorithmic HBM bytes of one k_icp launch over its duration.
+
+    The launch is timed with HIP events on the engine's stream (median of 3).
+    Bytes per problem-iteration are B_iter = 12 Ns + sum over the query cells
+    of 12 |N27(c)| at a reference grid of edge r_max (accounting.py); B_iter is
+    the compulsory 12 (Ns + Nt) scaled by the B_iter / B_min ratio measured on
+    a sample of problems along their own iteration histories."""
+    from realtime_robot_amd import accounting
+    reg.set_profiling(True)
+    icp_ms, grid_ms = [], []
+    for _ in range(3):
+        res = reg.align(guesses)
+        t = reg.timing()
+        icp_ms.append(t["ms_icp"])
+        grid_ms.append(t["ms_grid"])
+    reg.set_profiling(False)
+    iters = np.array([r["iterations"] for r in res], np.int64)
+    ns = np.array([len(srcs[i]) for i in pair_of], np.int64)
+    nt = np.array([len(tgts[i]) for i in pair_of], np.int64)
+    b_min_total = float(np.sum(iters * (12 * ns + 12 * nt)))
+    rng = np.random.default_rng(0)
+    sample = rng.choice(len(pair_of), min(n_sample, len(pair_of)), replace=False)
+    counts = {}
+    s_iter = s_min = 0
+    for p in sample:
+        i = int(pair_of[p])
+        if i not in counts:
+            counts[i] = accounting.TargetCounts(tgts[i], R_MAX)
+        hist = reg.history(int(p), MAX_ITER)
+        for it in range(res[p]["iterations"]):
+            bi, bm = accounting.iteration_bytes(srcs[i], counts[i], hist[it])
+            s_iter += bi
+            s_min += bm
+    ratio = s_iter / s_min if s_min else 1.0
+    b_iter_total = b_min_total * ratio
+    t_icp = float(np.median(icp_ms)) / 1e3
+    return {"achieved": b_iter_total / t_icp / 1e9, "kernel_ms": t_icp * 1e3,
+            "grid_build_ms": float(np.median(grid_ms)), "algorithmic_bytes": b_iter_total,
+            "compulsory_bytes": b_min_total, "achieved_compulsory_gbs": b_min_total / t_icp / 1e9,
+            "b_iter_over_b_min": ratio, "iterations_total": int(iters.sum()),
+            "b_iter_sample_problems": int(len(sample))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +145,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--starts", type=int, default=8)
     ap.add_argument("--hypotheses", type=int, default=8)
+    ap.add_argument("--refine-anderson", type=int, default=5,
+                    help="Anderson memory of the learned path's refine launch (0: plain iteration)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
                     help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile'")
@@ -108,8 +155,9 @@ def main():
     ap.add_argument("--fp64-sample", type=int, default=8)
     ap.add_argument("--account-sample", type=int, default=24)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
-    ap.add_argument("--leg", choices=["both", "refine"], default="both",
-                    help="'refine': only the learned path's refine launches (for a kernel profile)")
+    ap.add_argument("--leg", choices=["both", "refine", "learned"], default="both",
+                    help="'refine': only the learned path's refine launches; 'learned': only learned steps "
+                         "(kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): registrations are independent and "
@@ -130,7 +178,7 @@ def main():
     params = learned.default_params()
 
     T_ref = None
-    if args.leg == "both":
+    if args.leg in ("both", "learned"):
         # reference poses: GPU ICP from the generating pose
         ref = R.BatchRegistration()
         ref.set_clouds(srcs, tgts)
@@ -141,7 +189,7 @@ def main():
 
     weights, meta = learned.load_weights()
     lr = learned.LearnedRegistration(weights, params, n_starts=args.starts, hypotheses=args.hypotheses,
-                                     max_pairs=n)
+                                     max_pairs=n, refine_anderson=args.refine_anderson)
     lr.set_clouds(srcs, tgts, cs, ct)
     ms = None
     if args.leg == "both":
@@ -162,6 +210,7 @@ def main():
         # the refine launch (one problem per pair) is tuned on its own launch
         _, _, props = lr.prop.propose_multi(lr.rots, lr.tau)
         lr.reg.set_problems(range(n), range(n))
+        lr.use_params("refine")
         lr.reg.set_rebuild(True)           # candidates are timed with their grid build
         tuned_refine, tr_ms, _ = autotune.tune(lr.reg, props, candidates=autotune.LATENCY_CANDIDATES, key=None,
                                                use_cache=False)
@@ -181,12 +230,20 @@ def main():
     def n_ok(results):
         return np.array([synth.rmse_between(results[i]["T"], T_ref[i], srcs[i]) <= OK_RMSE for i in range(n)])
 
+    if args.leg == "learned":
+        # learned steps alone (kernel profiling of the whole step)
+        t_l, res_l = timed(lr.align, args.steps, args.warmup)
+        log(f"learned steps done: {t_l * 1e3:.1f} ms/step, {int(n_ok(res_l).sum())}/{n} within 1 mm, "
+            f"{lr.stats['n_fallback']} fallbacks")
+        return
+
     if args.leg == "refine":
         # the learned path's refine launch alone, repeated (kernel profiling):
         # graph-replayed launches, then HIP-event-timed ones, then the copy
         # probe that calibrates the profiler's byte counters
         _, _, props = lr.prop.propose_multi(lr.rots, lr.tau)
         lr.reg.set_problems(range(n), range(n))
+        lr.use_params("refine")
         lr.reg.set_rebuild(True)
         for _ in range(args.warmup + args.steps):
             res = lr.reg.align(props)
@@ -215,36 +272,15 @@ def main():
 
     # dominant kernel: k_icp of the learned path's refine launch, HIP events on the engine's stream
     lr.reg.set_problems(range(n), range(n))
+    lr.use_params("refine")
     lr.reg.set_rebuild(True)               # the grid build is timed with the launch here
-    lr.reg.set_profiling(True)
-    icp_ms, grid_ms = [], []
-    for _ in range(3):
-        res = lr.reg.align(st["proposals"])
-        t = lr.reg.timing()
-        icp_ms.append(t["ms_icp"])
-        grid_ms.append(t["ms_grid"])
-    lr.reg.set_profiling(False)
-    iters = np.array([r["iterations"] for r in res], np.int64)
-    ns = np.array([len(s) for s in srcs], np.int64)
-    nt = np.array([len(t) for t in tgts], np.int64)
-    b_min_total = float(np.sum(iters * (12 * ns + 12 * nt)))
-    rng = np.random.default_rng(0)
-    sample = rng.choice(n, min(args.account_sample, n), replace=False)
-    s_iter = s_min = 0
-    for p in sample:
-        counts = accounting.TargetCounts(tgts[p], R_MAX)
-        hist = lr.reg.history(int(p), MAX_ITER)
-        for it in range(res[p]["iterations"]):
-            bi, bm = accounting.iteration_bytes(srcs[p], counts, hist[it])
-            s_iter += bi
-            s_min += bm
-    ratio = s_iter / s_min if s_min else 1.0
-    b_iter_total = b_min_total * ratio
-    t_icp = float(np.median(icp_ms)) / 1e3
-    achieved = b_iter_total / t_icp / 1e9
+    roof = launch_roofline(lr.reg, st["proposals"], np.arange(n), srcs, tgts, args.account_sample)
     copy_ms = ctypes.c_float()
     L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 5, ctypes.byref(copy_ms)))
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms.value / 1e3) / 1e9
+    # the plain multi-start launch's search kernel on the same accounting
+    roof_plain = launch_roofline(ms.reg, ms.guesses, np.repeat(np.arange(n), args.starts), srcs, tgts,
+                                 args.account_sample)
 
     traffic, traffic_src = None, None
     prof = os.path.join(ROOT, "profiles", "k_icp_learned_refine_256.json")
@@ -272,6 +308,7 @@ def main():
                                f"rotation U[0,180deg], t U[-0.1,0.1] m), r_max {R_MAX} m, <={MAX_ITER} iterations, "
                                f"eps {EPS}; counted when within {OK_RMSE * 1e3:.0f} mm RMSE of the reference pose",
                    "pairs": n, "hypotheses": args.hypotheses, "starts": args.starts,
+                   "refine_anderson_memory": args.refine_anderson,
                    "tuning": list(tuned) if tuned else None,
                    "tuning_refine": list(tuned_refine) if tuned_refine else None,
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],
@@ -285,15 +322,16 @@ def main():
                    "plain_iterations_total": ms.iterations_total,
                    "speedup_vs_plain": float(value / plain_value) if plain_value > 0 else None,
                    "weights": {k: meta[k] for k in ("base_pairs", "pairs_seen", "steps")}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+        "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": roof["achieved"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_icp (persistent correspondence search + moments + FP64 solve), learned-path "
-                               "refine launch (256 problems)",
-                     "kernel_ms": t_icp * 1e3, "grid_build_ms": float(np.median(grid_ms)),
-                     "algorithmic_bytes": b_iter_total, "compulsory_bytes": b_min_total,
-                     "achieved_compulsory_gbs": b_min_total / t_icp / 1e9, "b_iter_over_b_min": ratio,
-                     "iterations_total": int(iters.sum()), "b_iter_sample_problems": int(len(sample)),
+                               f"refine launch ({n} problems)",
+                     **{k: v for k, v in roof.items() if k != "achieved"},
                      "hbm_copy_gbs": copy_gbs, "copy_probe_bytes": args.copy_bytes},
+        "roofline_plain": {"bound": "hbm", "achieved": roof_plain["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": roof_plain["achieved"] / HBM_PEAK_GBS,
+                           "kernel": f"k_icp, plain multi-start launch ({n * args.starts} problems)",
+                           **{k: v for k, v in roof_plain.items() if k != "achieved"}},
     }
     if not args.no_cpu_baseline:
         starts = [multistart.so3_start_poses(cs[i], ct[i], args.starts) for i in range(n)]

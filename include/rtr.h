@@ -62,7 +62,11 @@ typedef struct {
   int max_iterations;
   int min_correspondences;
   int mode;                           /* RTR_MODE_* */
-  int reserved0;
+  int anderson_memory;                /* 0 (default): PCL's plain iteration.  1..8: Anderson
+                                         acceleration with that memory (no PCL counterpart): the
+                                         iterate is extrapolated from the last m + 1 ICP steps;
+                                         convergence is judged on the plain ICP increment (the
+                                         transformation criterion) and the iteration limit only */
   double max_correspondence_distance; /* metres */
   double transformation_epsilon;      /* squared translation threshold; rotation 1-eps */
   double rotation_epsilon;            /* cos threshold; <= 0 means 1 - transformation_epsilon */
@@ -344,6 +348,16 @@ int rtr_selfcheck_nn_stats(const float* tgt, size_t nt, const float* q, size_t n
  * a[i] (transformed source), b[i] (target), nrm (nullable) -> Tinc. */
 int rtr_selfcheck_solve(int mode, const float* a, const float* b, const float* nrm, size_t n,
                         const float anchor[3], double Tinc[16]);
+/* The Anderson mixing of anderson_memory (no reference counterpart) on the
+ * affine map g(x) = A x + c in R^6 (A row-major), from x = 0, with memory m:
+ * iterates until max|g(x) - x| <= tol or max_iter mixes; x_out is the last
+ * g(x), *iters the number of map evaluations. */
+int rtr_selfcheck_anderson(int m, const double A[36], const double c[6], double tol, int max_iter,
+                           double x_out[6], int* iters);
+/* Rotation chart of the acceleration: v = vector part of a unit quaternion
+ * with positive scalar part -> row-major R, then back to v_back.  Invalid
+ * argument when |v| > 1 or the angle exceeds about 174 degrees. */
+int rtr_selfcheck_so3(const double v[3], double R[9], double v_back[3]);
 
 #ifdef __cplusplus
 }

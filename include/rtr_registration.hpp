@@ -178,6 +178,12 @@ class IterativeClosestPoint {
     p_.euclidean_fitness_epsilon = e;
     check(rtr_icp_set_params(h_, &p_));
   }
+  // Anderson acceleration of the iteration, memory 1..8 (0 = PCL's plain
+  // iteration; no PCL counterpart)
+  void setAndersonMemory(int m) {
+    p_.anderson_memory = m;
+    check(rtr_icp_set_params(h_, &p_));
+  }
   // point-to-plane (pcl::IterativeClosestPointWithNormals) on the same handle
   void setUsePointToPlane(bool on) {
     p_.mode = on ? RTR_MODE_POINT_TO_PLANE : RTR_MODE_POINT_TO_POINT;

@@ -29,7 +29,7 @@ class RtrError(RuntimeError):
 
 class Params(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("min_correspondences", C.c_int), ("mode", C.c_int),
-                ("reserved0", C.c_int), ("max_correspondence_distance", C.c_double),
+                ("anderson_memory", C.c_int), ("max_correspondence_distance", C.c_double),
                 ("transformation_epsilon", C.c_double), ("rotation_epsilon", C.c_double),
                 ("euclidean_fitness_epsilon", C.c_double), ("cell_size", C.c_float),
                 ("cells_per_point", C.c_float)]
@@ -133,6 +133,8 @@ _SIGS = {
     "rtr_selfcheck_nn": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp]),
     "rtr_selfcheck_nn_stats": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp, _vp, _vp]),
     "rtr_selfcheck_solve": (_i, [_i, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "rtr_selfcheck_anderson": (_i, [_i, _vp, _vp, C.c_double, _i, _vp, _vp]),
+    "rtr_selfcheck_so3": (_i, [_vp, _vp, _vp]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

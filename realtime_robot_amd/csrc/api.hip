@@ -631,4 +631,57 @@ int rtr_selfcheck_solve(int mode, const float* a, const float* b, const float* n
   RTR_GUARD_END
 }
 
+int rtr_selfcheck_anderson(int m, const double A[36], const double c[6], double tol, int max_iter,
+                           double x_out[6], int* iters) {
+  RTR_GUARD_BEGIN
+  using namespace rtr;
+  if (!A || !c || !x_out || !iters || m < 1 || m > kAndersonMax || max_iter < 1) return RTR_ERR_INVALID_ARG;
+  // history of (x_j, g_j), oldest first, as anderson_pose_step keeps it
+  double H[12 * (kAndersonMax + 1)], ws[kAndersonWs], x[6] = {0, 0, 0, 0, 0, 0};
+  int k = 0, n_eval = 0;
+  for (int it = 0; it < max_iter; ++it) {
+    double g[6];
+    for (int r = 0; r < 6; ++r) {
+      double s = c[r];
+      for (int q = 0; q < 6; ++q) s += A[r * 6 + q] * x[q];
+      g[r] = s;
+    }
+    ++n_eval;
+    double res = 0.0;
+    for (int r = 0; r < 6; ++r) res = fmax(res, fabs(g[r] - x[r]));
+    for (int r = 0; r < 6; ++r) x_out[r] = g[r];
+    if (res <= tol) break;
+    if (k == m + 1) {
+      for (int i = 0; i < 12 * m; ++i) H[i] = H[i + 12];
+      k = m;
+    }
+    for (int r = 0; r < 6; ++r) {
+      H[12 * k + r] = x[r];
+      H[12 * k + 6 + r] = g[r];
+    }
+    ++k;
+    double xn[6];
+    if (k >= 2 && anderson_mix(H, k, xn, ws)) {
+      for (int r = 0; r < 6; ++r) x[r] = xn[r];
+    } else {
+      for (int r = 0; r < 6; ++r) x[r] = g[r];
+    }
+  }
+  *iters = n_eval;
+  return RTR_OK;
+  RTR_GUARD_END
+}
+
+int rtr_selfcheck_so3(const double v[3], double R[9], double v_back[3]) {
+  RTR_GUARD_BEGIN
+  using namespace rtr;
+  if (!v || !R || !v_back) return RTR_ERR_INVALID_ARG;
+  if (!qv_to_so3(v, R) || !so3_to_qv(R, v_back)) {
+    set_error("rotation outside the chart (|v| > 1 or angle beyond ~174 degrees)");
+    return RTR_ERR_INVALID_ARG;
+  }
+  return RTR_OK;
+  RTR_GUARD_END
+}
+
 }  // extern "C"

@@ -76,7 +76,7 @@ Engine::~Engine() {
                     &cell_start_, &cell_id_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
                     &bin_count_, &bin_start_, &bin_id_, &src_desc_, &prob_desc_, &state_,
                     &guesses_, &items_buf_, &partials_, &done_, &counters_, &scan_tmp_,
-                    &out_i32_, &out_f32_, &out_T_, &hist_};
+                    &out_i32_, &out_f32_, &out_T_, &hist_, &aa_};
   for (DevBuf* b : bufs) b->release();
   if (pin_) (void)hipHostFree(pin_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -222,13 +222,18 @@ int Engine::set_problems(int n, const int* src_id, const int* tgt_id) {
 
 int Engine::set_params(const rtr_params& p) {
   if (p.max_iterations < 1 || p.max_iterations > 100000 || !(p.max_correspondence_distance > 0) ||
-      (p.mode != RTR_MODE_POINT_TO_POINT && p.mode != RTR_MODE_POINT_TO_PLANE)) {
-    set_error("invalid parameters (max_iterations %d, max distance %g, mode %d)", p.max_iterations,
-              p.max_correspondence_distance, p.mode);
+      (p.mode != RTR_MODE_POINT_TO_POINT && p.mode != RTR_MODE_POINT_TO_PLANE) || p.anderson_memory < 0 ||
+      p.anderson_memory > kAndersonMax) {
+    set_error("invalid parameters (max_iterations %d, max distance %g, mode %d, anderson_memory %d)",
+              p.max_iterations, p.max_correspondence_distance, p.mode, p.anderson_memory);
     return RTR_ERR_INVALID_ARG;
   }
+  // the grids depend on the correspondence distance, the cell settings and
+  // the mode (normals); the iteration settings alone leave them valid
+  if (p.max_correspondence_distance != prm_.max_correspondence_distance || p.cell_size != prm_.cell_size ||
+      p.cells_per_point != prm_.cells_per_point || p.mode != prm_.mode)
+    grid_dirty_ = true;
   prm_ = p;
-  grid_dirty_ = true;
   graph_dirty_ = true;
   return upload_problems();
 }
@@ -345,6 +350,7 @@ int Engine::layout_problems() {
   RTR_TRY(counters_.ensure(counter_words() * 4));
   RTR_TRY(out_T_.ensure(16 * sizeof(double)));
   RTR_TRY(hist_.ensure(np * (size_t)(prm_.max_iterations + 1) * 12 * sizeof(float)));
+  if (prm_.anderson_memory > 0) RTR_TRY(aa_.ensure(np * (size_t)kAndersonStride * sizeof(double)));
   if (np) RTR_HIP(hipMemcpy(prob_desc_.p, probs_.data(), np * sizeof(Problem), hipMemcpyHostToDevice));
   if (!items_.empty())
     RTR_HIP(hipMemcpy(items_buf_.p, items_.data(), items_.size() * sizeof(uint2), hipMemcpyHostToDevice));
@@ -398,6 +404,7 @@ DevEngine Engine::dev_view() const {
   e.st = state_.as<ProblemState>();
   e.guesses = guesses_.as<double>();
   e.hist = hist_.as<float>();
+  e.aa = prm_.anderson_memory > 0 ? aa_.as<double>() : nullptr;
   e.items = items_buf_.as<uint2>();
   e.n_items = (int)items_.size();
   e.item_pts = tune_.item_pts;
@@ -427,6 +434,7 @@ DevEngine Engine::dev_view() const {
   e.prm.eps_T = prm_.transformation_epsilon;
   e.prm.eps_rot = prm_.rotation_epsilon;
   e.prm.eps_fit = prm_.euclidean_fitness_epsilon;
+  e.prm.anderson = prm_.anderson_memory;
   return e;
 }
 

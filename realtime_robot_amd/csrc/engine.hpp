@@ -129,7 +129,7 @@ class Engine {
   DevBuf tgt_desc_;
   DevBuf src_raw_, sx_, sy_, sz_, perm_, bin_count_, bin_start_, bin_id_, src_desc_;
   DevBuf prob_desc_, state_, guesses_, items_buf_, partials_, done_, counters_, scan_tmp_;
-  DevBuf out_i32_, out_f32_, out_T_, hist_;
+  DevBuf out_i32_, out_f32_, out_T_, hist_, aa_;
   // pinned host staging
   void* pin_ = nullptr;
   size_t pin_bytes_ = 0;

@@ -45,6 +45,7 @@ struct DevEngine {
   ProblemState* st;
   const double* guesses;     // n_prob * 16 (column-major)
   float* hist;               // n_prob * (max_iter + 1) * 12: float rows of T used by each iteration
+  double* aa;                // n_prob * kAndersonStride: acceleration state (null when off)
   const uint2* items;        // (problem, first sorted source point)
   int n_items;
   int item_pts;              // points per item

@@ -648,11 +648,265 @@ RTR_HD int convergence(const IcpParams& p, int iterations, const double* Tinc, d
   const double cos_angle = 0.5 * (Tinc[0] + Tinc[5] + Tinc[10] - 1.0);
   const double tsq = Tinc[12] * Tinc[12] + Tinc[13] * Tinc[13] + Tinc[14] * Tinc[14];
   if (cos_angle >= rot_thr && tsq <= p.eps_T) return RTR_CONV_TRANSFORM;
+  // With Anderson acceleration the iterates are extrapolated, and two of them
+  // can share an MSE far from the fixed point: only the increment decides.
+  if (p.anderson > 0) {
+    *prev_mse = mse;
+    return RTR_NOT_CONVERGED;
+  }
   const double dm = fabs(mse - *prev_mse);
   if (dm < 1e-12) return RTR_CONV_ABS_MSE;
   if (dm / *prev_mse < p.eps_fit) return RTR_CONV_REL_MSE;
   *prev_mse = mse;
   return RTR_NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------------------
+// Anderson acceleration of the ICP iteration (opt-in; with memory 0 the
+// iteration is PCL's plain one).  ICP is a fixed-point map T -> G(T) = dT(T) T
+// that converges linearly, slowly along weakly constrained directions.
+// Type-II Anderson mixing (Walker & Ni, SIAM J. Numer. Anal. 49(4), 2011)
+// extrapolates from the last m + 1 pairs (x_j, g_j = G(x_j)):
+//   gamma = argmin || f_k - dF gamma ||,  f_j = g_j - x_j,  dF_j = f_(j+1) - f_j
+//   x_(k+1) = g_k - dG gamma,             dG_j = g_(j+1) - g_j
+// on a 6-vector x = (v, t) of a pose: v charts the rotation relative to a base
+// rotation R_b (R R_b^T, below), t is the translation.  Safeguard as in AA-ICP (Pavlov,
+// Ovchinnikov, Shipulin, Parfenov, ICRA 2018): when the objective at an
+// iterate is not below the last accepted one's, the next iterate is the plain
+// step from that accepted iterate and the history restarts; a mixed step
+// under a tenth of the plain one (stagnation) takes the plain step and
+// restarts; one beyond ten plain steps is shortened to ten.  Convergence is judged on the plain ICP increment only (the
+// transformation criterion and the iteration limit; the MSE criteria are
+// off, see convergence()), so the final pose is the plain step from the last
+// iterate.
+// ---------------------------------------------------------------------------
+// kAndersonMax / kAndersonStride are in rtr_types.hpp.  State layout:
+// [0..8] R_b (row-major), [9] pairs held, [10] best MSE, [12 + 12 j ..] x_j (6)
+// then g_j (6), oldest first.  The update runs on one lane; every array it
+// touches is passed in (LDS on the device), and its loops are kept rolled, so
+// it adds neither private memory nor register pressure to the kernel.
+constexpr int kAndersonWs = 120;  // doubles of workspace of anderson_pose_step
+
+// Rotations are charted by the vector part v of their unit quaternion with
+// positive scalar part (v = sin(th/2) * axis): matrix -> v needs only a square
+// root and v -> matrix is polynomial, so the update calls no transcendental
+// function.  so3_to_qv fails beyond about 174 degrees (scalar part below
+// 0.05), where the chart is ill-conditioned; the step is then not accelerated.
+RTR_HD bool so3_to_qv(const double* R, double* v) {
+  const double s = 1.0 + ((R[0] + R[4]) + R[8]);  // 4 w^2
+  if (!(s > 0.01)) return false;
+  const double w = 0.5 * sqrt(s);
+  const double f = 0.25 / w;
+  v[0] = (R[7] - R[5]) * f;
+  v[1] = (R[2] - R[6]) * f;
+  v[2] = (R[3] - R[1]) * f;
+  return true;
+}
+
+// Row-major rotation of the unit quaternion (sqrt(1 - |v|^2), v); false when
+// |v| > 1.
+RTR_HD bool qv_to_so3(const double* v, double* R) {
+  const double x = v[0], y = v[1], z = v[2];
+  const double n2 = (x * x + y * y) + z * z;
+  if (!(n2 <= 1.0)) return false;
+  const double w = sqrt(1.0 - n2);
+  R[0] = 1.0 - 2.0 * (y * y + z * z);
+  R[1] = 2.0 * (x * y - w * z);
+  R[2] = 2.0 * (x * z + w * y);
+  R[3] = 2.0 * (x * y + w * z);
+  R[4] = 1.0 - 2.0 * (x * x + z * z);
+  R[5] = 2.0 * (y * z - w * x);
+  R[6] = 2.0 * (x * z - w * y);
+  R[7] = 2.0 * (y * z + w * x);
+  R[8] = 1.0 - 2.0 * (x * x + y * y);
+  return true;
+}
+
+// Mixing step over k >= 2 pairs H[12 j .. 12 j + 11] = (x_j, g_j), oldest
+// first.  ws: 72 doubles.  False when the least-squares system is singular.
+RTR_HD bool anderson_mix(const double* H, int k, double* x_new, double* ws) {
+  const int n = k - 1;
+  double* A = ws;       // n x n
+  double* b = ws + 64;  // n
+#pragma unroll 1
+  for (int r = 0; r < n; ++r) {
+#pragma unroll 1
+    for (int c = r; c < n; ++c) {
+      double s = 0.0;
+#pragma unroll 1
+      for (int i = 0; i < 6; ++i) {
+        const double dr = (H[12 * (r + 1) + 6 + i] - H[12 * (r + 1) + i]) - (H[12 * r + 6 + i] - H[12 * r + i]);
+        const double dc = (H[12 * (c + 1) + 6 + i] - H[12 * (c + 1) + i]) - (H[12 * c + 6 + i] - H[12 * c + i]);
+        s += dr * dc;
+      }
+      A[r * n + c] = s;
+      A[c * n + r] = s;
+    }
+    double s = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < 6; ++i) {
+      const double dr = (H[12 * (r + 1) + 6 + i] - H[12 * (r + 1) + i]) - (H[12 * r + 6 + i] - H[12 * r + i]);
+      s += dr * (H[12 * (k - 1) + 6 + i] - H[12 * (k - 1) + i]);
+    }
+    b[r] = s;
+  }
+  // Tikhonov term relative to the diagonal keeps near-collinear histories solvable
+  double tr = 0.0;
+#pragma unroll 1
+  for (int r = 0; r < n; ++r) tr += A[r * n + r];
+  if (!(tr > 0.0)) return false;
+#pragma unroll 1
+  for (int r = 0; r < n; ++r) A[r * n + r] += 1e-10 * tr / n;
+  // Cholesky in place (lower), then the two triangular solves into b
+#pragma unroll 1
+  for (int j = 0; j < n; ++j) {
+    double s = A[j * n + j];
+#pragma unroll 1
+    for (int q = 0; q < j; ++q) s -= A[j * n + q] * A[j * n + q];
+    if (!(s > 0.0)) return false;
+    const double d = sqrt(s);
+    A[j * n + j] = d;
+#pragma unroll 1
+    for (int r = j + 1; r < n; ++r) {
+      double t = A[r * n + j];
+#pragma unroll 1
+      for (int q = 0; q < j; ++q) t -= A[r * n + q] * A[j * n + q];
+      A[r * n + j] = t / d;
+    }
+  }
+#pragma unroll 1
+  for (int r = 0; r < n; ++r) {
+    double s = b[r];
+#pragma unroll 1
+    for (int q = 0; q < r; ++q) s -= A[r * n + q] * b[q];
+    b[r] = s / A[r * n + r];
+  }
+#pragma unroll 1
+  for (int r = n - 1; r >= 0; --r) {
+    double s = b[r];
+#pragma unroll 1
+    for (int q = r + 1; q < n; ++q) s -= A[q * n + r] * b[q];
+    b[r] = s / A[r * n + r];
+  }
+#pragma unroll 1
+  for (int i = 0; i < 6; ++i) {
+    double s = H[12 * (k - 1) + 6 + i];
+#pragma unroll 1
+    for (int j = 0; j < n; ++j) s -= (H[12 * (j + 1) + 6 + i] - H[12 * j + 6 + i]) * b[j];
+    if (!(fabs(s) < 1e300)) return false;  // NaN or overflow
+    x_new[i] = s;
+  }
+  return true;
+}
+
+// Pose (column-major 4x4) -> x relative to the row-major base rotation Rb.
+// M: 9 doubles of workspace.
+RTR_HD bool pose_to_x(const double* T, const double* Rb, double* x, double* M) {
+#pragma unroll 1
+  for (int i = 0; i < 9; ++i) {
+    const int r = i / 3, c = i % 3;  // (R Rb^T)[r][c]
+    M[i] = (T[r] * Rb[c * 3] + T[4 + r] * Rb[c * 3 + 1]) + T[8 + r] * Rb[c * 3 + 2];
+  }
+  if (!so3_to_qv(M, x)) return false;
+  x[3] = T[12];
+  x[4] = T[13];
+  x[5] = T[14];
+  return true;
+}
+
+// x relative to Rb -> pose (column-major); false outside the chart.  E: 9
+// doubles of workspace.
+RTR_HD bool x_to_pose(const double* x, const double* Rb, double* T, double* E) {
+  if (!qv_to_so3(x, E)) return false;
+#pragma unroll 1
+  for (int i = 0; i < 9; ++i) {
+    const int r = i / 3, c = i % 3;  // (E Rb)[r][c]
+    T[c * 4 + r] = (E[r * 3] * Rb[c] + E[r * 3 + 1] * Rb[3 + c]) + E[r * 3 + 2] * Rb[6 + c];
+  }
+  T[3] = T[7] = T[11] = 0.0;
+  T[12] = x[3];
+  T[13] = x[4];
+  T[14] = x[5];
+  T[15] = 1.0;
+  return true;
+}
+
+// One accelerated update.  st: the problem's state (layout above, set up by
+// anderson_reset); Tq: the transform this iteration queried with, mse the
+// objective there (the caller's choice: non-increasing along plain steps);
+// Tg: the plain ICP step from Tq.  Writes the next
+// iterate to Tout (Tg itself when the step is not accelerated).  ws:
+// kAndersonWs doubles.  Returns 1 when the step was accelerated.
+RTR_HD int anderson_pose_step(double* st, int m, const double* Tq, const double* Tg, double mse, double* Tout,
+                              double* ws) {
+  double* x = ws + 80;  // 6
+  double* M = ws + 88;  // 9
+#pragma unroll 1
+  for (int i = 0; i < 16; ++i) Tout[i] = Tg[i];
+  if (m < 1) return 0;
+  if (m > kAndersonMax) m = kAndersonMax;
+  int k = (int)st[9];
+  if (k > 0 && !(mse < st[10])) {
+    // the extrapolated iterate is worse than the last accepted one: go back
+    // to the plain step from that one (the last pair's g) and start a new
+    // history there
+    st[9] = 0.0;
+    st[10] = 1.7976931348623157e308;
+    if (x_to_pose(st + 12 + 12 * (k - 1) + 6, st, ws + 100, M)) {
+#pragma unroll 1
+      for (int i = 0; i < 16; ++i) Tout[i] = ws[100 + i];
+      return 0;
+    }
+    k = 0;
+  }
+  if (k == 0) {
+#pragma unroll 1
+    for (int i = 0; i < 9; ++i) st[i] = Tq[(i % 3) * 4 + i / 3];
+  }
+  if (k == m + 1) {  // drop the oldest pair
+#pragma unroll 1
+    for (int i = 12; i < 12 + 12 * m; ++i) st[i] = st[i + 12];
+    k = m;
+  }
+  double* pr = st + 12 + 12 * k;
+  if (!pose_to_x(Tq, st, pr, M) || !pose_to_x(Tg, st, pr + 6, M)) {
+    st[9] = 0.0;
+    st[10] = mse;
+    return 0;
+  }
+  k += 1;
+  st[9] = (double)k;
+  st[10] = mse;  // after a restart the first MSE, else an improvement
+  if (k < 2) return 0;
+  if (!anderson_mix(st + 12, k, x, ws)) return 0;
+  // stagnation: a mixed step far shorter than the plain one (noise in the
+  // map dominates the history) would stall the iteration; take the plain
+  // step and start a new history from it
+  double da = 0.0, dp = 0.0;
+#pragma unroll 1
+  for (int i = 0; i < 6; ++i) {
+    da += (x[i] - pr[i]) * (x[i] - pr[i]);
+    dp += (pr[6 + i] - pr[i]) * (pr[6 + i] - pr[i]);
+  }
+  if (da < 0.01 * dp) {
+    st[9] = 0.0;
+    return 0;
+  }
+  // trust region: an extrapolation beyond ten plain steps is shortened to ten
+  if (da > 100.0 * dp) {
+    const double f = 10.0 * sqrt(dp / da);
+#pragma unroll 1
+    for (int i = 0; i < 6; ++i) x[i] = pr[i] + f * (x[i] - pr[i]);
+  }
+  if (!x_to_pose(x, st, ws + 100, M)) return 0;  // outside the chart: plain step
+#pragma unroll 1
+  for (int i = 0; i < 16; ++i) Tout[i] = ws[100 + i];
+  return 1;
+}
+
+RTR_HD void anderson_reset(double* st) {
+  st[9] = 0.0;
+  st[10] = 1.7976931348623157e308;  // DBL_MAX
 }
 
 // Per-correspondence contributions, relative to the anchor (float).

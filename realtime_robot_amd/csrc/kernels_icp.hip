@@ -78,6 +78,7 @@ __global__ void k_icp_init(DevEngine e) {
   S.finished = 0;
   e.done[p] = 0;
   if (e.hist) tf_rows_f32(S.T, e.hist + (size_t)p * (e.prm.max_iter + 1) * 12);
+  if (e.aa) anderson_reset(e.aa + (size_t)p * kAndersonStride);
   uint32_t* word = (uint32_t*)&S.gen;
   if (e.probs[p].n_items == 0) {
     S.state = RTR_FAIL_NO_CORRESPONDENCES;
@@ -127,15 +128,19 @@ struct IcpShared {
   TileT tile[kWaves];
   float red[kWaves][NS][2];
   double dred[kBlock / NSUM_MAX][NSUM_MAX];
-  double ws[64];
+  double ws[kAndersonWs];
   double Tinc[16];
   double Tn[16];
+  double Ta[16];
+  double Tq[16];
+  double aa[kAndersonStride];
   float m[12];
   float anchor[3];
   int item;
   int run;
   int stop;
   int last;
+  int fin;
 };
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -246,6 +251,12 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
                               const TargetDesc& td, int pass, IcpShared<NS, TileT>& sh) {
   const int tid = threadIdx.x;
   const int v = tid % NSUM_MAX, grp = tid / NSUM_MAX;
+  // the pose and the acceleration state are staged in LDS by the whole block
+  // (one load per lane), so the single-thread FP64 work below reads no
+  // global memory
+  double* gaa = e.aa ? e.aa + (size_t)p * kAndersonStride : nullptr;
+  if (tid < 16) sh.Tq[tid] = e.st[p].T[tid];
+  if (gaa && tid < kAndersonStride) sh.aa[tid] = gaa[tid];
   double acc = 0.0;
   if (v < NS) {
     for (uint32_t i = grp; i < pr.n_items; i += kBlock / NSUM_MAX) {
@@ -276,20 +287,40 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
     } else {
       double mse;
       solve_increment(MODE, vals, td.anchor, sh.Tinc, &mse, sh.ws);
-      mat4_mul_to(sh.Tinc, S.T, sh.Tn);
-      for (int k = 0; k < 16; ++k) S.T[k] = sh.Tn[k];
+      mat4_mul_to(sh.Tinc, sh.Tq, sh.Tn);
       S.iter += 1;
-      if (e.hist) tf_rows_f32(sh.Tn, e.hist + ((size_t)p * (e.prm.max_iter + 1) + S.iter) * 12);
       S.mse = mse;
       double prev = S.prev_mse;
       const int stt = convergence(e.prm, S.iter, sh.Tinc, mse, &prev);
       S.prev_mse = prev;
       S.state = stt;
       fin = (stt != RTR_NOT_CONVERGED);
+      const double* Tnext = sh.Tn;
+      if (!fin && gaa) {
+        // the next iterate is extrapolated from the recent ICP steps; iterates
+        // are compared by the truncated energy (points without a
+        // correspondence count r_max^2), which the plain iteration never raises
+        const double N = (double)e.srcs[pr.src].n;
+        const double energy = (vals[SUM_D2] + (N - n) * (double)e.prm.r2f) / N;
+        anderson_pose_step(sh.aa, e.prm.anderson, sh.Tq, sh.Tn, energy, sh.Ta, sh.ws);
+        Tnext = sh.Ta;
+      }
+      for (int k = 0; k < 16; ++k) S.T[k] = Tnext[k];
+      if (e.hist) tf_rows_f32(Tnext, e.hist + ((size_t)p * (e.prm.max_iter + 1) + S.iter) * 12);
     }
     if (fin) S.finished = 1;
+    sh.fin = fin ? 1 : 0;
+  }
+  __syncthreads();
+  if (gaa) {
+    if (tid < kAndersonStride) gaa[tid] = sh.aa[tid];
     __threadfence();
-    store_release((uint32_t*)&S.gen, (uint32_t)(pass + 1) | (fin ? kFinishedBit : 0u));
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const bool fin = sh.fin != 0;
+    __threadfence();
+    store_release((uint32_t*)&e.st[p].gen, (uint32_t)(pass + 1) | (fin ? kFinishedBit : 0u));
     if (fin) atomicAdd(e.problems_done, 1u);
   }
 }

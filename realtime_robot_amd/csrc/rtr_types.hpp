@@ -33,6 +33,9 @@ struct GridView {
   float h, inv_h, margin;
 };
 
+constexpr int kAndersonMax = 8;       // largest Anderson acceleration memory m
+constexpr int kAndersonStride = 120;  // doubles of acceleration state per problem: 12 + 12 (m + 1)
+
 struct IcpParams {
   int max_iter;
   int min_corr;
@@ -41,6 +44,7 @@ struct IcpParams {
   double eps_T;      // transformation epsilon
   double eps_rot;    // rotation epsilon (<= 0: 1 - eps_T)
   double eps_fit;    // euclidean fitness epsilon
+  int anderson;      // Anderson acceleration memory (0: plain iteration)
 };
 
 // Per target cloud (device).  Grid fields are written by the grid build.

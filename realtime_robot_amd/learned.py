@@ -377,15 +377,23 @@ class LearnedRegistration:
     With hypotheses = k > 1 the network runs on the source samples rotated by
     k fixed orientations spread over SO(3) (the first the identity) and the
     candidate with the lowest truncated-chamfer score (rtr_prop_propose_multi)
-    is the proposal ICP refines."""
+    is the proposal ICP refines.
+
+    refine_anderson > 0 runs the refine launch with Anderson acceleration of
+    that memory (rtr_params.anderson_memory): the same fixed point in fewer
+    iterations.  The fallback is always the plain multi-start iteration."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 2048,
                  max_curvature: float = 0.02, hypotheses: int = 8, tau: float = 0.1, max_pairs: int = 256,
-                 device: int = 0):
+                 refine_anderson: int = 5, device: int = 0):
         if weights is None:
             weights, _ = load_weights()
-        self.params = params if params is not None else default_params()
+        self.params = L.Params.from_buffer_copy(params if params is not None else default_params())
+        self.params.anderson_memory = 0
+        self.refine_params = L.Params.from_buffer_copy(self.params)
+        self.refine_params.anderson_memory = refine_anderson
+        self._active = "plain"
         self.hypotheses = hypotheses
         self.rots = multistart.so3_rotations(hypotheses) if hypotheses > 1 else None
         self.tau = tau
@@ -417,6 +425,13 @@ class LearnedRegistration:
             self.reg.set_problems(ids, ids)
             self._problems = key
 
+    def use_params(self, which: str):
+        """Make the engine's parameters those of the 'refine' or the 'plain'
+        (fallback) launch."""
+        if self._active != which:
+            self.reg.set_params(self.refine_params if which == "refine" else self.params)
+            self._active = which
+
     def align(self):
         t0 = time.perf_counter()
         self.reg.build()
@@ -427,6 +442,7 @@ class LearnedRegistration:
             poses = self.prop.propose()
         t1 = time.perf_counter()
         self._set_problems("refine", self._ids)
+        self.use_params("refine")
         res = self.reg.align(poses)
         t2 = time.perf_counter()
         rmse, frac, pfrac = self.reg.judge(self.judge_k, self.judge_points, self.max_curvature)
@@ -435,6 +451,7 @@ class LearnedRegistration:
         if len(fail):
             ids = np.repeat(fail, self.n_starts)
             self._set_problems(("fallback", tuple(fail.tolist())), ids)
+            self.use_params("plain")
             g = np.stack([T for i in fail for T in multistart.so3_start_poses(self.cs[i], self.ct[i], self.n_starts)])
             res8 = self.reg.align(g)
             for k, i in enumerate(fail):

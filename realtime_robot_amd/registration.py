@@ -165,7 +165,9 @@ def result_dict(r: L.Result) -> dict:
 
 def make_params(max_iterations=10, max_correspondence_distance=float(np.sqrt(DBL_MAX)), transformation_epsilon=0.0,
                 euclidean_fitness_epsilon=-DBL_MAX, rotation_epsilon=0.0, mode=L.MODE_POINT_TO_POINT,
-                min_correspondences=3, cell_size=0.0, cells_per_point=0.0) -> L.Params:
+                min_correspondences=3, cell_size=0.0, cells_per_point=0.0, anderson_memory=0) -> L.Params:
+    """rtr_params from keyword arguments (PCL 1.8's defaults).  anderson_memory
+    > 0 turns on Anderson acceleration of the iteration (no PCL counterpart)."""
     p = L.default_params()
     p.max_iterations = max_iterations
     p.max_correspondence_distance = max_correspondence_distance
@@ -176,6 +178,7 @@ def make_params(max_iterations=10, max_correspondence_distance=float(np.sqrt(DBL
     p.min_correspondences = min_correspondences
     p.cell_size = cell_size
     p.cells_per_point = cells_per_point
+    p.anderson_memory = anderson_memory
     return p
 
 

@@ -178,3 +178,61 @@ def test_cpp_keypoint_icp_on_device(tmp_path):
     assert r.returncode == 0, r.stderr
     moved = read_xyz(str(out))
     assert np.abs(moved - read_xyz(golden("mcloud.pcd"))).max() < 1e-5
+
+
+def _affine_map(seed, radii):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((6, 6)))
+    A = Q @ np.diag(radii) @ Q.T
+    return np.ascontiguousarray(A), rng.standard_normal(6)
+
+
+@pytest.mark.parametrize("m", [6, 8])
+def test_anderson_mixing_terminates_on_affine_maps(m):
+    """Anderson type-II mixing with memory >= the dimension is equivalent to
+    GMRES on a linear fixed-point problem (Walker & Ni 2011), so on an affine
+    map of R^6 it reaches the fixed point (I - A)^-1 c after at most 6 + 1
+    mixing steps, whatever the contraction rate; the plain iteration needs
+    thousands of steps at spectral radius 0.99."""
+    A, c = _affine_map(11, [0.99, 0.97, 0.95, 0.9, 0.3, -0.5])
+    xs = np.linalg.solve(np.eye(6) - A, c)
+    x, it = np.empty(6), C.c_int()
+    L.check(L.lib.rtr_selfcheck_anderson(m, L.ptr(A), L.ptr(c), 1e-12, 5000, L.ptr(x), C.byref(it)))
+    assert it.value <= 12
+    assert np.abs(x - xs).max() < 1e-9
+    # small memory still helps, less
+    x1, it1 = np.empty(6), C.c_int()
+    L.check(L.lib.rtr_selfcheck_anderson(2, L.ptr(A), L.ptr(c), 1e-12, 5000, L.ptr(x1), C.byref(it1)))
+    assert it.value < it1.value < 2000
+    assert np.abs(x1 - xs).max() < 1e-8
+
+
+def test_anderson_mixing_rejects_bad_arguments():
+    A, c = _affine_map(3, [0.5] * 6)
+    x, it = np.empty(6), C.c_int()
+    for m in (0, 9):
+        assert L.lib.rtr_selfcheck_anderson(m, L.ptr(A), L.ptr(c), 1e-12, 10, L.ptr(x), C.byref(it)) != 0
+
+
+def test_rotation_chart_matches_quaternion_matrix():
+    """The acceleration charts rotations by the vector part v of a unit
+    quaternion with positive scalar part: R(v) is the textbook quaternion
+    matrix (scipy's), and the chart inverts it; angles beyond ~174 degrees
+    and |v| > 1 are refused."""
+    from scipy.spatial.transform import Rotation as Rot
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        q = rng.standard_normal(4)
+        q /= np.linalg.norm(q)
+        if q[3] < 0:
+            q = -q
+        if q[3] < 0.03:
+            continue
+        v = np.ascontiguousarray(q[:3])
+        Rm, vb = np.empty(9), np.empty(3)
+        L.check(L.lib.rtr_selfcheck_so3(L.ptr(v), L.ptr(Rm), L.ptr(vb)))
+        assert np.abs(Rm.reshape(3, 3) - Rot.from_quat(q).as_matrix()).max() < 1e-14
+        assert np.abs(vb - v).max() < 1e-12
+    Rm, vb = np.empty(9), np.empty(3)
+    for bad in ([0.8, 0.7, 0.0], [0.0, 0.0, 0.9995]):
+        assert L.lib.rtr_selfcheck_so3(L.ptr(np.array(bad)), L.ptr(Rm), L.ptr(vb)) != 0
