@@ -93,6 +93,24 @@ def fp64_check(srcs, tgts, gts, T_ref, inits, T_gpu, n_chk):
             "within_1mm": int(sum(x <= OK_RMSE for x in rb))}
 
 
+def prefer_profiled(reg, best, best_ms, table, name, tol=0.03):
+    """Among candidates the autotuner timed within `tol` of its fastest (a
+    statistical tie), prefer the tuning of the committed rocprofv3 summary of
+    this launch, so that its HBM byte counters apply to the reported launch."""
+    from realtime_robot_amd import autotune
+    prof = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(prof):
+        return best, best_ms
+    with open(prof) as f:
+        want = [round(float(v), 4) for v in json.load(f)["tuning"]]
+    for cfg, ms in table:
+        if [round(float(v), 4) for v in cfg] == want and ms <= best_ms * (1 + tol):
+            autotune.apply(reg, cfg)
+            log(f"autotune: {cfg} at {ms:.1f} ms ties the fastest {best} ({best_ms:.1f} ms); keeping the profiled one")
+            return tuple(cfg), ms
+    return best, best_ms
+
+
 def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample):
     """Algorithmic HBM bytes of one k_icp launch over its duration.
 
@@ -155,9 +173,9 @@ def main():
     ap.add_argument("--fp64-sample", type=int, default=8)
     ap.add_argument("--account-sample", type=int, default=24)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
-    ap.add_argument("--leg", choices=["both", "refine", "learned"], default="both",
-                    help="'refine': only the learned path's refine launches; 'learned': only learned steps "
-                         "(kernel profiles)")
+    ap.add_argument("--leg", choices=["both", "refine", "learned", "plain"], default="both",
+                    help="'refine': only the learned path's refine launches; 'learned': only learned steps; "
+                         "'plain': only the plain multi-start launches (kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): registrations are independent and "
@@ -192,7 +210,7 @@ def main():
                                      max_pairs=n, refine_anderson=args.refine_anderson)
     lr.set_clouds(srcs, tgts, cs, ct)
     ms = None
-    if args.leg == "both":
+    if args.leg in ("both", "plain"):
         ms = learned.MultiStartRegistration(params, n_starts=args.starts)
         ms.set_clouds(srcs, tgts, cs, ct)
     tuned = None
@@ -202,18 +220,37 @@ def main():
     elif not args.no_autotune and ms is not None:
         tuned, tuned_ms, table = autotune.tune(ms.reg, ms.guesses, candidates=autotune.LATENCY_CANDIDATES, key=None,
                                                use_cache=False)
+        tuned, tuned_ms = prefer_profiled(ms.reg, tuned, tuned_ms, table, "k_icp_plain_multistart_2048")
         log(f"autotune (plain multi-start launch): {tuned} ({tuned_ms:.1f} ms)")
     tuned_refine = tuned
     if tuned is not None and ms is not None:
         autotune.apply(ms.reg, tuned)
+    if args.leg == "plain":
+        # the plain multi-start launch alone, repeated (kernel profiling), then
+        # HIP-event-timed launches and the copy probe
+        for _ in range(args.warmup + args.steps):
+            res = ms.align()
+        ms.reg.set_profiling(True)
+        icp_ms = []
+        for _ in range(3):
+            ms.align()
+            icp_ms.append(ms.reg.timing()["ms_icp"])
+        ms.reg.set_profiling(False)
+        copy_ms = ctypes.c_float()
+        L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 3, ctypes.byref(copy_ms)))
+        log(f"plain launches done: k_icp HIP-event median {np.median(icp_ms):.3f} ms "
+            f"({ms.iterations_total} iterations); copy probe {copy_ms.value:.3f} ms")
+        return
+
     if not args.tuning and not args.no_autotune:
         # the refine launch (one problem per pair) is tuned on its own launch
         _, _, props = lr.prop.propose_multi(lr.rots, lr.tau)
         lr.reg.set_problems(range(n), range(n))
         lr.use_params("refine")
         lr.reg.set_rebuild(True)           # candidates are timed with their grid build
-        tuned_refine, tr_ms, _ = autotune.tune(lr.reg, props, candidates=autotune.LATENCY_CANDIDATES, key=None,
-                                               use_cache=False)
+        tuned_refine, tr_ms, table_r = autotune.tune(lr.reg, props, candidates=autotune.LATENCY_CANDIDATES, key=None,
+                                                     use_cache=False)
+        tuned_refine, tr_ms = prefer_profiled(lr.reg, tuned_refine, tr_ms, table_r, "k_icp_learned_refine_256")
         lr.reg.set_rebuild(False)
         log(f"autotune (learned refine launch): {tuned_refine} ({tr_ms:.1f} ms)")
     elif tuned is not None:
@@ -282,13 +319,20 @@ def main():
     roof_plain = launch_roofline(ms.reg, ms.guesses, np.repeat(np.arange(n), args.starts), srcs, tgts,
                                  args.account_sample)
 
-    traffic, traffic_src = None, None
-    prof = os.path.join(ROOT, "profiles", "k_icp_learned_refine_256.json")
-    if os.path.exists(prof) and tuned_refine is not None and n == 256:
+    def profiled_traffic(name, tuning):
+        """HBM bytes per launch from a committed rocprofv3 summary of the same
+        launch and tuning (FETCH_SIZE / WRITE_SIZE scaled by the copy probe)."""
+        prof = os.path.join(ROOT, "profiles", name + ".json")
+        if not (os.path.exists(prof) and tuning is not None and n == 256):
+            return None, None
         with open(prof) as f:
             pj = json.load(f)
-        if [round(v, 4) for v in pj["tuning"]] == [round(float(v), 4) for v in tuned_refine]:
-            traffic, traffic_src = float(pj["traffic_bytes"]), "profiles/k_icp_learned_refine_256.md"
+        same = [round(v, 4) for v in pj["tuning"]] == [round(float(v), 4) for v in tuning]
+        same = same and pj.get("refine_anderson_memory", 0) == (args.refine_anderson if "refine" in name else 0)
+        return (float(pj["traffic_bytes"]), f"profiles/{name}.md") if same else (None, None)
+
+    traffic, traffic_src = profiled_traffic("k_icp_learned_refine_256", tuned_refine)
+    traffic_plain, traffic_plain_src = profiled_traffic("k_icp_plain_multistart_2048", tuned)
 
     out = {
         "metric": METRIC,
@@ -329,7 +373,8 @@ def main():
                      **{k: v for k, v in roof.items() if k != "achieved"},
                      "hbm_copy_gbs": copy_gbs, "copy_probe_bytes": args.copy_bytes},
         "roofline_plain": {"bound": "hbm", "achieved": roof_plain["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": roof_plain["achieved"] / HBM_PEAK_GBS,
+                           "frac": roof_plain["achieved"] / HBM_PEAK_GBS, "traffic": traffic_plain,
+                           "traffic_source": traffic_plain_src,
                            "kernel": f"k_icp, plain multi-start launch ({n * args.starts} problems)",
                            **{k: v for k, v in roof_plain.items() if k != "achieved"}},
     }

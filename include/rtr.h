@@ -174,6 +174,9 @@ int rtr_batch_set_params(rtr_batch* b, const rtr_params* p);
  * (and <= 0 for the first three) keep the current setting. */
 int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point, int item_pts,
                          int blocks_per_cu, int use_tile);
+/* The current tuning (any pointer may be null). */
+int rtr_batch_get_tuning(rtr_batch* b, float* cell_factor, float* cells_per_point, int* item_pts,
+                         int* blocks_per_cu, int* use_tile);
 /* Grid policy.  always = 1 (default): every align rebuilds the target grids
  * and source orders inside its launch sequence (a registration pays for its
  * own search structure, as PCL's setInputTarget builds its kd-tree).

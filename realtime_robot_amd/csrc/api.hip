@@ -401,6 +401,8 @@ int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point,
     return RTR_ERR_INVALID_ARG;
   }
   rtr::TuneParams& t = b->eng.tune();
+  if ((cell_factor > 0 && cell_factor != t.cell_factor) || (cells_per_point > 0 && cells_per_point != t.cells_per_point))
+    b->eng.invalidate_grids();
   if (cell_factor > 0) t.cell_factor = cell_factor;
   if (cells_per_point > 0) t.cells_per_point = cells_per_point;
   if (item_pts > 0) t.item_pts = item_pts;
@@ -408,6 +410,18 @@ int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point,
   if (use_tile >= 0) t.use_tile = use_tile != 0;
   return b->eng.set_params(b->eng.params());
   RTR_GUARD_END
+}
+
+int rtr_batch_get_tuning(rtr_batch* b, float* cell_factor, float* cells_per_point, int* item_pts,
+                         int* blocks_per_cu, int* use_tile) {
+  if (!b) return RTR_ERR_INVALID_ARG;
+  const rtr::TuneParams& t = b->eng.tune();
+  if (cell_factor) *cell_factor = t.cell_factor;
+  if (cells_per_point) *cells_per_point = t.cells_per_point;
+  if (item_pts) *item_pts = t.item_pts;
+  if (blocks_per_cu) *blocks_per_cu = t.blocks_per_cu;
+  if (use_tile) *use_tile = t.use_tile ? 1 : 0;
+  return RTR_OK;
 }
 
 int rtr_batch_set_rebuild(rtr_batch* b, int always) {

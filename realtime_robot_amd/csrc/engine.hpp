@@ -67,6 +67,8 @@ class Engine {
   // clouds, parameters or tuning change, unless rebuild-always is on).
   int build();
   TuneParams& tune() { return tune_; }
+  // the next align (or build) rebuilds the grids: cell settings changed
+  void invalidate_grids() { grid_dirty_ = true; }
   void invalidate_graph() { graph_dirty_ = true; }
 
   int align(const double* guesses, rtr_result* results);

@@ -380,19 +380,24 @@ class LearnedRegistration:
     is the proposal ICP refines.
 
     refine_anderson > 0 runs the refine launch with Anderson acceleration of
-    that memory (rtr_params.anderson_memory): the same fixed point in fewer
-    iterations.  The fallback is always the plain multi-start iteration."""
+    that memory (rtr_params.anderson_memory): an equally good minimum in
+    fewer iterations.  The fallback is always the plain multi-start
+    iteration; its launch (a few pairs x n_starts problems, latency-bound)
+    uses work items of fallback_item_pts source points, the refine launch the
+    engine's tuned item size."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 2048,
                  max_curvature: float = 0.02, hypotheses: int = 8, tau: float = 0.1, max_pairs: int = 256,
-                 refine_anderson: int = 5, device: int = 0):
+                 refine_anderson: int = 5, fallback_item_pts: int = 256, device: int = 0):
         if weights is None:
             weights, _ = load_weights()
         self.params = L.Params.from_buffer_copy(params if params is not None else default_params())
         self.params.anderson_memory = 0
         self.refine_params = L.Params.from_buffer_copy(self.params)
         self.refine_params.anderson_memory = refine_anderson
+        self.fallback_item_pts = fallback_item_pts
+        self._refine_item_pts = None
         self._active = "plain"
         self.hypotheses = hypotheses
         self.rots = multistart.so3_rotations(hypotheses) if hypotheses > 1 else None
@@ -429,7 +434,15 @@ class LearnedRegistration:
         """Make the engine's parameters those of the 'refine' or the 'plain'
         (fallback) launch."""
         if self._active != which:
-            self.reg.set_params(self.refine_params if which == "refine" else self.params)
+            if which == "refine":
+                if self._refine_item_pts is not None:
+                    self.reg.set_tuning(item_pts=self._refine_item_pts)
+                self.reg.set_params(self.refine_params)
+            else:
+                self._refine_item_pts = self.reg.tuning()[2]
+                if self.fallback_item_pts:
+                    self.reg.set_tuning(item_pts=self.fallback_item_pts)
+                self.reg.set_params(self.params)
             self._active = which
 
     def align(self):

@@ -246,6 +246,14 @@ class BatchRegistration:
         L.check(L.lib.rtr_batch_set_tuning(self._h, float(cell_factor), float(cells_per_point), int(item_pts),
                                            int(blocks_per_cu), int(use_tile)))
 
+    def tuning(self):
+        """(cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile) in effect."""
+        cf, cp = C.c_float(), C.c_float()
+        ip, bp, ut = C.c_int(), C.c_int(), C.c_int()
+        L.check(L.lib.rtr_batch_get_tuning(self._h, C.byref(cf), C.byref(cp), C.byref(ip), C.byref(bp),
+                                           C.byref(ut)))
+        return (cf.value, cp.value, ip.value, bp.value, ut.value)
+
     def set_profiling(self, on: bool):
         L.check(L.lib.rtr_batch_set_profiling(self._h, int(on)))
 

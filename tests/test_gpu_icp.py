@@ -268,3 +268,30 @@ def test_anderson_reaches_an_equal_minimum_in_fewer_iterations():
         assert np.array_equal(x["T"], y["T"]) and x["iterations"] == y["iterations"]
     with pytest.raises(R.RtrError):
         b.set_params(R.make_params(anderson_memory=9, **work))
+
+
+def test_cell_tuning_change_rebuilds_reused_grids():
+    """With grids reused across aligns (rebuild-always off), a new cell
+    factor takes effect at the next align: the grid's cell edge changes and
+    the results equal an engine that rebuilds on every align; a later change
+    of iteration settings only keeps that cell edge."""
+    srcs, tgts, gts = synth.config3_batch(3, seed_offset=980, ns_range=(2000, 4000), nt_range=(5000, 9000))
+    p = R.make_params(max_iterations=15, max_correspondence_distance=0.05, transformation_epsilon=1e-8)
+    a, b = R.BatchRegistration(), R.BatchRegistration()
+    for e in (a, b):
+        e.set_clouds(srcs, tgts)
+        e.set_params(p)
+        e.set_problems(range(3), range(3))
+    b.set_rebuild(False)
+    b.align(gts)
+    cell0 = b.grid(0)[0]
+    for e in (a, b):
+        e.set_tuning(cell_factor=0.6)
+    rb, ra = b.align(gts), a.align(gts)
+    assert b.grid(0)[0] != cell0
+    for x, y in zip(ra, rb):
+        assert np.array_equal(x["T"], y["T"]) and x["iterations"] == y["iterations"]
+    cell1 = b.grid(0)[0]
+    b.set_params(R.make_params(max_iterations=20, max_correspondence_distance=0.05, transformation_epsilon=1e-8))
+    b.align(gts)
+    assert b.grid(0)[0] == cell1
