@@ -411,9 +411,12 @@ DevEngine Engine::dev_view() const {
   e.use_tile = tune_.use_tile;
   e.partials = partials_.as<float>();
   uint32_t* ctr = counters_.as<uint32_t>();
+  // the done count, the error flag and every claim counter on cache lines of
+  // their own: spinning workgroups poll the flag while others claim
   e.problems_done = ctr;
-  e.error = reinterpret_cast<int*>(ctr + 1);
-  e.claim = ctr + 8;
+  e.error = reinterpret_cast<int*>(ctr + 32);
+  e.claim = ctr + 64;
+  e.claim_stride = claim_stride();
   for (int x = 0; x <= kXcds; ++x) e.xcd_off[x] = xcd_off_[x];
   e.done = done_.as<uint32_t>();
   e.scan_tmp = scan_tmp_.as<uint32_t>();
@@ -486,7 +489,7 @@ int Engine::align(const double* guesses, rtr_result* results) {
   const size_t np = probs_.size();
   const size_t gbytes = np * 16 * sizeof(double);
   const size_t sbytes = np * sizeof(ProblemState);
-  RTR_TRY(ensure_pinned(gbytes + sbytes + 64));
+  RTR_TRY(ensure_pinned(gbytes + sbytes + 256));
   double* g = static_cast<double*>(pin_);
   for (size_t p = 0; p < np; ++p)
     for (int k = 0; k < 16; ++k) g[16 * p + k] = guesses ? guesses[16 * p + k] : ((k % 5 == 0) ? 1.0 : 0.0);
@@ -530,7 +533,7 @@ int Engine::align(const double* guesses, rtr_result* results) {
   ProblemState* hs = reinterpret_cast<ProblemState*>(static_cast<char*>(pin_) + gbytes);
   uint32_t* hc = reinterpret_cast<uint32_t*>(static_cast<char*>(pin_) + gbytes + sbytes);
   RTR_HIP(hipMemcpyAsync(hs, state_.p, sbytes, hipMemcpyDeviceToHost, stream_));
-  RTR_HIP(hipMemcpyAsync(hc, counters_.p, 8, hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipMemcpyAsync(hc, counters_.p, 33 * 4, hipMemcpyDeviceToHost, stream_));  // done count .. error flag
   if (profiling_) RTR_HIP(hipEventRecord(ev_[3], stream_));
   RTR_HIP(hipStreamSynchronize(stream_));
   if (profiling_) {
@@ -555,7 +558,7 @@ int Engine::align(const double* guesses, rtr_result* results) {
     r.reserved0 = 0;
   }
   timing_.iterations_total = it_total;
-  if (hc[1] != 0) {
+  if (hc[32] != 0) {  // the error flag (word 32)
     set_error("work-queue wait exceeded its bound; results are incomplete");
     return RTR_ERR_DEVICE;
   }

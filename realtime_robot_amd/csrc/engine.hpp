@@ -123,7 +123,8 @@ class Engine {
   int max_cells_ = 0;
   uint32_t xcd_off_[kXcds + 1] = {0};
   // [0] problems done, [1] error, [8 + pass * kXcds + x] claim counters
-  size_t counter_words() const { return 8 + (size_t)prm_.max_iterations * kXcds; }
+  int claim_stride() const { return prm_.max_iterations <= 256 ? 32 : 1; }
+  size_t counter_words() const { return 64 + (size_t)prm_.max_iterations * kXcds * claim_stride(); }
 
   // device arena
   DevBuf tgt_raw_, tgt_nrm_raw_, tgt_sorted_, tgt_nrm_sorted_, cell_count_, cell_start_, cell_id_;

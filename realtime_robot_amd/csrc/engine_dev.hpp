@@ -51,7 +51,8 @@ struct DevEngine {
   int item_pts;              // points per item
   int use_tile;              // stage candidate tiles in LDS (else global gathers only)
   float* partials;           // n_items * NSUM_MAX * 2 floats
-  uint32_t* claim;           // claim counter per (pass, XCD queue)
+  uint32_t* claim;           // claim counter per (pass, XCD queue), claim_stride words apart
+  int claim_stride;          // 32 (one 128-byte line per counter) unless the iteration limit is huge
   uint32_t xcd_off[kXcds + 1];  // item range of each XCD queue
   uint32_t* done;            // per problem: items finished so far
   uint32_t* problems_done;

@@ -550,11 +550,21 @@ __global__ __launch_bounds__(kBlock, MODE == RTR_MODE_POINT_TO_POINT ? 3 : 2) vo
         for (int t = 0; t < kXcds; ++t) {
           const int x = (x0 + t) % kXcds;
           const uint32_t cnt = e.xcd_off[x + 1] - e.xcd_off[x];
-          uint32_t* ctr = &e.claim[pass * kXcds + x];
+          uint32_t* ctr = &e.claim[(size_t)(pass * kXcds + x) * e.claim_stride];
           if (cnt == 0u || __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
           const uint32_t c = atomicAdd(ctr, 1u);
           if (c < cnt) {
             it = (int)(e.xcd_off[x] + c);
+            const uint32_t p = e.items[it].x;
+            if (load_acquire((const uint32_t*)&e.st[p].gen) & kFinishedBit) {
+              // a finished problem: its remaining items of this pass (contiguous
+              // in this queue) are skipped in one step, and the claim goes on
+              const Problem& pr = e.probs[p];
+              atomicMax(ctr, pr.item_off + pr.n_items - e.xcd_off[x]);
+              it = e.n_items;
+              t = -1;  // rescan the queues from the own XCD's
+              continue;
+            }
             break;
           }
         }
@@ -564,12 +574,18 @@ __global__ __launch_bounds__(kBlock, MODE == RTR_MODE_POINT_TO_POINT ? 3 : 2) vo
           const uint32_t* word = (const uint32_t*)&e.st[p].gen;
           uint32_t wv = load_acquire(word);
           int spins = 0;
+          // bounded wait with backoff; the error flag (on its own cache line)
+          // is polled every 256 rounds
           while (!(wv & kFinishedBit) && (int)wv < pass) {
-            if (++spins > kMaxSpins || load_acquire((const uint32_t*)e.error) != 0u) {
+            ++spins;
+            if (spins > kMaxSpins || ((spins & 255) == 0 && load_acquire((const uint32_t*)e.error) != 0u)) {
               atomicOr(e.error, 1);
               break;
             }
-            __builtin_amdgcn_s_sleep(2);
+            if (spins < 64)
+              __builtin_amdgcn_s_sleep(1);
+            else
+              __builtin_amdgcn_s_sleep(8);
             wv = load_acquire(word);
           }
           if (!(wv & kFinishedBit) && (int)wv == pass) {

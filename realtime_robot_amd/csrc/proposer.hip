@@ -334,7 +334,7 @@ constexpr int kFpsThreads = 512;
 constexpr int kFpsPer = kFpsMax / kFpsThreads;
 constexpr int kFpsWaves = kFpsThreads / 64;
 
-__global__ __launch_bounds__(kFpsThreads) void k_fps(const float* raw, const uint32_t* off, const uint32_t* cnt,
+__global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const uint32_t* off, const uint32_t* cnt,
                                                      float* pts_out /* cloud * kP * 3 */,
                                                      uint16_t* x0 /* rows of kC0 */, float* centroid /* cloud * 3 */,
                                                      int32_t* sel_out /* cloud*kP */) {
@@ -343,8 +343,11 @@ __global__ __launch_bounds__(kFpsThreads) void k_fps(const float* raw, const uin
   const float* p = raw + 3 * (size_t)off[c];
   const int m = (int)min((uint32_t)kFpsMax, n);
   __shared__ float s_q[3];
-  __shared__ float red_v[kFpsWaves];
-  __shared__ int red_i[kFpsWaves];
+  // per wave: its farthest point (distance, index, coordinates); double
+  // buffered by iteration parity, so one barrier per iteration suffices
+  __shared__ float red_v[2][kFpsWaves];
+  __shared__ int red_i[2][kFpsWaves];
+  __shared__ float red_p[2][kFpsWaves][3];
   __shared__ double cen[kFpsWaves][3];
   float px[kFpsPer], py[kFpsPer], pz[kFpsPer], dist[kFpsPer];
   double ax = 0, ay = 0, az = 0;
@@ -396,8 +399,8 @@ __global__ __launch_bounds__(kFpsThreads) void k_fps(const float* raw, const uin
     centroid[3 * c + 2] = cz;
   }
   int cur = 0;
+  float qx = s_q[0], qy = s_q[1], qz = s_q[2];
   for (int s = 0; s < kP; ++s) {
-    const float qx = s_q[0], qy = s_q[1], qz = s_q[2];
     if (tid == 0) {
       const size_t o = ((size_t)c * kP + s);
       pts_out[3 * o] = qx - cx;
@@ -410,6 +413,7 @@ __global__ __launch_bounds__(kFpsThreads) void k_fps(const float* raw, const uin
       const float v = tid == 0 ? qx - cx : (tid == 1 ? qy - cy : (tid == 2 ? qz - cz : 0.f));
       x0[o] = f2bf(v);
     }
+    if (s + 1 == kP) break;
     float bv = -1.f;
     int bi = 0x7fffffff;
 #pragma unroll
@@ -435,30 +439,43 @@ __global__ __launch_bounds__(kFpsThreads) void k_fps(const float* raw, const uin
         bi = oi;
       }
     }
-    if (lane == 0) {
-      red_v[w] = bv;
-      red_i[w] = bi;
-    }
-    __syncthreads();  // s_q has been read by every thread; red_* written
-    float gv = red_v[0];
-    int gi = red_i[0];
-    for (int k = 1; k < kFpsWaves; ++k)
-      if (red_v[k] > gv || (red_v[k] == gv && red_i[k] < gi)) {
-        gv = red_v[k];
-        gi = red_i[k];
-      }
-    cur = (gi >= 0 && gi < m) ? gi : 0;
-    if (tid == (cur % kFpsThreads)) {
-      const int kk = cur / kFpsThreads;
+    // the lane that owns the wave's winner publishes it with its coordinates
+    const int buf = s & 1;
+    if (bi >= 0 && bi < m && (bi % kFpsThreads) == tid) {
+      const int kk = bi / kFpsThreads;
 #pragma unroll
       for (int k = 0; k < kFpsPer; ++k)
         if (k == kk) {
-          s_q[0] = px[k];
-          s_q[1] = py[k];
-          s_q[2] = pz[k];
+          red_p[buf][w][0] = px[k];
+          red_p[buf][w][1] = py[k];
+          red_p[buf][w][2] = pz[k];
         }
+      red_v[buf][w] = bv;
+      red_i[buf][w] = bi;
+    } else if (lane == 0 && !(bi >= 0 && bi < m)) {  // no subsample point in this wave
+      red_v[buf][w] = -1.f;
+      red_i[buf][w] = 0x7fffffff;
     }
-    __syncthreads();  // red_* read by every thread; s_q holds the next point
+    __syncthreads();
+    float gv = red_v[buf][0];
+    int gi = red_i[buf][0], gw = 0;
+    for (int k = 1; k < kFpsWaves; ++k)
+      if (red_v[buf][k] > gv || (red_v[buf][k] == gv && red_i[buf][k] < gi)) {
+        gv = red_v[buf][k];
+        gi = red_i[buf][k];
+        gw = k;
+      }
+    if (gi >= 0 && gi < m) {
+      cur = gi;
+      qx = red_p[buf][gw][0];
+      qy = red_p[buf][gw][1];
+      qz = red_p[buf][gw][2];
+    } else {  // fewer subsample points than selections: repeat the first one
+      cur = 0;
+      qx = s_q[0];
+      qy = s_q[1];
+      qz = s_q[2];
+    }
   }
 }
 
