@@ -409,6 +409,7 @@ DevEngine Engine::dev_view() const {
   e.n_items = (int)items_.size();
   e.item_pts = tune_.item_pts;
   e.use_tile = tune_.use_tile;
+  e.deep = deep_gathers() ? 1 : 0;
   e.partials = partials_.as<float>();
   uint32_t* ctr = counters_.as<uint32_t>();
   // the done count, the error flag and every claim counter on cache lines of
@@ -442,7 +443,7 @@ DevEngine Engine::dev_view() const {
 }
 
 int Engine::grid_blocks() const {
-  int bpc = icp_blocks_per_cu(prm_.mode, tune_.use_tile);
+  int bpc = icp_blocks_per_cu(prm_.mode, tune_.use_tile, deep_gathers());
   if (tune_.blocks_per_cu > 0 && tune_.blocks_per_cu < bpc) bpc = tune_.blocks_per_cu;
   long long b = (long long)cu_count_ * bpc;
   if (b > (long long)items_.size()) b = (long long)items_.size();

@@ -124,6 +124,9 @@ class Engine {
   uint32_t xcd_off_[kXcds + 1] = {0};
   // [0] problems done, [1] error, [8 + pass * kXcds + x] claim counters
   int claim_stride() const { return prm_.max_iterations <= 256 ? 32 : 1; }
+  // the 8-deep search kernel (2 waves per SIMD) serves launches tuned to at
+  // most 2 workgroups per CU, where a third wave would not be resident
+  bool deep_gathers() const { return tune_.blocks_per_cu == 1 || tune_.blocks_per_cu == 2; }
   size_t counter_words() const { return 64 + (size_t)prm_.max_iterations * kXcds * claim_stride(); }
 
   // device arena

@@ -50,6 +50,7 @@ struct DevEngine {
   int n_items;
   int item_pts;              // points per item
   int use_tile;              // stage candidate tiles in LDS (else global gathers only)
+  int deep;                  // 8-deep gathers, 2 waves per SIMD (point-to-point; always for point-to-plane)
   float* partials;           // n_items * NSUM_MAX * 2 floats
   uint32_t* claim;           // claim counter per (pass, XCD queue), claim_stride words apart
   int claim_stride;          // 32 (one 128-byte line per counter) unless the iteration limit is huge
@@ -69,7 +70,7 @@ size_t scan_tmp_words(uint32_t n);
 void launch_target_grid(const DevEngine& e, int max_tgt_pts, int max_cells, hipStream_t s);
 void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s);
 void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s);
-int icp_blocks_per_cu(int mode, int use_tile);
+int icp_blocks_per_cu(int mode, int use_tile, int deep);
 // correspondences / fitness / transformed output for problem set; T per
 // problem from `T` (n_prob*16 column-major, device) or from the state when null.
 void launch_correspondences(const DevEngine& e, int prob, const double* T, float r2f, int32_t* idx,

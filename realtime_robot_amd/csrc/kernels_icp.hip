@@ -167,6 +167,7 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) 
 
 // Nearest-neighbour state: packed (d2, index) key and where the winner lives
 // (an LDS tile index, or a cell-sorted slot with the top bit set).
+template <int D>
 struct GlobalVis {
   const float4* pts;
   float qx, qy, qz;
@@ -179,9 +180,25 @@ struct GlobalVis {
       where = j | 0x80000000u;
     }
   }
-  // four independent loads in flight per step (gathers are latency-bound)
+  // D (4 or 8) independent loads in flight per step: the gathers are
+  // latency-bound, and the deeper variant needs the registers of a kernel
+  // that runs 2 waves per SIMD
   __device__ __forceinline__ void scan(uint32_t b, uint32_t e) {
     uint32_t j = b;
+    if (D == 8) {
+      for (; j + 8 <= e; j += 8) {
+        const float4 t0 = pts[j], t1 = pts[j + 1], t2 = pts[j + 2], t3 = pts[j + 3];
+        const float4 t4 = pts[j + 4], t5 = pts[j + 5], t6 = pts[j + 6], t7 = pts[j + 7];
+        take(t0, j);
+        take(t1, j + 1);
+        take(t2, j + 2);
+        take(t3, j + 3);
+        take(t4, j + 4);
+        take(t5, j + 5);
+        take(t6, j + 6);
+        take(t7, j + 7);
+      }
+    }
     for (; j + 4 <= e; j += 4) {
       const float4 t0 = pts[j], t1 = pts[j + 1], t2 = pts[j + 2], t3 = pts[j + 3];
       take(t0, j);
@@ -196,8 +213,9 @@ struct GlobalVis {
 
 // The 3x3x3 block straight from global memory: the 18 row bounds are loaded
 // together before any row is scanned.
+template <int D>
 __device__ __forceinline__ void walk_block_global(const GridView& g, float qx, float qy, float qz, int cx, int cy,
-                                                  int cz, float r2f, GlobalVis& vis) {
+                                                  int cz, float r2f, GlobalVis<D>& vis) {
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
   // all 18 row bounds in flight together; rows are then scanned own row
@@ -325,7 +343,7 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
   }
 }
 
-template <int MODE, int NS, bool TILE, class TileT>
+template <int MODE, int NS, bool TILE, class TileT, int DEPTH>
 __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS, TileT>& sh) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   TileT& tl = sh.tile[w];
@@ -367,7 +385,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         k0 = D - 1;
       }
     }
-    GlobalVis vis;
+    GlobalVis<DEPTH> vis;
     vis.pts = g.pts;
     vis.qx = qx;
     vis.qy = qy;
@@ -526,12 +544,15 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   }
 }
 
-template <int MODE, bool TILE>
-// Point-to-point asks for at least 3 waves per SIMD (<= 168 VGPRs, no
-// spills): the gathers of the ring walk are latency-bound, so resident waves
-// are what hides them.  Point-to-plane carries 29 compensated sums and keeps
-// 2 waves per SIMD rather than spill.
-__global__ __launch_bounds__(kBlock, MODE == RTR_MODE_POINT_TO_POINT ? 3 : 2) void k_icp(DevEngine e) {
+template <int MODE, bool TILE, int DEPTH>
+// Point-to-point with 4-deep gathers asks for at least 3 waves per SIMD
+// (<= 168 VGPRs, no spills): the gathers of the ring walk are latency-bound,
+// so resident waves are what hides them.  The 8-deep variant keeps more
+// gathers in flight per wave and runs 2 waves per SIMD; it serves launches
+// tuned to 2 workgroups per CU, where the third wave would not be resident
+// anyway.  Point-to-plane carries 29 compensated sums, runs 2 waves per SIMD
+// and is always 8-deep.
+__global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH == 4) ? 3 : 2) void k_icp(DevEngine e) {
   constexpr int NS = MODE == RTR_MODE_POINT_TO_POINT ? SUM_P2P_COUNT : SUM_P2L_COUNT;
   using TileT = typename std::conditional<TILE, WaveTile, WaveTileNone>::type;
   __shared__ IcpShared<NS, TileT> sh;
@@ -603,23 +624,28 @@ __global__ __launch_bounds__(kBlock, MODE == RTR_MODE_POINT_TO_POINT ? 3 : 2) vo
       __syncthreads();
       const int it = sh.item;
       if (it >= e.n_items) break;
-      if (sh.run) process_item<MODE, NS, TILE, TileT>(e, it, pass, sh);
+      if (sh.run) process_item<MODE, NS, TILE, TileT, DEPTH>(e, it, pass, sh);
       __syncthreads();
     }
   }
 }
 
-int icp_blocks_per_cu(int mode, int use_tile) {
+template <int MODE, bool TILE, int D>
+static int blocks_per_cu_of() {
   int nb = 0;
-  hipError_t err;
-  if (mode == RTR_MODE_POINT_TO_PLANE)
-    err = use_tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_PLANE, true>, kBlock, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_PLANE, false>, kBlock, 0);
-  else
-    err = use_tile ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_POINT, true>, kBlock, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<RTR_MODE_POINT_TO_POINT, false>, kBlock, 0);
-  if (err != hipSuccess || nb < 1) nb = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<MODE, TILE, D>, kBlock, 0) != hipSuccess || nb < 1) nb = 1;
   return nb;
+}
+
+int icp_blocks_per_cu(int mode, int use_tile, int deep) {
+  if (mode == RTR_MODE_POINT_TO_PLANE)
+    return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_PLANE, true, 8>()
+                    : blocks_per_cu_of<RTR_MODE_POINT_TO_PLANE, false, 8>();
+  if (deep)
+    return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, true, 8>()
+                    : blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, false, 8>();
+  return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, true, 4>()
+                  : blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, false, 4>();
 }
 
 void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s) {
@@ -627,14 +653,19 @@ void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s) {
   const dim3 g(grid_blocks), b(kBlock);
   if (e.prm.mode == RTR_MODE_POINT_TO_PLANE) {
     if (e.use_tile)
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, true>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, true, 8>), g, b, 0, s, e);
     else
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, false>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, false, 8>), g, b, 0, s, e);
+  } else if (e.deep) {
+    if (e.use_tile)
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true, 8>), g, b, 0, s, e);
+    else
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false, 8>), g, b, 0, s, e);
   } else {
     if (e.use_tile)
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true, 4>), g, b, 0, s, e);
     else
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false, 4>), g, b, 0, s, e);
   }
 }
 
