@@ -273,6 +273,37 @@ int rtr_tdf_points(int device, const float* xyz, size_t n, size_t stride_bytes, 
                    float resolution, const int dims[3], float trunc2, float* out);
 
 /* ------------------------------------------------------------------ */
+/* Keypoint + TDF + yaw-search coarse proposer: the reference's own     */
+/* initial alignment (RealTimeRobot.cpp:45-105: keypoints, occupancy    */
+/* key_point.h:112-161, TDF key_point.h:251-334, get_Distance's 36 yaw  */
+/* steps matching.h:122-222, consensus function.h:35-109), kept as the  */
+/* baseline of the learned proposer.  Per pair: 32 FPS keypoints per    */
+/* cloud; source keypoint occupancy (1 cm voxels within +-10 cm);       */
+/* target keypoint TDF (30^3, cap 900); the best of 36 yaws about z for */
+/* every (target, source) keypoint pair; the `candidates` best pairs    */
+/* become poses T = Trans(k_t) Rz(yaw) Trans(-k_s) (source -> target,   */
+/* column-major) and the lowest truncated chamfer (tau, metres) over    */
+/* 1024-point subsamples wins.  Yaw only: the reference assumes the     */
+/* gravity axis is shared.                                              */
+/* ------------------------------------------------------------------ */
+typedef struct rtr_coarse rtr_coarse;
+int rtr_coarse_create(rtr_coarse** out, int device);
+int rtr_coarse_destroy(rtr_coarse* h);
+int rtr_coarse_set_clouds(rtr_coarse* h, int n, const float* const* src, const size_t* ns, const float* const* tgt,
+                          const size_t* nt);
+/* poses: n * 16 doubles (column-major); scores (nullable): n chamfer scores. */
+int rtr_coarse_propose(rtr_coarse* h, int candidates, float resolution, float tau, double* poses, double* scores);
+/* The keypoint-pair scores behind rtr_coarse_propose: scores/yaw n*32*32
+ * (pair, target keypoint, source keypoint; 3.4e38 = rejected by the
+ * occupancy-count filter), keypoints (nullable) 2n*32*3 (cloud 2p = source,
+ * 2p+1 = target). */
+int rtr_coarse_pair_scores(rtr_coarse* h, float resolution, float* scores, int* yaw, float* keypoints);
+/* Source keypoint occupancy of the last scoring: voxels n*32*2048 (index
+ * x + 20 y + 400 z, ascending), counts n*32. */
+int rtr_coarse_occupancy(rtr_coarse* h, uint16_t* voxels, int* counts);
+float rtr_coarse_last_ms(rtr_coarse* h);
+
+/* ------------------------------------------------------------------ */
 /* Learned pose proposer: farthest-point sampling (1024 points per      */
 /* cloud), shared point MLP 3-64-128-256 + max-pool per cloud, head     */
 /* 512-256-9 (6D rotation + translation), BF16 MFMA with FP32           */

@@ -117,6 +117,13 @@ _SIGS = {
     "rtr_copy_probe": (_i, [_i, _sz, _i, C.POINTER(_f)]),
     "rtr_tdf": (_i, [_i, _vp, _sz, _vp, _f, _vp]),
     "rtr_tdf_points": (_i, [_i, _vp, _sz, _sz, _vp, _f, _vp, _f, _vp]),
+    "rtr_coarse_create": (_i, [C.POINTER(_vp), _i]),
+    "rtr_coarse_destroy": (_i, [_vp]),
+    "rtr_coarse_set_clouds": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "rtr_coarse_propose": (_i, [_vp, _i, _f, _f, _vp, _vp]),
+    "rtr_coarse_pair_scores": (_i, [_vp, _f, _vp, _vp, _vp]),
+    "rtr_coarse_occupancy": (_i, [_vp, _vp, _vp]),
+    "rtr_coarse_last_ms": (_f, [_vp]),
     "rtr_prop_create": (_i, [C.POINTER(_vp), _i, _i]),
     "rtr_prop_destroy": (_i, [_vp]),
     "rtr_prop_num_params": (_i, [C.POINTER(_sz)]),

@@ -40,11 +40,6 @@ void DevBuf::release() {
   bytes = 0;
 }
 
-#define RTR_TRY(x)             \
-  do {                         \
-    int _rc = (x);             \
-    if (_rc != RTR_OK) return _rc; \
-  } while (0)
 
 Engine::Engine(int device) : device_(device) {
   rtr_params_default(&prm_);

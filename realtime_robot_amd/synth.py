@@ -244,6 +244,28 @@ def config4_batch(n_pairs: int = 256, seed_offset: int = 0, ns_range=(2000, 5000
     return srcs, tgts, gts
 
 
+def config4_yaw_batch(n_pairs: int = 256, seed_offset: int = 0, ns_range=(2000, 50000), nt_range=(10000, 200000),
+                      max_deg: float = 180.0, max_t: float = 0.1, noise: float = 0.001):
+    """config4_batch with the source turned about the vertical (z) axis only,
+    by U[-max_deg, max_deg]: the gravity-aligned setting the reference's own
+    coarse alignment (a yaw search) assumes.  Returns (sources, targets, T_gt)."""
+    srcs, tgts, gts = [], [], []
+    for p in range(n_pairs):
+        rng = rng_for(41, seed_offset + p)
+        boxes = random_composite(rng, asymmetric=True)
+        ns = loguniform_int(rng, *ns_range)
+        nt = loguniform_int(rng, *nt_range)
+        tgt, src0 = sample_many([boxes, boxes], [nt, ns], rng, noise)
+        a = np.deg2rad(rng.uniform(-max_deg, max_deg))
+        T_src = np.eye(4)
+        T_src[:3, :3] = [[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]]
+        T_src[:3, 3] = rng.uniform(-max_t, max_t, size=3)
+        srcs.append(apply(T_src, src0))
+        tgts.append(tgt)
+        gts.append(np.linalg.inv(T_src))
+    return srcs, tgts, gts
+
+
 def training_base(n_pairs: int, seed: int, ns_range=(2000, 6000), nt_range=(4000, 12000), noise: float = 0.001,
                   chunk: int = 512):
     """Canonical training pairs (source and target independent samples of the
