@@ -123,9 +123,18 @@ struct WaveTileNone {
   uint32_t row_off[1];
 };
 
+// Queries of an item are classified, then walked from a compacted list in
+// chunks of at most kChunk (the throughput search variant).
+constexpr int kChunk = 1024;
+constexpr int kQueryClasses = 4;  // block walk; rings from 1, 2, 3 and more cells out
+
 template <int NS, class TileT>
 struct IcpShared {
   TileT tile[kWaves];
+  uint8_t qcls[kChunk];   // per query of the chunk: 0 no correspondence possible, 1 block walk, 2.. rings only
+  uint8_t qd[kChunk];     // its cell's distance-field value
+  uint16_t qlist[kChunk]; // the chunk's queries to walk by class, each class in query order
+  int nlist;
   float red[kWaves][NS][2];
   double dred[kBlock / NSUM_MAX][NSUM_MAX];
   double ws[kAndersonWs];
@@ -362,7 +371,87 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
 #pragma unroll
   for (int k = 0; k < NS; ++k) acc_s[k] = acc_c[k] = 0.f;
   const uint32_t end = min(itm.y + (uint32_t)e.item_pts, sd.n);
-  for (uint32_t base = itm.y + (uint32_t)w * 64u; base < end; base += kBlock) {
+  constexpr bool kCompact = !TILE && DEPTH == 4;
+  if (kCompact) {
+    // Compacted walk (the throughput variant).  Queries that cannot have a
+    // correspondence (distance field beyond r_max, NaN) take no lane, and the
+    // others are grouped by where their walk starts (their 3x3x3 block, or
+    // the outer ring the distance field allows: 2, 3, 4 and beyond), so a
+    // wave's 64 lanes carry similar work.  The list order is fixed (class,
+    // then query order), so the results are deterministic.
+    for (uint32_t chunk = itm.y; chunk < end; chunk += kChunk) {
+      const uint32_t cn = min(end - chunk, (uint32_t)kChunk);
+      for (uint32_t o = tid; o < cn; o += kBlock) {
+        const uint32_t sj = sd.off + chunk + o;
+        float qx, qy, qz;
+        int cx, cy, cz;
+        apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+        uint8_t cls = 0, dv = 0;
+        if (query_cell(g, qx, qy, qz, cx, cy, cz)) {
+          const int D = e.cell_dist ? (int)e.cell_dist[td.cell_off + ((uint32_t)cz * g.dy + cy) * g.dx + cx] : 0;
+          if (!(e.cell_dist && D >= td.skip_ring)) {
+            cls = (uint8_t)(D <= 1 ? 1 : (D >= kQueryClasses ? kQueryClasses : D));
+            dv = (uint8_t)D;
+          }
+        }
+        sh.qcls[o] = cls;
+        sh.qd[o] = dv;
+      }
+      __syncthreads();
+      if (w == 0) {
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t nl = 0;
+        for (int cls = 1; cls <= kQueryClasses; ++cls)
+          for (uint32_t b = 0; b < cn; b += 64) {
+            const bool f = b + lane < cn && sh.qcls[b + lane] == cls;
+            const uint64_t bal = __ballot(f);
+            if (f) sh.qlist[nl + (uint32_t)__popcll(bal & below)] = (uint16_t)(b + lane);
+            nl += (uint32_t)__popcll(bal);
+          }
+        if (lane == 0) sh.nlist = (int)nl;
+      }
+      __syncthreads();
+      const int nl = sh.nlist;
+      for (int base = w * 64; base < nl; base += kBlock) {
+        const int li = base + lane;
+        if (li >= nl) continue;
+        const uint32_t o = sh.qlist[li];
+        const uint32_t sj = sd.off + chunk + o;
+        float qx, qy, qz;
+        int cx, cy, cz;
+        apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+        query_cell(g, qx, qy, qz, cx, cy, cz);
+        const int D = sh.qd[o];
+        GlobalVis<DEPTH> vis;
+        vis.pts = g.pts;
+        vis.qx = qx;
+        vis.qy = qy;
+        vis.qz = qz;
+        vis.key = ~0ull;
+        vis.where = 0u;
+        if (D < 2) walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
+        walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, D >= 2 ? D - 1 : 1);
+        if (vis.key != ~0ull && key_d2(vis.key) <= r2f) {
+          const uint32_t slot = vis.where & 0x7fffffffu;
+          const float4 c = g.pts[slot];
+          float nx = 0.f, ny = 0.f, nz = 0.f;
+          if (MODE == RTR_MODE_POINT_TO_PLANE) {
+            const float4 nn = e.tgt_nrm_sorted[slot];
+            nx = nn.x;
+            ny = nn.y;
+            nz = nn.z;
+          }
+          float terms[NSUM_MAX];
+          point_terms(MODE, qx - ax, qy - ay, qz - az, c.x - ax, c.y - ay, c.z - az, nx, ny, nz,
+                      key_d2(vis.key), terms);
+#pragma unroll
+          for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+        }
+      }
+      __syncthreads();  // the next chunk rewrites the lists
+    }
+  }
+  for (uint32_t base = itm.y + (uint32_t)w * 64u; !kCompact && base < end; base += kBlock) {
     const uint32_t j = base + lane;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     int cx = 0, cy = 0, cz = 0;
