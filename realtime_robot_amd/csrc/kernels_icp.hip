@@ -126,7 +126,7 @@ struct WaveTileNone {
 // Queries of an item are classified, then walked from a compacted list in
 // chunks of at most kChunk (the throughput search variant).
 constexpr int kChunk = 1024;
-constexpr int kQueryClasses = 4;  // block walk; rings from 1, 2, 3 and more cells out
+constexpr int kQueryClasses = 5;  // distance field 0 (own cell occupied), 1 (block), 2, 3, 4+ (rings)
 
 template <int NS, class TileT>
 struct IcpShared {
@@ -375,9 +375,9 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   if (kCompact) {
     // Compacted walk (the throughput variant).  Queries that cannot have a
     // correspondence (distance field beyond r_max, NaN) take no lane, and the
-    // others are grouped by where their walk starts (their 3x3x3 block, or
-    // the outer ring the distance field allows: 2, 3, 4 and beyond), so a
-    // wave's 64 lanes carry similar work.  The list order is fixed (class,
+    // others are grouped by their cell's distance-field value (own cell
+    // occupied; a neighbour occupied; the outer ring the walk starts on: 2, 3,
+    // 4 and beyond), so a wave's 64 lanes carry similar work.  The list order is fixed (class,
     // then query order), so the results are deterministic.
     for (uint32_t chunk = itm.y; chunk < end; chunk += kChunk) {
       const uint32_t cn = min(end - chunk, (uint32_t)kChunk);
@@ -390,7 +390,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         if (query_cell(g, qx, qy, qz, cx, cy, cz)) {
           const int D = e.cell_dist ? (int)e.cell_dist[td.cell_off + ((uint32_t)cz * g.dy + cy) * g.dx + cx] : 0;
           if (!(e.cell_dist && D >= td.skip_ring)) {
-            cls = (uint8_t)(D <= 1 ? 1 : (D >= kQueryClasses ? kQueryClasses : D));
+            cls = (uint8_t)(D + 1 >= kQueryClasses ? kQueryClasses : D + 1);
             dv = (uint8_t)D;
           }
         }
