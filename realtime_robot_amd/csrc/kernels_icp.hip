@@ -192,12 +192,14 @@ struct GlobalVis {
   // D (4 or 8) independent loads in flight per step: the gathers are
   // latency-bound, and the deeper variant needs the registers of a kernel
   // that runs 2 waves per SIMD
+  // (a walking pointer: one address per group, the loads at immediate offsets)
   __device__ __forceinline__ void scan(uint32_t b, uint32_t e) {
     uint32_t j = b;
+    const float4* q = pts + b;
     if (D == 8) {
-      for (; j + 8 <= e; j += 8) {
-        const float4 t0 = pts[j], t1 = pts[j + 1], t2 = pts[j + 2], t3 = pts[j + 3];
-        const float4 t4 = pts[j + 4], t5 = pts[j + 5], t6 = pts[j + 6], t7 = pts[j + 7];
+      for (; j + 8 <= e; j += 8, q += 8) {
+        const float4 t0 = q[0], t1 = q[1], t2 = q[2], t3 = q[3];
+        const float4 t4 = q[4], t5 = q[5], t6 = q[6], t7 = q[7];
         take(t0, j);
         take(t1, j + 1);
         take(t2, j + 2);
@@ -208,14 +210,14 @@ struct GlobalVis {
         take(t7, j + 7);
       }
     }
-    for (; j + 4 <= e; j += 4) {
-      const float4 t0 = pts[j], t1 = pts[j + 1], t2 = pts[j + 2], t3 = pts[j + 3];
+    for (; j + 4 <= e; j += 4, q += 4) {
+      const float4 t0 = q[0], t1 = q[1], t2 = q[2], t3 = q[3];
       take(t0, j);
       take(t1, j + 1);
       take(t2, j + 2);
       take(t3, j + 3);
     }
-    for (; j < e; ++j) take(pts[j], j);
+    for (; j < e; ++j, ++q) take(*q, j);
   }
   __device__ __forceinline__ float bound() const { return key == ~0ull ? 3.4e38f : key_d2(key); }
 };
