@@ -11,6 +11,17 @@
 // the increment, applies the convergence rules and publishes iteration k+1.
 // Problems therefore advance independently; no grid-wide barrier is needed,
 // and every wait is bounded (a timeout sets e.error and the waiters bail out).
+// Each pass has one queue per XCD (problems of one target share an XCD, whose
+// L2 then holds that target's grid); a workgroup claims from its own XCD's
+// queue and steals from the others when it is empty, and a claim that lands
+// on a finished problem skips that problem's remaining items of the pass.
+//
+// Two search variants: the throughput variant (4 gathers in flight per lane,
+// 3 waves/SIMD) classifies an item's queries first and walks them from a
+// compacted list grouped by distance-field class; the latency variant (8
+// gathers in flight, 2 waves/SIMD) walks each 64-query round directly.  With
+// Anderson acceleration on, the solving workgroup also extrapolates the next
+// iterate (geom.hpp).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
