@@ -387,7 +387,7 @@ class LearnedRegistration:
     engine's tuned item size."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
-                 min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 2048,
+                 min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 1024,
                  max_curvature: float = 0.02, hypotheses: int = 8, tau: float = 0.1, max_pairs: int = 256,
                  refine_anderson: int = 5, fallback_item_pts: int = 256, device: int = 0):
         if weights is None:
