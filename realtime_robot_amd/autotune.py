@@ -47,6 +47,8 @@ LATENCY_CANDIDATES = CANDIDATES + [
     (0.25, 2.0, 256, 2, 0),
     (0.35, 2.0, 512, 2, 0),
     (0.2, 3.0, 512, 2, 0),
+    (0.35, 2.0, 512, 1, 0),
+    (0.35, 2.0, 256, 1, 0),
 ]
 
 

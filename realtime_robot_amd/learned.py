@@ -382,32 +382,35 @@ class LearnedRegistration:
     refine_anderson > 0 runs the refine launch with Anderson acceleration of
     that memory (rtr_params.anderson_memory): an equally good minimum in
     fewer iterations.  The fallback is always the plain multi-start
-    iteration; its launch (a few pairs x n_starts problems, latency-bound)
-    uses work items of fallback_item_pts source points, the refine launch the
-    engine's tuned item size."""
+    iteration, on an engine of its own that holds only the rejected pairs'
+    clouds: the refine engine keeps one layout (its hipGraph replays every
+    step), and the fallback launch (a few pairs x n_starts problems,
+    latency-bound) runs work items of fallback_item_pts source points on
+    fallback_blocks_per_cu persistent workgroups per CU (fewer idle
+    workgroups polling the work queue)."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 1024,
                  max_curvature: float = 0.02, hypotheses: int = 8, tau: float = 0.1, max_pairs: int = 256,
-                 refine_anderson: int = 5, fallback_item_pts: int = 256, device: int = 0):
+                 refine_anderson: int = 5, fallback_item_pts: int = 256, fallback_blocks_per_cu: int = 1,
+                 device: int = 0):
         if weights is None:
             weights, _ = load_weights()
         self.params = L.Params.from_buffer_copy(params if params is not None else default_params())
         self.params.anderson_memory = 0
         self.refine_params = L.Params.from_buffer_copy(self.params)
         self.refine_params.anderson_memory = refine_anderson
-        self.fallback_item_pts = fallback_item_pts
-        self._refine_item_pts = None
-        self._active = "plain"
+        self._active = "refine"
         self.hypotheses = hypotheses
         self.rots = multistart.so3_rotations(hypotheses) if hypotheses > 1 else None
         self.tau = tau
         self.prop = Proposer(max_pairs=max_pairs * max(1, hypotheses), device=device, weights=weights)
-        self.reg = BatchRegistration(device)
-        self.reg.set_params(self.params)
-        # one grid build per align() call, shared by the refine and fallback launches
-        self.reg.set_rebuild(False)
-        self._problems = None
+        self.reg = BatchRegistration(device)       # refine: every pair from its proposal
+        self.reg.set_params(self.refine_params)
+        self.reg.set_rebuild(False)                # one grid build per align() call
+        self.fb = BatchRegistration(device)        # fallback: the rejected pairs only
+        self.fb.set_params(self.params)
+        self.fb.set_tuning(item_pts=fallback_item_pts, blocks_per_cu=fallback_blocks_per_cu)
         self.n_starts = n_starts
         self.rmse_threshold = rmse_threshold
         self.min_inlier_frac = min_inlier_frac
@@ -418,31 +421,19 @@ class LearnedRegistration:
 
     def set_clouds(self, sources, targets, src_centroids=None, tgt_centroids=None):
         self.n = len(sources)
+        self._sources, self._targets = list(sources), list(targets)
         self.prop.set_clouds(sources, targets)
         self.reg.set_clouds(sources, targets)
-        self._problems = None
+        self._ids = np.arange(self.n)
+        self.reg.set_problems(self._ids, self._ids)
         self.cs = _centroids(sources) if src_centroids is None else np.asarray(src_centroids, np.float64)
         self.ct = _centroids(targets) if tgt_centroids is None else np.asarray(tgt_centroids, np.float64)
-        self._ids = np.arange(self.n)
-
-    def _set_problems(self, key, ids):
-        if self._problems != key:
-            self.reg.set_problems(ids, ids)
-            self._problems = key
 
     def use_params(self, which: str):
-        """Make the engine's parameters those of the 'refine' or the 'plain'
-        (fallback) launch."""
+        """Give the refine engine the parameters of the 'refine' launch or of
+        the 'plain' iteration (the same launch unaccelerated)."""
         if self._active != which:
-            if which == "refine":
-                if self._refine_item_pts is not None:
-                    self.reg.set_tuning(item_pts=self._refine_item_pts)
-                self.reg.set_params(self.refine_params)
-            else:
-                self._refine_item_pts = self.reg.tuning()[2]
-                if self.fallback_item_pts:
-                    self.reg.set_tuning(item_pts=self.fallback_item_pts)
-                self.reg.set_params(self.params)
+            self.reg.set_params(self.refine_params if which == "refine" else self.params)
             self._active = which
 
     def align(self):
@@ -454,7 +445,6 @@ class LearnedRegistration:
         else:
             poses = self.prop.propose()
         t1 = time.perf_counter()
-        self._set_problems("refine", self._ids)
         self.use_params("refine")
         res = self.reg.align(poses)
         t2 = time.perf_counter()
@@ -462,11 +452,16 @@ class LearnedRegistration:
         t3 = time.perf_counter()
         fail = np.where((rmse > self.rmse_threshold) | (frac < self.min_inlier_frac))[0]
         if len(fail):
-            ids = np.repeat(fail, self.n_starts)
-            self._set_problems(("fallback", tuple(fail.tolist())), ids)
-            self.use_params("plain")
+            # the fallback engine takes the rejected pairs' clouds (a few MB at
+            # most) and the refine engine's cell settings
+            cf, cpp = self.reg.tuning()[:2]
+            if self.fb.tuning()[:2] != (cf, cpp):
+                self.fb.set_tuning(cell_factor=cf, cells_per_point=cpp)
+            self.fb.set_clouds([self._sources[i] for i in fail], [self._targets[i] for i in fail])
+            ids = np.repeat(np.arange(len(fail)), self.n_starts)
+            self.fb.set_problems(ids, ids)
             g = np.stack([T for i in fail for T in multistart.so3_start_poses(self.cs[i], self.ct[i], self.n_starts)])
-            res8 = self.reg.align(g)
+            res8 = self.fb.align(g)
             for k, i in enumerate(fail):
                 r = res8[k * self.n_starts:(k + 1) * self.n_starts]
                 res[i] = r[multistart.select_best(r)]
