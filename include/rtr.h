@@ -96,6 +96,7 @@ typedef struct {
   float ms_finish;     /* output transforms / fitness */
   int64_t icp_launches;
   int64_t iterations_total; /* sum of iterations over problems */
+  int64_t queries_searched; /* nearest-neighbour searches of the launch (every align) */
 } rtr_timing;
 
 const char* rtr_last_error(void);
@@ -199,6 +200,13 @@ int rtr_batch_get_grid(rtr_batch* b, int tgt, float* cell, int dims[3], float an
  * (12 floats per entry): entry i for iteration i, entry `iterations` the final
  * one.  *n_out = iterations + 1. */
 int rtr_batch_get_history(rtr_batch* b, int prob, float* rows12, int cap, int* n_out);
+/* Correspondences of the last iteration the latest align ran for problem
+ * `prob`, as its search found them (parity hook; the per-iteration
+ * correspondences PCL's ICP keeps, function.h:117): idx[i] the nearest target
+ * point of source point i and d2[i] its float squared distance, or -1 / +inf
+ * when none lies within the max correspondence distance.  Arrays of the
+ * source's size.  Recorded by the global-walk search (use_tile 0) only. */
+int rtr_batch_get_last_correspondences(rtr_batch* b, int prob, int32_t* idx, float* d2);
 /* Fitness score of every problem at its final transform (see rtr_icp_get_fitness_score). */
 int rtr_batch_fitness(rtr_batch* b, double max_range, double* out);
 /* Registration judge at each problem's final transform.  A strided sample of

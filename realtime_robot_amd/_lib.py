@@ -43,7 +43,8 @@ class Result(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("ms_total", C.c_float), ("ms_grid", C.c_float), ("ms_icp", C.c_float), ("ms_finish", C.c_float),
-                ("icp_launches", C.c_int64), ("iterations_total", C.c_int64)]
+                ("icp_launches", C.c_int64), ("iterations_total", C.c_int64),
+                ("queries_searched", C.c_int64)]
 
 
 def _load():
@@ -102,6 +103,7 @@ _SIGS = {
     "rtr_batch_get_grid": (_i, [_vp, _i, _vp, _vp, _vp]),
     "rtr_batch_fitness": (_i, [_vp, _d, _vp]),
     "rtr_batch_get_history": (_i, [_vp, _i, _vp, _i, C.POINTER(_i)]),
+    "rtr_batch_get_last_correspondences": (_i, [_vp, _i, _vp, _vp]),
     "rtr_batch_judge": (_i, [_vp, _i, _i, _d, _vp, _vp, _vp]),
     "rtr_map_create": (_i, [C.POINTER(_vp), _i, _f, _sz]),
     "rtr_map_destroy": (_i, [_vp]),

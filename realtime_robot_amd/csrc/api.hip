@@ -476,6 +476,13 @@ int rtr_batch_get_history(rtr_batch* b, int prob, float* rows12, int cap, int* n
   RTR_GUARD_END
 }
 
+int rtr_batch_get_last_correspondences(rtr_batch* b, int prob, int32_t* idx, float* d2) {
+  RTR_GUARD_BEGIN
+  if (!b || !idx || !d2) return RTR_ERR_INVALID_ARG;
+  return b->eng.last_correspondences(prob, idx, d2);
+  RTR_GUARD_END
+}
+
 int rtr_batch_fitness(rtr_batch* b, double max_range, double* out) {
   RTR_GUARD_BEGIN
   if (!b || !out) return RTR_ERR_INVALID_ARG;
@@ -561,6 +568,7 @@ int rtr_selfcheck_nn_stats(const float* tgt, size_t nt, const float* q, size_t n
   g.pts = pts.data();
   g.cell_start = start.data();
   g.n_pts = (uint32_t)nv;
+  g.pt_lo = 0;
   const double r2 = max_dist * max_dist;
   float r2f;
   if (!(r2 < (double)FLT_MAX))

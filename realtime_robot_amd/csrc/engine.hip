@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <cfloat>
+#include "geom.hpp"
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -71,7 +72,7 @@ Engine::~Engine() {
                     &cell_start_, &cell_id_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
                     &bin_count_, &bin_start_, &bin_id_, &src_desc_, &prob_desc_, &state_,
                     &guesses_, &items_buf_, &partials_, &done_, &counters_, &scan_tmp_,
-                    &out_i32_, &out_f32_, &out_T_, &hist_, &aa_};
+                    &out_i32_, &out_f32_, &out_T_, &hist_, &aa_, &nn_seed_};
   for (DevBuf* b : bufs) b->release();
   if (pin_) (void)hipHostFree(pin_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -346,6 +347,8 @@ int Engine::layout_problems() {
   RTR_TRY(out_T_.ensure(16 * sizeof(double)));
   RTR_TRY(hist_.ensure(np * (size_t)(prm_.max_iterations + 1) * 12 * sizeof(float)));
   if (prm_.anderson_memory > 0) RTR_TRY(aa_.ensure(np * (size_t)kAndersonStride * sizeof(double)));
+  // search seeds: one slot per problem and source point (pt_off is 32-bit)
+  if (pt_total_ > 0 && pt_total_ <= 0xffffffffull) RTR_TRY(nn_seed_.ensure(pt_total_ * 4));
   if (np) RTR_HIP(hipMemcpy(prob_desc_.p, probs_.data(), np * sizeof(Problem), hipMemcpyHostToDevice));
   if (!items_.empty())
     RTR_HIP(hipMemcpy(items_buf_.p, items_.data(), items_.size() * sizeof(uint2), hipMemcpyHostToDevice));
@@ -400,6 +403,7 @@ DevEngine Engine::dev_view() const {
   e.guesses = guesses_.as<double>();
   e.hist = hist_.as<float>();
   e.aa = prm_.anderson_memory > 0 ? aa_.as<double>() : nullptr;
+  e.nn_seed = (pt_total_ > 0 && pt_total_ <= 0xffffffffull) ? nn_seed_.as<uint32_t>() : nullptr;
   e.items = items_buf_.as<uint2>();
   e.n_items = (int)items_.size();
   e.item_pts = tune_.item_pts;
@@ -411,7 +415,8 @@ DevEngine Engine::dev_view() const {
   // their own: spinning workgroups poll the flag while others claim
   e.problems_done = ctr;
   e.error = reinterpret_cast<int*>(ctr + 32);
-  e.claim = ctr + 64;
+  e.queries_searched = reinterpret_cast<unsigned long long*>(ctr + 64);
+  e.claim = ctr + 96;
   e.claim_stride = claim_stride();
   for (int x = 0; x <= kXcds; ++x) e.xcd_off[x] = xcd_off_[x];
   e.done = done_.as<uint32_t>();
@@ -485,7 +490,7 @@ int Engine::align(const double* guesses, rtr_result* results) {
   const size_t np = probs_.size();
   const size_t gbytes = np * 16 * sizeof(double);
   const size_t sbytes = np * sizeof(ProblemState);
-  RTR_TRY(ensure_pinned(gbytes + sbytes + 256));
+  RTR_TRY(ensure_pinned(gbytes + sbytes + 512));
   double* g = static_cast<double*>(pin_);
   for (size_t p = 0; p < np; ++p)
     for (int k = 0; k < 16; ++k) g[16 * p + k] = guesses ? guesses[16 * p + k] : ((k % 5 == 0) ? 1.0 : 0.0);
@@ -529,7 +534,7 @@ int Engine::align(const double* guesses, rtr_result* results) {
   ProblemState* hs = reinterpret_cast<ProblemState*>(static_cast<char*>(pin_) + gbytes);
   uint32_t* hc = reinterpret_cast<uint32_t*>(static_cast<char*>(pin_) + gbytes + sbytes);
   RTR_HIP(hipMemcpyAsync(hs, state_.p, sbytes, hipMemcpyDeviceToHost, stream_));
-  RTR_HIP(hipMemcpyAsync(hc, counters_.p, 33 * 4, hipMemcpyDeviceToHost, stream_));  // done count .. error flag
+  RTR_HIP(hipMemcpyAsync(hc, counters_.p, 66 * 4, hipMemcpyDeviceToHost, stream_));  // done count .. searches
   if (profiling_) RTR_HIP(hipEventRecord(ev_[3], stream_));
   RTR_HIP(hipStreamSynchronize(stream_));
   if (profiling_) {
@@ -539,6 +544,11 @@ int Engine::align(const double* guesses, rtr_result* results) {
     RTR_HIP(hipEventElapsedTime(&timing_.ms_total, ev_[0], ev_[3]));
   }
   timing_.icp_launches = 1;
+  {
+    unsigned long long ns;
+    memcpy(&ns, hc + 64, sizeof(ns));
+    timing_.queries_searched = (int64_t)ns;
+  }
   int64_t it_total = 0;
   for (size_t p = 0; p < np; ++p) {
     it_total += hs[p].iter;
@@ -741,6 +751,71 @@ int Engine::history(int prob, float* rows12, int cap, int* n_out) {
   RTR_HIP(hipMemcpyAsync(rows12, hist_.as<float>() + off, (size_t)n * 12 * sizeof(float), hipMemcpyDeviceToHost,
                          stream_));
   RTR_HIP(hipStreamSynchronize(stream_));
+  return RTR_OK;
+}
+
+// The correspondences of the last iteration the latest align ran for `prob`,
+// as its search found them: each searched source point leaves the slot of its
+// nearest target point (the next iteration's seed).  The distance is taken
+// again here, in the kernel's float arithmetic, from the transform rows that
+// iteration queried with.
+int Engine::last_correspondences(int prob, int32_t* idx, float* d2) {
+  RTR_HIP(hipSetDevice(device_));
+  if (prob < 0 || prob >= (int)probs_.size()) {
+    set_error("problem %d out of range", prob);
+    return RTR_ERR_INVALID_ARG;
+  }
+  if (!results_valid_) {
+    set_error("last correspondences need a completed align of the current clouds and problems");
+    return RTR_ERR_NOT_READY;
+  }
+  if (tune_.use_tile || !nn_seed_.p || pt_total_ > 0xffffffffull) {
+    set_error("last correspondences are recorded by the global-walk search only (use_tile 0)");
+    return RTR_ERR_NOT_READY;
+  }
+  ProblemState st;
+  RTR_HIP(hipMemcpyAsync(&st, state_.as<ProblemState>() + prob, sizeof(st), hipMemcpyDeviceToHost, stream_));
+  const Problem pr = probs_[prob];
+  TargetDesc td;
+  RTR_HIP(hipMemcpyAsync(&td, tgt_desc_.as<TargetDesc>() + pr.tgt, sizeof(td), hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  if (st.iter < 1) {
+    set_error("problem %d ran no iteration", prob);
+    return RTR_ERR_NOT_READY;
+  }
+  const SourceDesc& sd = srcs_[pr.src];
+  const size_t n = sd.n;
+  std::vector<uint32_t> slot(n), perm(n);
+  std::vector<float> x(n), y(n), z(n), rows(12);
+  std::vector<float4> tp(td.n_valid);
+  RTR_HIP(hipMemcpyAsync(slot.data(), nn_seed_.as<uint32_t>() + pr.pt_off, n * 4, hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipMemcpyAsync(perm.data(), perm_.as<uint32_t>() + sd.off, n * 4, hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipMemcpyAsync(x.data(), sx_.as<float>() + sd.off, n * 4, hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipMemcpyAsync(y.data(), sy_.as<float>() + sd.off, n * 4, hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipMemcpyAsync(z.data(), sz_.as<float>() + sd.off, n * 4, hipMemcpyDeviceToHost, stream_));
+  if (td.n_valid)
+    RTR_HIP(hipMemcpyAsync(tp.data(), tgt_sorted_.as<float4>() + td.raw_off, (size_t)td.n_valid * sizeof(float4),
+                           hipMemcpyDeviceToHost, stream_));
+  const size_t hoff = ((size_t)prob * (prm_.max_iterations + 1) + (size_t)(st.iter - 1)) * 12;
+  RTR_HIP(hipMemcpyAsync(rows.data(), hist_.as<float>() + hoff, 12 * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  const float r2f = dev_view().prm.r2f;
+  for (size_t j = 0; j < n; ++j) {
+    const uint32_t o = perm[j];
+    idx[o] = -1;
+    d2[o] = INFINITY;
+    const uint32_t s = slot[j] - td.raw_off;
+    if (s >= td.n_valid) continue;
+    float qx, qy, qz;
+    apply_f32(rows.data(), x[j], y[j], z[j], qx, qy, qz);
+    const float4 t = tp[s];
+    const float d = d2_f32(qx, qy, qz, t.x, t.y, t.z);
+    if (!(d <= r2f)) continue;
+    int32_t i;
+    memcpy(&i, &t.w, 4);
+    idx[o] = i;
+    d2[o] = d;
+  }
   return RTR_OK;
 }
 

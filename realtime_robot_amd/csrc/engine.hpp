@@ -85,6 +85,7 @@ class Engine {
   int judge(int k, int max_pts, double max_curv, double* plane_rmse, double* inlier_frac, double* planar_frac);
   int grid_info(int tgt, float* cell, int dims[3], float anchor[3]);
   int history(int prob, float* rows12, int cap, int* n_out);
+  int last_correspondences(int prob, int32_t* idx, float* d2);
   int target_normals(int tgt, float* out);
   const rtr_timing& timing() const { return timing_; }
   int n_problems() const { return (int)probs_.size(); }
@@ -129,12 +130,13 @@ class Engine {
   uint32_t n_cells_ = 0, n_bins_ = 0;
   int max_cells_ = 0;
   uint32_t xcd_off_[kXcds + 1] = {0};
-  // [0] problems done, [1] error, [8 + pass * kXcds + x] claim counters
+  // [0] problems done, [32] error, [64..65] searches (a 64-bit count),
+  // [96 + (pass * kXcds + x) * stride] claim counters
   int claim_stride() const { return prm_.max_iterations <= 256 ? 32 : 1; }
   // the 8-deep search kernel (2 waves per SIMD) serves launches tuned to at
   // most 2 workgroups per CU, where a third wave would not be resident
   bool deep_gathers() const { return tune_.blocks_per_cu == 1 || tune_.blocks_per_cu == 2; }
-  size_t counter_words() const { return 64 + (size_t)prm_.max_iterations * kXcds * claim_stride(); }
+  size_t counter_words() const { return 96 + (size_t)prm_.max_iterations * kXcds * claim_stride(); }
 
   // device arena
   DevBuf tgt_raw_, tgt_nrm_raw_, tgt_sorted_, tgt_nrm_sorted_, cell_count_, cell_start_, cell_id_;
@@ -142,7 +144,7 @@ class Engine {
   DevBuf tgt_desc_;
   DevBuf src_raw_, sx_, sy_, sz_, perm_, bin_count_, bin_start_, bin_id_, src_desc_;
   DevBuf prob_desc_, state_, guesses_, items_buf_, partials_, done_, counters_, scan_tmp_;
-  DevBuf out_i32_, out_f32_, out_T_, hist_, aa_;
+  DevBuf out_i32_, out_f32_, out_T_, hist_, aa_, nn_seed_;
   // pinned host staging
   void* pin_ = nullptr;
   size_t pin_bytes_ = 0;

@@ -46,6 +46,8 @@ struct DevEngine {
   const double* guesses;     // n_prob * 16 (column-major)
   float* hist;               // n_prob * (max_iter + 1) * 12: float rows of T used by each iteration
   double* aa;                // n_prob * kAndersonStride: acceleration state (null when off)
+  uint32_t* nn_seed;         // per problem and sorted source point (at Problem::pt_off): cell-sorted
+                             // slot of its last nearest neighbour, the next search's seed (nullable)
   const uint2* items;        // (problem, first sorted source point)
   int n_items;
   int item_pts;              // points per item
@@ -58,6 +60,7 @@ struct DevEngine {
   uint32_t* done;            // per problem: items finished so far
   uint32_t* problems_done;
   int* error;                // nonzero: a bounded wait expired
+  unsigned long long* queries_searched;  // nearest-neighbour searches of the launch
   uint32_t* scan_tmp;        // block sums for the scans
   IcpParams prm;
 };

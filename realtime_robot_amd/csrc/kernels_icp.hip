@@ -67,6 +67,7 @@ __device__ __forceinline__ GridView grid_of(const DevEngine& e, const TargetDesc
   g.dy = td.dy;
   g.dz = td.dz;
   g.n_pts = td.n_valid;
+  g.pt_lo = td.raw_off;
   g.ox = td.ox;
   g.oy = td.oy;
   g.oz = td.oz;
@@ -145,6 +146,7 @@ struct IcpShared {
   uint8_t qcls[kChunk];   // per query of the chunk: 0 no correspondence possible, 1 block walk, 2.. rings only
   uint8_t qd[kChunk];     // its cell's distance-field value
   uint16_t qlist[kChunk]; // the chunk's queries to walk by class, each class in query order
+  uint32_t qseed[kChunk]; // its search seed: the last iteration's nearest-neighbour slot (or ~0)
   int nlist;
   float red[kWaves][NS][2];
   double dred[kBlock / NSUM_MAX][NSUM_MAX];
@@ -156,6 +158,7 @@ struct IcpShared {
   double aa[kAndersonStride];
   float m[12];
   float anchor[3];
+  uint32_t n_search;      // queries this workgroup searched
   int item;
   int run;
   int stop;
@@ -185,6 +188,10 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) 
   return inc - v;
 }
 
+// "No point yet": above every real key, with a squared distance of +inf.
+constexpr uint64_t kNoKey = 0x7f800000ffffffffull;
+
+
 // Nearest-neighbour state: packed (d2, index) key and where the winner lives
 // (an LDS tile index, or a cell-sorted slot with the top bit set).
 template <int D>
@@ -193,6 +200,14 @@ struct GlobalVis {
   float qx, qy, qz;
   uint64_t key;
   uint32_t where;
+  __device__ __forceinline__ void init(const float4* p, float x, float y, float z) {
+    pts = p;
+    qx = x;
+    qy = y;
+    qz = z;
+    key = kNoKey;
+    where = 0u;
+  }
   __device__ __forceinline__ void take(const float4& t, uint32_t j) {
     const uint64_t k = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), __float_as_int(t.w));
     if (k < key) {
@@ -230,39 +245,59 @@ struct GlobalVis {
     }
     for (; j < e; ++j, ++q) take(*q, j);
   }
-  __device__ __forceinline__ float bound() const { return key == ~0ull ? 3.4e38f : key_d2(key); }
+  __device__ __forceinline__ float bound() const { return key_d2(key); }
+  // Start from target point s (cell-sorted slot): its distance is an upper
+  // bound on the nearest one, and the walk's pruning (strict comparisons on
+  // the packed key) still finds the exact winner, ties to the lowest index.
+  __device__ __forceinline__ void seed(uint32_t s) {
+    const float4 t = pts[s];
+    key = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), __float_as_int(t.w));
+    where = s | 0x80000000u;
+  }
 };
 
-// The 3x3x3 block straight from global memory: the 18 row bounds are loaded
-// together before any row is scanned.
+// The 3x3x3 block straight from global memory.  Each x-row of three cells is
+// first cut to the cells whose box lies within the bound known before the walk
+// (a seed's distance, else r2f): a row is dropped whole, or loses its end
+// cells.  The row bounds of what is left are loaded together before any row is
+// scanned; rows are then scanned own row first, each skipped when its box lies
+// beyond the best so far (or beyond r2f).
 template <int D>
 __device__ __forceinline__ void walk_block_global(const GridView& g, float qx, float qy, float qz, int cx, int cy,
                                                   int cz, float r2f, GlobalVis<D>& vis) {
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < g.dx - 1 ? cx + 1 : g.dx - 1;
-  // all 18 row bounds in flight together; rows are then scanned own row
-  // first, skipping any whose box lies beyond the best so far (or beyond r2f)
-  uint32_t rb[9], re[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    const int y = cy + kBlockRowOrder[k][0], z = cz + kBlockRowOrder[k][1];
-    if (z >= 0 && z < g.dz && y >= 0 && y < g.dy) {
-      const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
-      rb[k] = g.cell_start[row + x0];
-      re[k] = g.cell_start[row + x1 + 1];
-    } else {
-      rb[k] = re[k] = 0u;
-    }
-  }
-  // separable squared gaps of the side rows (the x extent is shared)
+  // separable squared gaps: the x extent of the whole row and of its end
+  // cells, the y and z offsets of the side rows
   const float ax = slab_gap(qx, g.ox, g.h, g.margin, x0, x1);
+  const float axl = slab_gap(qx, g.ox, g.h, g.margin, x0, x0);
+  const float axr = slab_gap(qx, g.ox, g.h, g.margin, x1, x1);
   const float ay0 = slab_gap(qy, g.oy, g.h, g.margin, cy - 1, cy - 1);
   const float ay1 = slab_gap(qy, g.oy, g.h, g.margin, cy + 1, cy + 1);
   const float az0 = slab_gap(qz, g.oz, g.h, g.margin, cz - 1, cz - 1);
   const float az1 = slab_gap(qz, g.oz, g.h, g.margin, cz + 1, cz + 1);
-  const float gx2 = ax * ax;
+  const float gx2 = ax * ax, gxl2 = axl * axl, gxr2 = axr * axr;
   const float gy2[3] = {ay0 * ay0, 0.f, ay1 * ay1};
   const float gz2[3] = {az0 * az0, 0.f, az1 * az1};
+  const float b0 = vis.bound() < r2f ? vis.bound() : r2f;
+  uint32_t rb[9], re[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int dy = kBlockRowOrder[k][0], dz = kBlockRowOrder[k][1];
+    const int y = cy + dy, z = cz + dz;
+    rb[k] = re[k] = 0u;
+    if (z >= 0 && z < g.dz && y >= 0 && y < g.dy) {
+      if (!(lb2_of(gx2, gy2[dy + 1], gz2[dz + 1]) > b0)) {
+        // the own cell is the row's nearest (the query lies in its slab, or
+        // beyond the grid's edge next to it), so what is kept is contiguous
+        const int xa = lb2_of(gxl2, gy2[dy + 1], gz2[dz + 1]) > b0 ? cx : x0;
+        const int xb = lb2_of(gxr2, gy2[dy + 1], gz2[dz + 1]) > b0 ? cx : x1;
+        const uint32_t row = ((uint32_t)z * g.dy + y) * g.dx;
+        rb[k] = g.cell_start[row + xa];
+        re[k] = g.cell_start[row + xb + 1];
+      }
+    }
+  }
   vis.scan(rb[0], re[0]);
 #pragma unroll
   for (int k = 1; k < 9; ++k) {
@@ -365,6 +400,26 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
   }
 }
 
+// One correspondence (query q, target point c at cell-sorted slot, float d2)
+// into the lane's compensated moment sums.
+template <int MODE, int NS>
+__device__ __forceinline__ void add_pair(const DevEngine& e, float qx, float qy, float qz, const float4& c,
+                                         uint32_t slot, float d2, float ax, float ay, float az, float* acc_s,
+                                         float* acc_c) {
+  float nx = 0.f, ny = 0.f, nz = 0.f;
+  if (MODE == RTR_MODE_POINT_TO_PLANE) {
+    const float4 nn = e.tgt_nrm_sorted[slot];
+    nx = nn.x;
+    ny = nn.y;
+    nz = nn.z;
+  }
+  float terms[NSUM_MAX];
+  point_terms(MODE, qx - ax, qy - ay, qz - az, c.x - ax, c.y - ay, c.z - az, nx, ny, nz, d2, terms);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+}
+
+
 template <int MODE, int NS, bool TILE, class TileT, int DEPTH>
 __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS, TileT>& sh) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -383,8 +438,18 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   float acc_s[NS], acc_c[NS];
 #pragma unroll
   for (int k = 0; k < NS; ++k) acc_s[k] = acc_c[k] = 0.f;
+  uint32_t n_search = 0u;
   const uint32_t end = min(itm.y + (uint32_t)e.item_pts, sd.n);
   constexpr bool kCompact = !TILE && DEPTH == 4;
+  // Search seeds (global walks only): every searched query stores its
+  // nearest neighbour's slot, and the next iteration starts from it.  The
+  // transform moves little between iterations, so the seed's distance is
+  // close to the nearest one: it trims the block's rows before their bounds
+  // are loaded and stops the outer walk early.  Any slot of this target gives
+  // a valid bound, so a seed is only range-checked, and the first iteration
+  // of a launch (slots from an earlier launch) starts without one.
+  uint32_t* const seeds = !TILE && e.nn_seed ? e.nn_seed + pr.pt_off : nullptr;
+  const bool seeded = seeds && pass > 0;
   if (kCompact) {
     // Compacted walk (the throughput variant).  Queries that cannot have a
     // correspondence (distance field beyond r_max, NaN) take no lane, and the
@@ -409,6 +474,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         }
         sh.qcls[o] = cls;
         sh.qd[o] = dv;
+        sh.qseed[o] = seeded && cls ? seeds[chunk + o] : ~0u;
       }
       __syncthreads();
       if (w == 0) {
@@ -435,30 +501,17 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
         query_cell(g, qx, qy, qz, cx, cy, cz);
         const int D = sh.qd[o];
+        ++n_search;
         GlobalVis<DEPTH> vis;
-        vis.pts = g.pts;
-        vis.qx = qx;
-        vis.qy = qy;
-        vis.qz = qz;
-        vis.key = ~0ull;
-        vis.where = 0u;
+        vis.init(g.pts, qx, qy, qz);
+        const uint32_t s0 = sh.qseed[o];
+        if (s0 - g.pt_lo < g.n_pts) vis.seed(s0);
         if (D < 2) walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
         walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, D >= 2 ? D - 1 : 1);
-        if (vis.key != ~0ull && key_d2(vis.key) <= r2f) {
+        if (seeds && vis.key != kNoKey) seeds[chunk + o] = vis.where & 0x7fffffffu;
+        if (key_d2(vis.key) <= r2f) {
           const uint32_t slot = vis.where & 0x7fffffffu;
-          const float4 c = g.pts[slot];
-          float nx = 0.f, ny = 0.f, nz = 0.f;
-          if (MODE == RTR_MODE_POINT_TO_PLANE) {
-            const float4 nn = e.tgt_nrm_sorted[slot];
-            nx = nn.x;
-            ny = nn.y;
-            nz = nn.z;
-          }
-          float terms[NSUM_MAX];
-          point_terms(MODE, qx - ax, qy - ay, qz - az, c.x - ax, c.y - ay, c.z - az, nx, ny, nz,
-                      key_d2(vis.key), terms);
-#pragma unroll
-          for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+          add_pair<MODE, NS>(e, qx, qy, qz, g.pts[slot], slot, key_d2(vis.key), ax, ay, az, acc_s, acc_c);
         }
       }
       __syncthreads();  // the next chunk rewrites the lists
@@ -469,10 +522,12 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
     float qx = 0.f, qy = 0.f, qz = 0.f;
     int cx = 0, cy = 0, cz = 0;
     bool act = false;
+    uint32_t s0 = ~0u;
     if (j < end) {
       const uint32_t sj = sd.off + j;
       apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
       act = query_cell(g, qx, qy, qz, cx, cy, cz);
+      if (seeded) s0 = seeds[j];
     }
     // distance field: cells within D - 1 rings of the query's cell are empty
     bool blk = act;  // the lane walks its 3x3x3 block
@@ -487,13 +542,10 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         k0 = D - 1;
       }
     }
+    if (act) ++n_search;
     GlobalVis<DEPTH> vis;
-    vis.pts = g.pts;
-    vis.qx = qx;
-    vis.qy = qy;
-    vis.qz = qz;
-    vis.key = ~0ull;
-    vis.where = 0u;
+    vis.init(g.pts, qx, qy, qz);
+    if (act && s0 - g.pt_lo < g.n_pts) vis.seed(s0);
     bool staged = false;
     int X0 = 0, Y0 = 0, Z0 = 0, NY = 1, W1 = 1, nrows = 0;
     if (TILE && e.use_tile && __ballot(blk) != 0ull) {
@@ -580,7 +632,8 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
       }
       walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
-      if (vis.key != ~0ull && key_d2(vis.key) <= r2f) {
+      if (seeds && vis.key != kNoKey) seeds[j] = vis.where & 0x7fffffffu;
+      if (key_d2(vis.key) <= r2f) {
         float4 c;
         uint32_t slot = 0u;
         if (vis.where & 0x80000000u) {
@@ -610,6 +663,10 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
     // the next round rewrites this wave's tile
     __builtin_amdgcn_wave_barrier();
   }
+  // search statistics: per wave, then the workgroup's LDS totals
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n_search += __shfl_xor(n_search, off, 64);
+  if (lane == 0) atomicAdd(&sh.n_search, n_search);
   // wave reduction with compensation, then across the four waves
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
@@ -659,6 +716,7 @@ __global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH =
   using TileT = typename std::conditional<TILE, WaveTile, WaveTileNone>::type;
   __shared__ IcpShared<NS, TileT> sh;
   const int tid = threadIdx.x;
+  if (tid == 0) sh.n_search = 0u;
   for (int pass = 0; pass < e.prm.max_iter; ++pass) {
     if (tid == 0)
       sh.stop = (load_acquire(e.problems_done) >= (uint32_t)e.n_prob) ||
@@ -730,6 +788,8 @@ __global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH =
       __syncthreads();
     }
   }
+  __syncthreads();
+  if (tid == 0) atomicAdd(e.queries_searched, (unsigned long long)sh.n_search);
 }
 
 template <int MODE, bool TILE, int D>

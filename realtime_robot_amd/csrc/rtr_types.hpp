@@ -29,6 +29,7 @@ struct GridView {
   const uint32_t* cell_start;  // dx*dy*dz + 1 global positions into pts
   int dx, dy, dz;
   uint32_t n_pts;
+  uint32_t pt_lo;              // the grid's points are pts[pt_lo, pt_lo + n_pts)
   float ox, oy, oz;
   float h, inv_h, margin;
 };

@@ -226,6 +226,7 @@ class BatchRegistration:
         L.check(L.lib.rtr_batch_set_problems(self._h, len(s), L.ptr(s), L.ptr(t)))
         self.n_problems = len(s)
         self._max_ids = (int(s.max(initial=-1)), int(t.max(initial=-1)))
+        self._prob_src = s.copy()
 
     def set_params(self, p: L.Params):
         L.check(L.lib.rtr_batch_set_params(self._h, C.byref(p)))
@@ -288,6 +289,17 @@ class BatchRegistration:
         n = C.c_int()
         L.check(L.lib.rtr_batch_get_history(self._h, int(prob), L.ptr(buf), len(buf), C.byref(n)))
         return buf[:n.value].reshape(-1, 3, 4).copy()
+
+    def last_correspondences(self, prob: int):
+        """(idx int32, d2 float32) per source point: the correspondences the
+        last iteration of problem `prob` used (-1 / inf: none within range)."""
+        if not 0 <= prob < self.n_problems:
+            raise IndexError(f"problem {prob} out of range")
+        n = self._ns[int(self._prob_src[prob])]
+        idx = np.empty(n, np.int32)
+        d2 = np.empty(n, np.float32)
+        L.check(L.lib.rtr_batch_get_last_correspondences(self._h, int(prob), L.ptr(idx), L.ptr(d2)))
+        return idx, d2
 
     def fitness(self, max_range=DBL_MAX):
         out = np.empty(self.n_problems)

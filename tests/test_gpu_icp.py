@@ -12,7 +12,7 @@ import oracle
 import realtime_robot_amd as R
 from helpers import golden, rot_err, trans_err
 from pcd_np import read_xyz
-from realtime_robot_amd import synth
+from realtime_robot_amd import multistart, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -295,3 +295,45 @@ def test_cell_tuning_change_rebuilds_reused_grids():
     b.set_params(R.make_params(max_iterations=20, max_correspondence_distance=0.05, transformation_epsilon=1e-8))
     b.align(gts)
     assert b.grid(0)[0] == cell1
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_seeded_search_last_correspondences_bitexact(mode):
+    """After the first iteration each query's search starts from its last
+    nearest neighbour (a seed that trims the walk).  The correspondences of the
+    last iteration a launch ran, as the seeded search found them, equal brute
+    force at the transform that iteration queried with, bit for bit: near and
+    far (multi-start) poses, launches stopped after 2, 4 and 7 iterations, both
+    search variants, point-to-point and point-to-plane."""
+    srcs, tgts, gts = synth.config3_batch(3, seed_offset=610, ns_range=(2000, 6000), nt_range=(8000, 20000),
+                                          max_deg=10.0, max_t=0.05)
+    starts = [multistart.so3_start_poses(s.astype(np.float64).mean(0), t.astype(np.float64).mean(0), 4)
+              for s, t in zip(srcs, tgts)]
+    guesses = [T for k in range(3) for T in [gts[k]] + list(starts[k][:2])]
+    ids = np.repeat(np.arange(3), 3)
+    for bpc in ((0, 1) if mode == 0 else (0,)):     # 4-deep compacted walk, 8-deep walk
+        b = R.BatchRegistration()
+        b.set_clouds(srcs, tgts)
+        if mode == 1:
+            b.compute_target_normals(20)
+        b.set_problems(ids, ids)
+        b.set_tuning(item_pts=256, blocks_per_cu=bpc, use_tile=0)
+        for iters in (2, 4, 7):
+            b.set_params(R.make_params(max_iterations=iters, max_correspondence_distance=0.05,
+                                       transformation_epsilon=0.0, mode=mode))
+            b.set_profiling(True)
+            res = b.align(guesses)
+            searched = b.timing()["queries_searched"]
+            assert 0 < searched <= sum(len(srcs[i]) for i in ids) * iters
+            for p in range(len(ids)):
+                it = res[p]["iterations"]
+                assert it >= 2
+                rows = b.history(p, iters)[it - 1]
+                T = np.eye(4)
+                T[:3] = rows
+                idx, d2 = b.last_correspondences(p)
+                q = oracle.transform(T, srcs[ids[p]])
+                i_ref, d_ref = oracle.nn_brute(q, tgts[ids[p]], 0.05)
+                assert np.array_equal(idx, i_ref), f"{iters} iterations, problem {p}: {np.sum(idx != i_ref)} mismatches"
+                assert np.array_equal(d2, d_ref)
+            print(mode, bpc, iters, searched)
