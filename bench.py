@@ -172,6 +172,10 @@ def main():
     ap.add_argument("--hypotheses", type=int, default=8)
     ap.add_argument("--refine-anderson", type=int, default=5,
                     help="Anderson memory of the learned path's refine launch (0: plain iteration)")
+    ap.add_argument("--fallback-anderson", type=int, default=0,
+                    help="Anderson memory of the learned path's fallback multi-start launch (0: plain iteration)")
+    ap.add_argument("--judge-points", type=int, default=1024,
+                    help="source points sampled per pair by the registration judge")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
                     help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile'")
@@ -214,7 +218,8 @@ def main():
 
     weights, meta = learned.load_weights()
     lr = learned.LearnedRegistration(weights, params, n_starts=args.starts, hypotheses=args.hypotheses,
-                                     max_pairs=n, refine_anderson=args.refine_anderson)
+                                     max_pairs=n, refine_anderson=args.refine_anderson,
+                                     fallback_anderson=args.fallback_anderson, judge_points=args.judge_points)
     lr.set_clouds(srcs, tgts, cs, ct)
     ms = None
     if args.leg in ("both", "plain"):
@@ -277,8 +282,10 @@ def main():
     if args.leg == "learned":
         # learned steps alone (kernel profiling of the whole step)
         t_l, res_l = timed(lr.align, args.steps, args.warmup)
+        st = lr.stats
         log(f"learned steps done: {t_l * 1e3:.1f} ms/step, {int(n_ok(res_l).sum())}/{n} within 1 mm, "
-            f"{lr.stats['n_fallback']} fallbacks")
+            f"{st['n_fallback']} fallbacks; build {st['ms_build']:.2f} propose {st['ms_propose']:.2f} refine "
+            f"{st['ms_refine']:.2f} judge {st['ms_judge']:.2f} fallback {st['ms_fallback']:.2f} ms")
         return
 
     if args.leg == "refine":
@@ -360,6 +367,7 @@ def main():
                                f"eps {EPS}; counted when within {OK_RMSE * 1e3:.0f} mm RMSE of the reference pose",
                    "pairs": n, "hypotheses": args.hypotheses, "starts": args.starts,
                    "refine_anderson_memory": args.refine_anderson,
+                   "fallback_anderson_memory": args.fallback_anderson, "judge_points": args.judge_points,
                    "tuning": list(tuned) if tuned else None,
                    "tuning_refine": list(tuned_refine) if tuned_refine else None,
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],

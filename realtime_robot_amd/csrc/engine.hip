@@ -69,7 +69,7 @@ Engine::~Engine() {
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&tgt_raw_, &tgt_nrm_raw_, &tgt_sorted_, &tgt_nrm_sorted_, &cell_count_,
-                    &cell_start_, &cell_id_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
+                    &cell_start_, &cell_id_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &bbox_tmp_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
                     &bin_count_, &bin_start_, &bin_id_, &src_desc_, &prob_desc_, &state_,
                     &guesses_, &items_buf_, &partials_, &done_, &counters_, &scan_tmp_,
                     &out_i32_, &out_f32_, &out_T_, &hist_, &aa_, &nn_seed_};
@@ -277,6 +277,7 @@ int Engine::layout_targets() {
   RTR_TRY(scan_tmp_.ensure(scan_words * 4));
   RTR_TRY(tgt_desc_.ensure(tgts_.size() * sizeof(TargetDesc)));
   RTR_TRY(src_desc_.ensure(srcs_.size() * sizeof(SourceDesc)));
+  RTR_TRY(bbox_tmp_.ensure((tgts_.size() + srcs_.size()) * 8 * sizeof(uint32_t)));
   if (!tgts_.empty())
     RTR_HIP(hipMemcpy(tgt_desc_.p, tgts_.data(), tgts_.size() * sizeof(TargetDesc), hipMemcpyHostToDevice));
   if (!srcs_.empty())
@@ -421,6 +422,7 @@ DevEngine Engine::dev_view() const {
   for (int x = 0; x <= kXcds; ++x) e.xcd_off[x] = xcd_off_[x];
   e.done = done_.as<uint32_t>();
   e.scan_tmp = scan_tmp_.as<uint32_t>();
+  e.bbox_tmp = bbox_tmp_.as<uint32_t>();
   e.prm.max_iter = prm_.max_iterations;
   e.prm.min_corr = prm_.min_correspondences > 0 ? prm_.min_correspondences : 3;
   e.prm.mode = prm_.mode;

@@ -62,6 +62,7 @@ struct DevEngine {
   int* error;                // nonzero: a bounded wait expired
   unsigned long long* queries_searched;  // nearest-neighbour searches of the launch
   uint32_t* scan_tmp;        // block sums for the scans
+  uint32_t* bbox_tmp;        // per target, then per source: 8 words of bounding-box partials
   IcpParams prm;
 };
 

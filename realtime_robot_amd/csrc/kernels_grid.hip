@@ -130,14 +130,36 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
   return v;
 }
 
-// bbox of finite points of one cloud: lo[3], hi[3], count -> shared result
-__device__ void block_bbox(const float* xyz, uint32_t n, float* lo, float* hi, uint32_t* cnt) {
-  __shared__ float s_lo[kWaves][3], s_hi[kWaves][3];
-  __shared__ uint32_t s_cnt[kWaves];
+
+// Bounding boxes of many clouds at once: each cloud is split over many
+// workgroups, whose wave minima / maxima (as order-preserving integer keys)
+// and finite-point counts meet in 8 words per cloud through atomics.  Min and
+// max do not depend on the order they are taken in, so the box is the same
+// as a serial pass.
+__device__ __forceinline__ uint32_t ord_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_val(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// words per cloud: lo key x3, hi key x3, finite count, unused
+__global__ void k_bbox_init(uint32_t* bb, uint32_t n_words) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_words) bb[i] = (i & 7u) < 3u ? 0xffffffffu : 0u;
+}
+
+template <bool TGT>
+__global__ void k_cloud_bbox(DevEngine e) {
+  const uint32_t c = blockIdx.y;
+  const float* xyz = TGT ? e.tgt_raw + 3 * (size_t)e.tgts[c].raw_off : e.src_raw + 3 * (size_t)e.srcs[c].off;
+  const uint32_t n = TGT ? e.tgts[c].n : e.srcs[c].n;
+  uint32_t* bb = e.bbox_tmp + 8 * (size_t)(TGT ? c : e.n_tgt + c);
   float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, h[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
-  uint32_t c = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    float x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
+  uint32_t cnt = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
     if (!(isfinite(x) && isfinite(y) && isfinite(z))) continue;
     l[0] = fminf(l[0], x);
     l[1] = fminf(l[1], y);
@@ -145,37 +167,28 @@ __device__ void block_bbox(const float* xyz, uint32_t n, float* lo, float* hi, u
     h[0] = fmaxf(h[0], x);
     h[1] = fmaxf(h[1], y);
     h[2] = fmaxf(h[2], z);
-    ++c;
+    ++cnt;
   }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int a = 0; a < 3; ++a) {
     l[a] = wave_min(l[a]);
     h[a] = wave_max(h[a]);
   }
-  c = wave_sum_u32(c);
-  if (lane == 0) {
+  cnt = wave_sum_u32(cnt);
+  if ((threadIdx.x & 63) == 0 && cnt) {
     for (int a = 0; a < 3; ++a) {
-      s_lo[w][a] = l[a];
-      s_hi[w][a] = h[a];
+      atomicMin(&bb[a], ord_key(l[a]));
+      atomicMax(&bb[3 + a], ord_key(h[a]));
     }
-    s_cnt[w] = c;
+    atomicAdd(&bb[6], cnt);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = s_lo[0][a];
-      hi[a] = s_hi[0][a];
-    }
-    *cnt = s_cnt[0];
-    for (int i = 1; i < kWaves; ++i) {
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = fminf(lo[a], s_lo[i][a]);
-        hi[a] = fmaxf(hi[a], s_hi[i][a]);
-      }
-      *cnt += s_cnt[i];
-    }
+}
+
+__device__ __forceinline__ void bbox_of(const uint32_t* bb, float* lo, float* hi, uint32_t* cnt) {
+  *cnt = bb[6];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = *cnt ? ord_val(bb[a]) : 3.0e38f;
+    hi[a] = *cnt ? ord_val(bb[3 + a]) : -3.0e38f;
   }
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -251,15 +264,16 @@ __device__ void choose_dist_cap(TargetDesc& t, float r_max) {
   dist_caps(r_max, t.h, t.margin, &t.skip_ring, &t.dt_cap);
 }
 
-__global__ void k_tgt_bbox(DevEngine e) {
-  TargetDesc& t = e.tgts[blockIdx.x];
-  __shared__ float lo[3], hi[3];
-  __shared__ uint32_t cnt;
-  block_bbox(e.tgt_raw + 3 * (size_t)t.raw_off, t.n, lo, hi, &cnt);
-  if (threadIdx.x == 0) {
-    choose_grid(t, lo, hi, cnt);
-    choose_dist_cap(t, e.r_max);
-  }
+// one thread per target, after k_cloud_bbox<true>
+__global__ void k_tgt_choose(DevEngine e) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= e.n_tgt) return;
+  TargetDesc& t = e.tgts[j];
+  float lo[3], hi[3];
+  uint32_t cnt;
+  bbox_of(e.bbox_tmp + 8 * (size_t)j, lo, hi, &cnt);
+  choose_grid(t, lo, hi, cnt);
+  choose_dist_cap(t, e.r_max);
 }
 
 __device__ __forceinline__ bool cell_of(const TargetDesc& t, float x, float y, float z,
@@ -338,6 +352,12 @@ __global__ void k_tgt_dist(DevEngine e, const uint8_t* src, uint8_t* dst) {
   }
 }
 
+// workgroups per cloud of a bounding-box pass: about 8 points per thread
+static unsigned bbox_blocks_for(int max_pts) {
+  unsigned b = (unsigned)((max_pts + 8 * kBlock - 1) / (8 * kBlock));
+  return b < 1 ? 1 : (b > 128 ? 128 : b);
+}
+
 static unsigned blocks_for(int max_pts) {
   unsigned b = (unsigned)((max_pts + kBlock - 1) / kBlock);
   if (b < 1) b = 1;
@@ -348,8 +368,11 @@ static unsigned blocks_for(int max_pts) {
 void launch_target_grid(const DevEngine& e, int max_tgt_pts, int max_cells, hipStream_t s) {
   if (e.n_tgt == 0) return;
   launch_fill_u32(e.cell_count, 0u, e.n_cells_total, s);
-  hipLaunchKernelGGL(k_tgt_bbox, dim3(e.n_tgt), dim3(kBlock), 0, s, e);
+  const uint32_t bw = 8u * (uint32_t)e.n_tgt;
+  hipLaunchKernelGGL(k_bbox_init, dim3((bw + kBlock - 1) / kBlock), dim3(kBlock), 0, s, e.bbox_tmp, bw);
   dim3 g(blocks_for(max_tgt_pts), e.n_tgt);
+  hipLaunchKernelGGL(k_cloud_bbox<true>, dim3(bbox_blocks_for(max_tgt_pts), e.n_tgt), dim3(kBlock), 0, s, e);
+  hipLaunchKernelGGL(k_tgt_choose, dim3((e.n_tgt + 63) / 64), dim3(64), 0, s, e);
   hipLaunchKernelGGL(k_tgt_count, g, dim3(kBlock), 0, s, e);
   launch_exclusive_scan(e.cell_count, e.cell_start, e.n_cells_total, e.scan_tmp, s);
   hipLaunchKernelGGL(k_tgt_scatter, g, dim3(kBlock), 0, s, e);
@@ -373,12 +396,15 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
   return r;
 }
 
-__global__ void k_src_bbox(DevEngine e) {
-  SourceDesc& s = e.srcs[blockIdx.x];
-  __shared__ float lo[3], hi[3];
-  __shared__ uint32_t cnt;
-  block_bbox(e.src_raw + 3 * (size_t)s.off, s.n, lo, hi, &cnt);
-  if (threadIdx.x == 0) {
+// one thread per source, after k_cloud_bbox<false>
+__global__ void k_src_choose(DevEngine e) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= e.n_src) return;
+  SourceDesc& s = e.srcs[j];
+  float lo[3], hi[3];
+  uint32_t cnt;
+  bbox_of(e.bbox_tmp + 8 * (size_t)(e.n_tgt + j), lo, hi, &cnt);
+  {
     // bin_budget = P^3 with P a power of two <= 32 (chosen by the host)
     int P = 1;
     while (P * P * P < (int)s.bin_budget) P <<= 1;
@@ -444,8 +470,7 @@ __device__ __forceinline__ uint32_t spread3_4(uint32_t v) {
   return r;
 }
 
-// Waves stride over the bins (most bins of a sparse surface are empty, so a
-// wave per bin would launch mostly idle workgroups).
+// One wave orders one bin.
 __device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32_t total, int lane, int w,
                                           uint64_t (*s_key)[kBinCap], float4 (*s_pt)[kBinCap]) {
   const uint32_t beg = e.bin_start[b];
@@ -516,23 +541,40 @@ __device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32
 
 constexpr uint32_t kOrderBlocks = 16384;
 
-// one wave per workgroup (24 KB of LDS each)
+// One wave per workgroup (24 KB of LDS each).  Waves stride over groups of 64
+// bins: the lanes read the 64 bins' sizes at once, and the wave then orders
+// the bins of more than one point one after another (most bins of a sparse
+// surface hold none or one, so they cost one coalesced read).
 __global__ __launch_bounds__(64) void k_src_order_bins(DevEngine e, uint32_t total) {
   __shared__ uint64_t s_key[1][kBinCap];
   __shared__ float4 s_pt[1][kBinCap];
   const int lane = threadIdx.x & 63;
-  for (uint32_t b = blockIdx.x; b < e.n_bins_total; b += gridDim.x) order_bin(e, b, total, lane, 0, s_key, s_pt);
+  for (uint32_t b0 = blockIdx.x * 64u; b0 < e.n_bins_total; b0 += gridDim.x * 64u) {
+    const uint32_t b = b0 + (uint32_t)lane;
+    uint32_t n = 0u;
+    if (b < e.n_bins_total) n = ((b + 1 < e.n_bins_total) ? e.bin_start[b + 1] : total) - e.bin_start[b];
+    uint64_t todo = __ballot(n > 1u);
+    while (todo) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(todo);
+      todo &= todo - 1ull;
+      order_bin(e, b0 + l, total, lane, 0, s_key, s_pt);
+    }
+  }
 }
 
 void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s) {
   if (e.n_src == 0) return;
   launch_fill_u32(e.bin_count, 0u, e.n_bins_total, s);
-  hipLaunchKernelGGL(k_src_bbox, dim3(e.n_src), dim3(kBlock), 0, s, e);
+  const uint32_t bw = 8u * (uint32_t)e.n_src;
+  hipLaunchKernelGGL(k_bbox_init, dim3((bw + kBlock - 1) / kBlock), dim3(kBlock), 0, s, e.bbox_tmp + 8 * (size_t)e.n_tgt,
+                     bw);
+  hipLaunchKernelGGL(k_cloud_bbox<false>, dim3(bbox_blocks_for(max_src_pts), e.n_src), dim3(kBlock), 0, s, e);
+  hipLaunchKernelGGL(k_src_choose, dim3((e.n_src + 63) / 64), dim3(64), 0, s, e);
   dim3 g(blocks_for(max_src_pts), e.n_src);
   hipLaunchKernelGGL(k_src_count, g, dim3(kBlock), 0, s, e);
   launch_exclusive_scan(e.bin_count, e.bin_start, e.n_bins_total, e.scan_tmp, s);
   hipLaunchKernelGGL(k_src_scatter, g, dim3(kBlock), 0, s, e);
-  const uint32_t ob = e.n_bins_total;
+  const uint32_t ob = (e.n_bins_total + 63) / 64;
   hipLaunchKernelGGL(k_src_order_bins, dim3(ob < kOrderBlocks ? ob : kOrderBlocks), dim3(64), 0, s, e, src_total);
 }
 

@@ -381,9 +381,9 @@ class LearnedRegistration:
 
     refine_anderson > 0 runs the refine launch with Anderson acceleration of
     that memory (rtr_params.anderson_memory): an equally good minimum in
-    fewer iterations.  The fallback is always the plain multi-start
-    iteration, on an engine of its own that holds only the rejected pairs'
-    clouds: the refine engine keeps one layout (its hipGraph replays every
+    fewer iterations; fallback_anderson does the same for the fallback's
+    multi-start launch (0: the plain iteration).  The fallback runs on an
+    engine of its own that holds only the rejected pairs' clouds: the refine engine keeps one layout (its hipGraph replays every
     step), and the fallback launch (a few pairs x n_starts problems,
     latency-bound) runs work items of fallback_item_pts source points on
     fallback_blocks_per_cu persistent workgroups per CU (fewer idle
@@ -393,7 +393,7 @@ class LearnedRegistration:
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 1024,
                  max_curvature: float = 0.02, hypotheses: int = 8, tau: float = 0.1, max_pairs: int = 256,
                  refine_anderson: int = 5, fallback_item_pts: int = 256, fallback_blocks_per_cu: int = 1,
-                 device: int = 0):
+                 fallback_anderson: int = 0, device: int = 0):
         if weights is None:
             weights, _ = load_weights()
         self.params = L.Params.from_buffer_copy(params if params is not None else default_params())
@@ -409,7 +409,9 @@ class LearnedRegistration:
         self.reg.set_params(self.refine_params)
         self.reg.set_rebuild(False)                # one grid build per align() call
         self.fb = BatchRegistration(device)        # fallback: the rejected pairs only
-        self.fb.set_params(self.params)
+        fb_params = L.Params.from_buffer_copy(self.params)
+        fb_params.anderson_memory = fallback_anderson
+        self.fb.set_params(fb_params)
         self.fb.set_tuning(item_pts=fallback_item_pts, blocks_per_cu=fallback_blocks_per_cu)
         self.n_starts = n_starts
         self.rmse_threshold = rmse_threshold
