@@ -329,7 +329,8 @@ __global__ __launch_bounds__(kFThreads) void k_mlp_pool(const uint16_t* x0, int 
 // chosen greedily, starting at subsample index 0, each time the point with
 // the largest distance to the chosen set (lowest index on ties).  Each
 // thread keeps its kFpsPer subsample points (i = tid + k * kFpsThreads) and
-// their distances in registers; the chosen point is broadcast through LDS.
+// their distances in registers; the chosen point is broadcast through LDS,
+// and the chosen points are kept in LDS and written out once at the end.
 constexpr int kFpsThreads = 512;
 constexpr int kFpsPer = kFpsMax / kFpsThreads;
 constexpr int kFpsWaves = kFpsThreads / 64;
@@ -349,6 +350,8 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
   __shared__ int red_i[2][kFpsWaves];
   __shared__ float red_p[2][kFpsWaves][3];
   __shared__ double cen[kFpsWaves][3];
+  __shared__ float s_sel[kP][3];  // chosen points, centroid removed
+  __shared__ int s_cur[kP];       // their subsample indices
   float px[kFpsPer], py[kFpsPer], pz[kFpsPer], dist[kFpsPer];
   double ax = 0, ay = 0, az = 0;
 #pragma unroll
@@ -402,54 +405,46 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
   float qx = s_q[0], qy = s_q[1], qz = s_q[2];
   for (int s = 0; s < kP; ++s) {
     if (tid == 0) {
-      const size_t o = ((size_t)c * kP + s);
-      pts_out[3 * o] = qx - cx;
-      pts_out[3 * o + 1] = qy - cy;
-      pts_out[3 * o + 2] = qz - cz;
-      if (sel_out) sel_out[o] = cur;
-    }
-    if (tid < kC0) {
-      const size_t o = ((size_t)c * kP + s) * kC0 + tid;
-      const float v = tid == 0 ? qx - cx : (tid == 1 ? qy - cy : (tid == 2 ? qz - cz : 0.f));
-      x0[o] = f2bf(v);
+      s_sel[s][0] = qx - cx;
+      s_sel[s][1] = qy - cy;
+      s_sel[s][2] = qz - cz;
+      s_cur[s] = cur;
     }
     if (s + 1 == kP) break;
-    float bv = -1.f;
+    float bv = -1.f, wx = 0.f, wy = 0.f, wz = 0.f;
     int bi = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < kFpsPer; ++k) {
-      if (dist[k] >= 0.f) {  // a subsample point (i < m)
-        const float dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
-        float d = dx * dx;
-        d = d + dy * dy;
-        d = d + dz * dz;
-        dist[k] = fminf(dist[k], d);
-        if (dist[k] > bv) {  // k ascending = index ascending: strict keeps the lowest
-          bv = dist[k];
-          bi = tid + k * kFpsThreads;
-        }
-      }
+      // slots past the subsample hold -1, which fminf keeps and the strict
+      // comparison never picks, so every slot is updated without a branch
+      const float dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+      float d = dx * dx;
+      d = d + dy * dy;
+      d = d + dz * dz;
+      dist[k] = fminf(dist[k], d);
+      // k ascending = index ascending: strict keeps the lowest (selects, no branch)
+      const bool gt = dist[k] > bv;
+      bv = gt ? dist[k] : bv;
+      bi = gt ? tid + k * kFpsThreads : bi;
+      wx = gt ? px[k] : wx;
+      wy = gt ? py[k] : wy;
+      wz = gt ? pz[k] : wz;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       const float ov = __shfl_xor(bv, o, 64);
       const int oi = __shfl_xor(bi, o, 64);
-      if (ov > bv || (ov == bv && oi < bi)) {
-        bv = ov;
-        bi = oi;
-      }
+      const bool take = (ov > bv) | ((ov == bv) & (oi < bi));
+      bv = take ? ov : bv;
+      bi = take ? oi : bi;
     }
     // the lane that owns the wave's winner publishes it with its coordinates
     const int buf = s & 1;
     if (bi >= 0 && bi < m && (bi % kFpsThreads) == tid) {
-      const int kk = bi / kFpsThreads;
-#pragma unroll
-      for (int k = 0; k < kFpsPer; ++k)
-        if (k == kk) {
-          red_p[buf][w][0] = px[k];
-          red_p[buf][w][1] = py[k];
-          red_p[buf][w][2] = pz[k];
-        }
+      // the wave's winner is this lane's own best
+      red_p[buf][w][0] = wx;
+      red_p[buf][w][1] = wy;
+      red_p[buf][w][2] = wz;
       red_v[buf][w] = bv;
       red_i[buf][w] = bi;
     } else if (lane == 0 && !(bi >= 0 && bi < m)) {  // no subsample point in this wave
@@ -457,14 +452,23 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
       red_i[buf][w] = 0x7fffffff;
     }
     __syncthreads();
-    float gv = red_v[buf][0];
-    int gi = red_i[buf][0], gw = 0;
-    for (int k = 1; k < kFpsWaves; ++k)
-      if (red_v[buf][k] > gv || (red_v[buf][k] == gv && red_i[buf][k] < gi)) {
-        gv = red_v[buf][k];
-        gi = red_i[buf][k];
-        gw = k;
-      }
+    // the waves' candidates, read together, then compared without branches
+    float rv[kFpsWaves];
+    int ri[kFpsWaves];
+#pragma unroll
+    for (int k = 0; k < kFpsWaves; ++k) {
+      rv[k] = red_v[buf][k];
+      ri[k] = red_i[buf][k];
+    }
+    float gv = rv[0];
+    int gi = ri[0], gw = 0;
+#pragma unroll
+    for (int k = 1; k < kFpsWaves; ++k) {
+      const bool take = (rv[k] > gv) | ((rv[k] == gv) & (ri[k] < gi));
+      gv = take ? rv[k] : gv;
+      gi = take ? ri[k] : gi;
+      gw = take ? k : gw;
+    }
     if (gi >= 0 && gi < m) {
       cur = gi;
       qx = red_p[buf][gw][0];
@@ -476,6 +480,18 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
       qy = s_q[1];
       qz = s_q[2];
     }
+  }
+  __syncthreads();
+  for (int s = tid; s < kP; s += kFpsThreads) {
+    const size_t o = (size_t)c * kP + s;
+    pts_out[3 * o] = s_sel[s][0];
+    pts_out[3 * o + 1] = s_sel[s][1];
+    pts_out[3 * o + 2] = s_sel[s][2];
+    if (sel_out) sel_out[o] = s_cur[s];
+  }
+  for (int t = tid; t < kP * kC0; t += kFpsThreads) {
+    const int s = t / kC0, ch = t % kC0;
+    x0[(size_t)c * kP * kC0 + t] = f2bf(ch < 3 ? s_sel[s][ch] : 0.f);
   }
 }
 
