@@ -387,7 +387,8 @@ class LearnedRegistration:
     step), and the fallback launch (a few pairs x n_starts problems,
     latency-bound) runs work items of fallback_item_pts source points on
     fallback_blocks_per_cu persistent workgroups per CU (fewer idle
-    workgroups polling the work queue)."""
+    workgroups polling the work queue), launched directly rather than from a
+    captured hipGraph (its layout changes every call)."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 1024,
@@ -413,6 +414,8 @@ class LearnedRegistration:
         fb_params.anderson_memory = fallback_anderson
         self.fb.set_params(fb_params)
         self.fb.set_tuning(item_pts=fallback_item_pts, blocks_per_cu=fallback_blocks_per_cu)
+        # its layout changes every call: a captured graph would be used once
+        self.fb.set_graph(False)
         self.n_starts = n_starts
         self.rmse_threshold = rmse_threshold
         self.min_inlier_frac = min_inlier_frac
