@@ -98,6 +98,12 @@ def fp64_check(srcs, tgts, gts, T_ref, inits, T_plain, T_learned, n_chk):
             "learned_vs_fp64_continued_median_rmse_m": float(np.median([x[2] for x in r]))}
 
 
+def tuning6(cfg):
+    """A tuning as 6 rounded values (a 5-value tuning has no early exit)."""
+    v = [round(float(x), 4) for x in cfg]
+    return v + [0.0] * (6 - len(v))
+
+
 def prefer_profiled(reg, best, best_ms, table, name, tol=0.03):
     """Among candidates the autotuner timed within `tol` of its fastest (a
     statistical tie), prefer the tuning of the committed rocprofv3 summary of
@@ -107,11 +113,11 @@ def prefer_profiled(reg, best, best_ms, table, name, tol=0.03):
     if not os.path.exists(prof):
         return best, best_ms
     with open(prof) as f:
-        want = [round(float(v), 4) for v in json.load(f)["tuning"]]
-    if [round(float(v), 4) for v in best] == want:
+        want = tuning6(json.load(f)["tuning"])
+    if tuning6(best) == want:
         return best, best_ms
     for cfg, ms in table:
-        if [round(float(v), 4) for v in cfg] == want and ms <= best_ms * (1 + tol):
+        if tuning6(cfg) == want and ms <= best_ms * (1 + tol):
             autotune.apply(reg, cfg)
             log(f"autotune: {cfg} at {ms:.1f} ms ties the fastest {best} ({best_ms:.1f} ms); keeping the profiled one")
             return tuple(cfg), ms
@@ -178,7 +184,7 @@ def main():
                     help="source points sampled per pair by the registration judge")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
-                    help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile'")
+                    help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile[,retire]'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--fp64-sample", type=int, default=8)
@@ -228,7 +234,7 @@ def main():
     tuned = None
     if args.tuning:
         f = [float(v) for v in args.tuning.split(",")]
-        tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4]))
+        tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4])) + ((f[5],) if len(f) > 5 else ())
     elif not args.no_autotune and ms is not None:
         tuned, tuned_ms, table = autotune.tune(ms.reg, ms.guesses, candidates=autotune.LATENCY_CANDIDATES, key=None,
                                                use_cache=False)
@@ -341,7 +347,7 @@ def main():
             return None, None
         with open(prof) as f:
             pj = json.load(f)
-        same = [round(v, 4) for v in pj["tuning"]] == [round(float(v), 4) for v in tuning]
+        same = tuning6(pj["tuning"]) == tuning6(tuning)
         same = same and pj.get("refine_anderson_memory", 0) == (args.refine_anderson if "refine" in name else 0)
         return (float(pj["traffic_bytes"]), f"profiles/{name}.md") if same else (None, None)
 

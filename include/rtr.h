@@ -175,6 +175,13 @@ int rtr_batch_set_params(rtr_batch* b, const rtr_params* p);
  * (and <= 0 for the first three) keep the current setting. */
 int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point, int item_pts,
                          int blocks_per_cu, int use_tile);
+/* Early exit of surplus persistent workgroups: those beyond one per CU leave
+ * at the start of an iteration once at most below * n_problems problems are
+ * unfinished (0 = never, the default; below in [0, 1]).  Many co-resident
+ * workgroups raise throughput while most problems iterate; once few remain,
+ * idle workgroups polling the work queue stretch their iterations. */
+int rtr_batch_set_retire(rtr_batch* b, float below);
+int rtr_batch_get_retire(rtr_batch* b, float* below);
 /* The current tuning (any pointer may be null). */
 int rtr_batch_get_tuning(rtr_batch* b, float* cell_factor, float* cells_per_point, int* item_pts,
                          int* blocks_per_cu, int* use_tile);

@@ -22,7 +22,9 @@ from . import _lib as L
 CACHE = os.environ.get("RTR_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "realtime_robot_amd", "tune.json"))
 
-# (cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile)
+# (cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile[, retire])
+# retire: workgroups beyond one per CU leave once fewer than this fraction of
+# the problems are unfinished (0: never; rtr_batch_set_retire)
 DEFAULT = (0.25, 2.0, 1024, 0, 0)
 CANDIDATES = [
     (0.25, 2.0, 1024, 0, 0),
@@ -31,6 +33,8 @@ CANDIDATES = [
     (0.2, 3.0, 1024, 0, 0),
     (0.25, 2.0, 512, 0, 0),
     (0.25, 2.0, 1024, 2, 0),
+    (0.35, 2.0, 1024, 0, 0, 0.25),
+    (0.35, 2.0, 1024, 0, 0, 0.5),
 ]
 # Few problems per launch (one refine per pair, a small fallback set): the
 # launch is bound by each problem's per-iteration latency, so smaller work
@@ -49,6 +53,12 @@ LATENCY_CANDIDATES = CANDIDATES + [
     (0.2, 3.0, 512, 2, 0),
     (0.35, 2.0, 512, 1, 0),
     (0.35, 2.0, 256, 1, 0),
+    (0.35, 2.0, 512, 0, 0, 0.25),
+    (0.35, 2.0, 512, 0, 0, 0.5),
+    (0.35, 2.0, 512, 0, 0, 0.75),
+    (0.35, 2.0, 512, 2, 0, 0.5),
+    (0.35, 2.0, 256, 2, 0, 0.5),
+    (0.25, 2.0, 512, 0, 0, 0.5),
 ]
 
 
@@ -84,8 +94,8 @@ def save_cache(cache: dict) -> None:
 
 
 def apply(batch, cfg) -> None:
-    cf, cpp, item, bpc, tile = cfg
-    batch.set_tuning(cf, cpp, item, bpc, tile)
+    cf, cpp, item, bpc, tile = cfg[:5]
+    batch.set_tuning(cf, cpp, item, bpc, tile, cfg[5] if len(cfg) > 5 else 0.0)
 
 
 def tune(batch, guesses=None, candidates=None, reps: int = 2, key: str | None = None, use_cache: bool = True,

@@ -424,6 +424,24 @@ int rtr_batch_get_tuning(rtr_batch* b, float* cell_factor, float* cells_per_poin
   return RTR_OK;
 }
 
+int rtr_batch_set_retire(rtr_batch* b, float below) {
+  RTR_GUARD_BEGIN
+  if (!b || !(below >= 0.f && below <= 1.f)) {
+    set_error("rtr_batch_set_retire: the fraction must lie in [0, 1]");
+    return RTR_ERR_INVALID_ARG;
+  }
+  b->eng.tune().retire = below;
+  b->eng.invalidate_graph();
+  return RTR_OK;
+  RTR_GUARD_END
+}
+
+int rtr_batch_get_retire(rtr_batch* b, float* below) {
+  if (!b || !below) return RTR_ERR_INVALID_ARG;
+  *below = b->eng.tune().retire;
+  return RTR_OK;
+}
+
 int rtr_batch_set_rebuild(rtr_batch* b, int always) {
   if (!b) return RTR_ERR_INVALID_ARG;
   b->eng.set_rebuild_always(always != 0);

@@ -410,6 +410,8 @@ DevEngine Engine::dev_view() const {
   e.item_pts = tune_.item_pts;
   e.use_tile = tune_.use_tile;
   e.deep = deep_gathers() ? 1 : 0;
+  e.retire_from = tune_.retire > 0.f ? cu_count_ : 0x7fffffff;
+  e.retire_active = (uint32_t)((double)tune_.retire * (double)probs_.size());
   e.partials = partials_.as<float>();
   uint32_t* ctr = counters_.as<uint32_t>();
   // the done count, the error flag and every claim counter on cache lines of

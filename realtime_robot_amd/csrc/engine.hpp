@@ -49,6 +49,7 @@ struct TuneParams {
   int item_pts = 1024;            // source points per work item
   int blocks_per_cu = 0;          // persistent workgroups per CU (0 = occupancy limit)
   int use_tile = 0;               // LDS candidate tiles on/off
+  float retire = 0.f;             // surplus workgroups leave below this fraction of unfinished problems
 };
 
 class Engine {

@@ -53,6 +53,8 @@ struct DevEngine {
   int item_pts;              // points per item
   int use_tile;              // stage candidate tiles in LDS (else global gathers only)
   int deep;                  // 8-deep gathers, 2 waves per SIMD (point-to-point; always for point-to-plane)
+  int retire_from;           // workgroups from this index on leave at the start of an iteration ...
+  uint32_t retire_active;    // ... once at most this many problems are unfinished
   float* partials;           // n_items * NSUM_MAX * 2 floats
   uint32_t* claim;           // claim counter per (pass, XCD queue), claim_stride words apart
   int claim_stride;          // 32 (one 128-byte line per counter) unless the iteration limit is huge

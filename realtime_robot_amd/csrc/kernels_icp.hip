@@ -718,9 +718,16 @@ __global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH =
   const int tid = threadIdx.x;
   if (tid == 0) sh.n_search = 0u;
   for (int pass = 0; pass < e.prm.max_iter; ++pass) {
-    if (tid == 0)
-      sh.stop = (load_acquire(e.problems_done) >= (uint32_t)e.n_prob) ||
-                (load_acquire((const uint32_t*)e.error) != 0u);
+    // Workgroups beyond one per CU leave once few problems are unfinished:
+    // they raise throughput while many problems iterate, but idle ones
+    // polling the work queue stretch the last problems' iterations.  They
+    // leave between passes, holding no claimed item, and one workgroup per CU
+    // stays, so every remaining item is still claimed.
+    if (tid == 0) {
+      const uint32_t done = load_acquire(e.problems_done);
+      sh.stop = (done >= (uint32_t)e.n_prob) || (load_acquire((const uint32_t*)e.error) != 0u) ||
+                ((int)blockIdx.x >= e.retire_from && done + e.retire_active >= (uint32_t)e.n_prob);
+    }
     __syncthreads();
     if (sh.stop) break;
     for (;;) {

@@ -337,3 +337,24 @@ def test_seeded_search_last_correspondences_bitexact(mode):
                 assert np.array_equal(idx, i_ref), f"{iters} iterations, problem {p}: {np.sum(idx != i_ref)} mismatches"
                 assert np.array_equal(d2, d_ref)
             print(mode, bpc, iters, searched)
+
+
+def test_surplus_workgroups_leaving_early_keeps_results():
+    """Workgroups beyond one per CU leaving once few problems are unfinished
+    (rtr_batch_set_retire) change who processes the remaining work items, not
+    what they compute: the results are bit-identical."""
+    srcs, tgts, gts = synth.config3_batch(8, seed_offset=700, ns_range=(2000, 8000), nt_range=(8000, 30000),
+                                          max_deg=20.0, max_t=0.05)
+    b = R.BatchRegistration()
+    b.set_clouds(srcs, tgts)
+    b.set_problems(np.arange(8), np.arange(8))
+    b.set_params(R.make_params(max_iterations=30, max_correspondence_distance=0.05, transformation_epsilon=1e-8))
+    b.set_tuning(item_pts=512, blocks_per_cu=0, use_tile=0, retire=0.0)
+    ref = b.align()
+    for frac in (0.25, 0.5, 1.0):
+        b.set_tuning(retire=frac)
+        assert b.tuning()[5] == frac
+        res = b.align()
+        for p in range(8):
+            assert np.array_equal(res[p]["T"], ref[p]["T"]), (frac, p)
+            assert res[p]["iterations"] == ref[p]["iterations"]
