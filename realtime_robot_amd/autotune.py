@@ -59,6 +59,10 @@ LATENCY_CANDIDATES = CANDIDATES + [
     (0.35, 2.0, 512, 2, 0, 0.5),
     (0.35, 2.0, 256, 2, 0, 0.5),
     (0.25, 2.0, 512, 0, 0, 0.5),
+    (0.25, 2.0, 512, 0, 0, 0.4),
+    (0.25, 2.0, 512, 0, 0, 0.6),
+    (0.25, 2.0, 256, 0, 0, 0.5),
+    (0.25, 2.0, 1024, 0, 0, 0.5),
 ]
 
 
