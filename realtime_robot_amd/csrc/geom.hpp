@@ -470,6 +470,81 @@ RTR_HD void jacobi4(double* a, double* V) {
   }
 }
 
+// Determinant of the 3x3 [[a, b, c], [d, e, f], [g, h, i]].
+RTR_HD double det3(double a, double b, double c, double d, double e, double f, double g, double h, double i) {
+  return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+
+// Largest eigenpair of a symmetric 4x4 (row-major) from its characteristic
+// polynomial (the QCP approach of Theobald 2005): the largest root by Newton's
+// method started at the Frobenius norm, an upper bound, from where the
+// iteration descends monotonically; the eigenvector is the column of
+// adj(N - lambda I) with the largest diagonal cofactor, since for a simple
+// eigenvalue adj(N - lambda I) = prod_j (lambda_j - lambda) v v^T.  Returns
+// false, leaving q untouched, when that cofactor is below 1e-6 |N|_F^3 (a
+// repeated or nearly repeated largest eigenvalue, where the cofactors lose
+// their relative precision): the caller then diagonalises N by Jacobi.  All
+// values stay in registers.
+RTR_HD bool max_eig4(const double* N, double* q) {
+  const double a00 = N[0], a01 = N[1], a02 = N[2], a03 = N[3];
+  const double a11 = N[5], a12 = N[6], a13 = N[7];
+  const double a22 = N[10], a23 = N[11], a33 = N[15];
+  const double fro2 = a00 * a00 + a11 * a11 + a22 * a22 + a33 * a33 +
+                      2.0 * (a01 * a01 + a02 * a02 + a03 * a03 + a12 * a12 + a13 * a13 + a23 * a23);
+  const double fro = sqrt(fro2);
+  if (!(fro > 0.0) || !(fro < 1e150)) return false;
+  // det(l I - N) = l^4 - e1 l^3 + e2 l^2 - e3 l + e4 (sums of principal minors)
+  const double e1 = a00 + a11 + a22 + a33;
+  const double e2 = (a00 * a11 - a01 * a01) + (a00 * a22 - a02 * a02) + (a00 * a33 - a03 * a03) +
+                    (a11 * a22 - a12 * a12) + (a11 * a33 - a13 * a13) + (a22 * a33 - a23 * a23);
+  const double e3 = det3(a00, a01, a02, a01, a11, a12, a02, a12, a22) + det3(a00, a01, a03, a01, a11, a13, a03, a13, a33) +
+                    det3(a00, a02, a03, a02, a22, a23, a03, a23, a33) + det3(a11, a12, a13, a12, a22, a23, a13, a23, a33);
+  const double s0 = a00 * a11 - a01 * a01, s1 = a00 * a12 - a01 * a02, s2 = a00 * a13 - a01 * a03;
+  const double s3 = a01 * a12 - a11 * a02, s4 = a01 * a13 - a11 * a03, s5 = a02 * a13 - a12 * a03;
+  const double c5 = a22 * a33 - a23 * a23, c4 = a12 * a33 - a13 * a23, c3 = a12 * a23 - a13 * a22;
+  const double c2 = a02 * a33 - a03 * a23, c1 = a02 * a23 - a03 * a22, c0 = a02 * a13 - a03 * a12;
+  const double e4 = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  double l = fro;
+#pragma unroll 1
+  for (int it = 0; it < 64; ++it) {
+    const double f = (((l - e1) * l + e2) * l - e3) * l + e4;
+    const double df = ((4.0 * l - 3.0 * e1) * l + 2.0 * e2) * l - e3;
+    if (!(df > 0.0)) break;
+    const double step = f / df;
+    l -= step;
+    if (!(fabs(step) > 1e-15 * fro)) break;
+  }
+  // adjugate of B = N - l I (symmetric): the ten distinct cofactors
+  const double b00 = a00 - l, b11 = a11 - l, b22 = a22 - l, b33 = a33 - l;
+  const double C00 = det3(b11, a12, a13, a12, b22, a23, a13, a23, b33);
+  const double C11 = det3(b00, a02, a03, a02, b22, a23, a03, a23, b33);
+  const double C22 = det3(b00, a01, a03, a01, b11, a13, a03, a13, b33);
+  const double C33 = det3(b00, a01, a02, a01, b11, a12, a02, a12, b22);
+  const double C01 = -det3(a01, a12, a13, a02, b22, a23, a03, a23, b33);
+  const double C02 = det3(a01, b11, a13, a02, a12, a23, a03, a13, b33);
+  const double C03 = -det3(a01, b11, a12, a02, a12, b22, a03, a13, a23);
+  const double C12 = -det3(b00, a01, a03, a02, a12, a23, a03, a13, b33);
+  const double C13 = det3(b00, a01, a02, a02, a12, b22, a03, a13, a23);
+  const double C23 = -det3(b00, a01, a02, a01, b11, a12, a03, a13, a23);
+  double v0 = C00, v1 = C01, v2 = C02, v3 = C03, best = fabs(C00);
+  if (fabs(C11) > best) {
+    v0 = C01; v1 = C11; v2 = C12; v3 = C13; best = fabs(C11);
+  }
+  if (fabs(C22) > best) {
+    v0 = C02; v1 = C12; v2 = C22; v3 = C23; best = fabs(C22);
+  }
+  if (fabs(C33) > best) {
+    v0 = C03; v1 = C13; v2 = C23; v3 = C33; best = fabs(C33);
+  }
+  if (!(best > 1e-6 * fro2 * fro)) return false;
+  const double nv = sqrt(v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3);
+  q[0] = v0 / nv;
+  q[1] = v1 / nv;
+  q[2] = v2 / nv;
+  q[3] = v3 / nv;
+  return true;
+}
+
 // Rigid fit from first and second moments (Horn 1987, closed form via unit
 // quaternions): n pairs (a_i -> b_i) with sums Sa, Sb and Sab[r*3+c] =
 // sum a_r b_c, all relative to a common origin.  Produces R, t with
@@ -502,11 +577,15 @@ RTR_HD void rigid_horn(double n, const double* Sa, const double* Sb, const doubl
   N[10] = -Sxx + Syy - Szz;
   N[11] = N[14] = Syz + Szy;
   N[15] = -Sxx - Syy + Szz;
-  jacobi4(N, V);
-  int best = 0;
-  for (int i = 1; i < 4; ++i)
-    if (N[i * 5] > N[best * 5]) best = i;
-  double q0 = V[0 * 4 + best], qx = V[1 * 4 + best], qy = V[2 * 4 + best], qz = V[3 * 4 + best];
+  double qv[4];
+  if (!max_eig4(N, qv)) {
+    jacobi4(N, V);
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+      if (N[i * 5] > N[best * 5]) best = i;
+    for (int i = 0; i < 4; ++i) qv[i] = V[i * 4 + best];
+  }
+  double q0 = qv[0], qx = qv[1], qy = qv[2], qz = qv[3];
   const double nq = sqrt(q0 * q0 + qx * qx + qy * qy + qz * qz);
   q0 /= nq;
   qx /= nq;

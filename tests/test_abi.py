@@ -133,6 +133,44 @@ def test_host_solver_matches_oracle():
     assert np.abs(T1 - oracle.euler_to_T(x, (0.5, 0.5, 0.5))).max() < 1e-6
 
 
+def test_host_rigid_solver_pinned_on_hard_cases():
+    """The rigid fit (Horn's quaternion: the largest eigenpair of his 4x4 by
+    its characteristic polynomial, Jacobi when that eigenvalue is nearly
+    repeated) against the oracle's SVD Umeyama, moments about the centroid:
+    general, planar and thin clouds, rotations from 1e-7 rad to nearly pi,
+    with and without noise (bound 5e-7: the per-point moment terms are FP32,
+    as in the kernels, which leaves up to 2.4e-7 at large rotations; a wrong
+    eigenvector is off by orders of magnitude more); a collinear cloud (rotation about the line undetermined) must
+    still give a proper rotation that maps the points."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(11)
+    for k in range(60):
+        kind = k % 3
+        a = rng.random((300, 3)) * 2 - 1
+        if kind == 1:
+            a[:, 2] = 0.0
+        elif kind == 2:
+            a[:, 1] *= 0.01
+        a = (a + 3).astype(np.float32)
+        ang = [1e-7, 1e-3, 0.5, 2.0, 3.1, 3.14159][k % 6]
+        ax = rng.normal(size=3)
+        T = np.eye(4)
+        T[:3, :3] = Rotation.from_rotvec(ax / np.linalg.norm(ax) * ang).as_matrix()
+        T[:3, 3] = rng.random(3)
+        b = a.astype(np.float64) @ T[:3, :3].T + T[:3, 3] + rng.normal(size=a.shape) * (1e-3 if k % 2 else 0.0)
+        b = b.astype(np.float32)
+        T1 = R.selfcheck_solve(0, a, b, anchor=tuple(a.astype(np.float64).mean(0)))
+        T2 = oracle.umeyama(a.astype(np.float64), b.astype(np.float64))
+        assert np.abs(T1 - T2).max() < 5e-7, (k, np.abs(T1 - T2).max())
+    a = (np.outer(rng.random(300), [1.0, 2.0, 3.0]) + 3).astype(np.float32)
+    Rm = Rotation.from_rotvec([0.2, -0.1, 0.3]).as_matrix()
+    b = (a.astype(np.float64) @ Rm.T + [0.1, 0.2, 0.3]).astype(np.float32)
+    T1 = R.selfcheck_solve(0, a, b, anchor=tuple(a.astype(np.float64).mean(0)))
+    Rs = T1[:3, :3]
+    assert abs(np.linalg.det(Rs) - 1) < 1e-12 and np.abs(Rs @ Rs.T - np.eye(3)).max() < 1e-12
+    assert np.abs(a.astype(np.float64) @ Rs.T + T1[:3, 3] - b).max() < 1e-5
+
+
 @pytest.mark.skipif(_gpu_present(), reason="a device is present")
 def test_no_device_is_a_clean_error():
     with pytest.raises(R.RtrError) as e:
