@@ -182,6 +182,8 @@ def main():
                     help="Anderson memory of the learned path's fallback multi-start launch (0: plain iteration)")
     ap.add_argument("--judge-points", type=int, default=1024,
                     help="source points sampled per pair by the registration judge")
+    ap.add_argument("--judge-k", type=int, default=20,
+                    help="neighbours of the registration judge's PCA planes")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
                     help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile[,retire]'")
@@ -225,7 +227,8 @@ def main():
     weights, meta = learned.load_weights()
     lr = learned.LearnedRegistration(weights, params, n_starts=args.starts, hypotheses=args.hypotheses,
                                      max_pairs=n, refine_anderson=args.refine_anderson,
-                                     fallback_anderson=args.fallback_anderson, judge_points=args.judge_points)
+                                     fallback_anderson=args.fallback_anderson, judge_points=args.judge_points,
+                                     judge_k=args.judge_k)
     lr.set_clouds(srcs, tgts, cs, ct)
     ms = None
     if args.leg in ("both", "plain"):
@@ -374,6 +377,7 @@ def main():
                    "pairs": n, "hypotheses": args.hypotheses, "starts": args.starts,
                    "refine_anderson_memory": args.refine_anderson,
                    "fallback_anderson_memory": args.fallback_anderson, "judge_points": args.judge_points,
+                   "judge_k": args.judge_k,
                    "tuning": list(tuned) if tuned else None,
                    "tuning_refine": list(tuned_refine) if tuned_refine else None,
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],
