@@ -30,6 +30,7 @@ import json
 import math
 import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -402,6 +403,7 @@ class LearnedRegistration:
         self.refine_params = L.Params.from_buffer_copy(self.params)
         self.refine_params.anderson_memory = refine_anderson
         self._active = "refine"
+        self._pool = None
         self.hypotheses = hypotheses
         self.rots = multistart.so3_rotations(hypotheses) if hypotheses > 1 else None
         self.tau = tau
@@ -443,12 +445,18 @@ class LearnedRegistration:
 
     def align(self):
         t0 = time.perf_counter()
-        self.reg.build()
-        tb = time.perf_counter()
+        # the ICP grids are built on the engine's stream while the proposal
+        # runs on the proposer's: a worker thread waits for the build (the
+        # library releases the interpreter lock in its calls)
+        if self._pool is None:
+            self._pool = ThreadPoolExecutor(1)
+        build = self._pool.submit(self.reg.build)
         if self.hypotheses > 1:
             _, _, poses = self.prop.propose_multi(self.rots, self.tau)
         else:
             poses = self.prop.propose()
+        tb = time.perf_counter()
+        build.result()
         t1 = time.perf_counter()
         self.use_params("refine")
         res = self.reg.align(poses)
@@ -472,7 +480,9 @@ class LearnedRegistration:
                 res[i] = r[multistart.select_best(r)]
         t4 = time.perf_counter()
         self.stats = {"n_fallback": int(len(fail)), "fallback_pairs": fail, "proposals": poses, "judge_rmse": rmse,
-                      "judge_frac": frac, "judge_planar_frac": pfrac, "ms_build": (tb - t0) * 1e3,
-                      "ms_propose": (t1 - tb) * 1e3, "ms_refine": (t2 - t1) * 1e3, "ms_judge": (t3 - t2) * 1e3,
+                      # ms_propose: the proposal, with the grid build running beside it;
+                      # ms_build: what is left of the build after the proposal
+                      "judge_frac": frac, "judge_planar_frac": pfrac, "ms_build": (t1 - tb) * 1e3,
+                      "ms_propose": (tb - t0) * 1e3, "ms_refine": (t2 - t1) * 1e3, "ms_judge": (t3 - t2) * 1e3,
                       "ms_fallback": (t4 - t3) * 1e3, "ms_propose_device": self.prop.last_ms()}
         return res
