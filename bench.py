@@ -124,14 +124,18 @@ def prefer_profiled(reg, best, best_ms, table, name, tol=0.03):
     return best, best_ms
 
 
-def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample):
+def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample, coarse_stride=1):
     """Algorithmic HBM bytes of one k_icp launch over its duration.
 
     The launch is timed with HIP events on the engine's stream (median of 3).
     Bytes per problem-iteration are B_iter = 12 Ns + sum over the query cells
     of 12 |N27(c)| at a reference grid of edge r_max (accounting.py); B_iter is
     the compulsory 12 (Ns + Nt) scaled by the B_iter / B_min ratio measured on
-    a sample of problems along their own iteration histories."""
+    a sample of problems along their own iteration histories.  Coarse
+    iterations (coarse_stride > 1) query every coarse_stride-th source point:
+    they are priced on such a subsample (every s-th point in input order; the
+    engine takes every s-th in its spatial order, a subsample of the same
+    size)."""
     from realtime_robot_amd import accounting
     reg.set_profiling(True)
     icp_ms, grid_ms = [], []
@@ -142,9 +146,11 @@ def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample):
         grid_ms.append(t["ms_grid"])
     reg.set_profiling(False)
     iters = np.array([r["iterations"] for r in res], np.int64)
+    coarse = np.array([r["coarse_iterations"] for r in res], np.int64)
     ns = np.array([len(srcs[i]) for i in pair_of], np.int64)
     nt = np.array([len(tgts[i]) for i in pair_of], np.int64)
-    b_min_total = float(np.sum(iters * (12 * ns + 12 * nt)))
+    ns_c = (ns + coarse_stride - 1) // coarse_stride
+    b_min_total = float(np.sum((iters - coarse) * (12 * ns + 12 * nt) + coarse * (12 * ns_c + 12 * nt)))
     rng = np.random.default_rng(0)
     sample = rng.choice(len(pair_of), min(n_sample, len(pair_of)), replace=False)
     counts = {}
@@ -154,8 +160,10 @@ def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample):
         if i not in counts:
             counts[i] = accounting.TargetCounts(tgts[i], R_MAX)
         hist = reg.history(int(p), MAX_ITER)
+        sub = srcs[i][::coarse_stride]
         for it in range(res[p]["iterations"]):
-            bi, bm = accounting.iteration_bytes(srcs[i], counts[i], hist[it])
+            bi, bm = accounting.iteration_bytes(sub if it < res[p]["coarse_iterations"] else srcs[i], counts[i],
+                                                hist[it])
             s_iter += bi
             s_min += bm
     ratio = s_iter / s_min if s_min else 1.0
@@ -165,6 +173,7 @@ def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample):
             "grid_build_ms": float(np.median(grid_ms)), "algorithmic_bytes": b_iter_total,
             "compulsory_bytes": b_min_total, "achieved_compulsory_gbs": b_min_total / t_icp / 1e9,
             "b_iter_over_b_min": ratio, "iterations_total": int(iters.sum()),
+            "coarse_iterations_total": int(coarse.sum()),
             "b_iter_sample_problems": int(len(sample))}
 
 
@@ -182,6 +191,10 @@ def main():
                     help="Anderson memory of the learned path's fallback multi-start launch (0: plain iteration)")
     ap.add_argument("--judge-points", type=int, default=1024,
                     help="source points sampled per pair by the registration judge")
+    ap.add_argument("--refine-coarse-stride", type=int, default=4,
+                    help="coarse-to-fine refine: start on every s-th source point (0: off)")
+    ap.add_argument("--refine-coarse-iterations", type=int, default=25,
+                    help="most coarse iterations of the refine")
     ap.add_argument("--judge-k", type=int, default=20,
                     help="neighbours of the registration judge's PCA planes")
     ap.add_argument("--no-autotune", action="store_true")
@@ -228,7 +241,8 @@ def main():
     lr = learned.LearnedRegistration(weights, params, n_starts=args.starts, hypotheses=args.hypotheses,
                                      max_pairs=n, refine_anderson=args.refine_anderson,
                                      fallback_anderson=args.fallback_anderson, judge_points=args.judge_points,
-                                     judge_k=args.judge_k)
+                                     judge_k=args.judge_k, refine_coarse_stride=args.refine_coarse_stride,
+                                     refine_coarse_iterations=args.refine_coarse_iterations)
     lr.set_clouds(srcs, tgts, cs, ct)
     ms = None
     if args.leg in ("both", "plain"):
@@ -334,7 +348,8 @@ def main():
     lr.reg.set_problems(range(n), range(n))
     lr.use_params("refine")
     lr.reg.set_rebuild(True)               # the grid build is timed with the launch here
-    roof = launch_roofline(lr.reg, st["proposals"], np.arange(n), srcs, tgts, args.account_sample)
+    roof = launch_roofline(lr.reg, st["proposals"], np.arange(n), srcs, tgts, args.account_sample,
+                           max(1, args.refine_coarse_stride) if args.refine_coarse_iterations > 0 else 1)
     copy_ms = ctypes.c_float()
     L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 5, ctypes.byref(copy_ms)))
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms.value / 1e3) / 1e9
@@ -352,6 +367,8 @@ def main():
             pj = json.load(f)
         same = tuning6(pj["tuning"]) == tuning6(tuning)
         same = same and pj.get("refine_anderson_memory", 0) == (args.refine_anderson if "refine" in name else 0)
+        same = same and pj.get("refine_coarse", [0, 0]) == (
+            [args.refine_coarse_stride, args.refine_coarse_iterations] if "refine" in name else [0, 0])
         return (float(pj["traffic_bytes"]), f"profiles/{name}.md") if same else (None, None)
 
     traffic, traffic_src = profiled_traffic("k_icp_learned_refine_256", tuned_refine)
@@ -377,7 +394,8 @@ def main():
                    "pairs": n, "hypotheses": args.hypotheses, "starts": args.starts,
                    "refine_anderson_memory": args.refine_anderson,
                    "fallback_anderson_memory": args.fallback_anderson, "judge_points": args.judge_points,
-                   "judge_k": args.judge_k,
+                   "judge_k": args.judge_k, "refine_coarse_stride": args.refine_coarse_stride,
+                   "refine_coarse_iterations": args.refine_coarse_iterations,
                    "tuning": list(tuned) if tuned else None,
                    "tuning_refine": list(tuned_refine) if tuned_refine else None,
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],

@@ -73,6 +73,15 @@ typedef struct {
   double euclidean_fitness_epsilon;   /* relative MSE threshold */
   float cell_size;                    /* grid cell edge; 0 = autotuned */
   float cells_per_point;              /* grid cell budget per target point; 0 = autotuned */
+  int coarse_stride;                  /* 0 or 1 (default): every iteration uses every source point.
+                                         s > 1: coarse-to-fine (no PCL counterpart): iterations
+                                         start on every s-th source point (in the engine's
+                                         spatial order) and switch to all of them once an
+                                         increment moves the source by under 1 mm and 1 mrad, or
+                                         after coarse_iterations or half of max_iterations; only
+                                         full-resolution iterations can converge, so the result
+                                         is a fixed point of the full iteration */
+  int coarse_iterations;              /* most coarse iterations (with coarse_stride > 1) */
 } rtr_params;
 
 /* Outcome of one registration. */
@@ -84,7 +93,7 @@ typedef struct {
   int iterations;
   int converged;       /* PCL hasConverged(): criteria fired (incl. max iterations) */
   int state;           /* RTR_CONV_* or RTR_FAIL_* */
-  int reserved0;
+  int coarse_iterations;    /* of those, iterations on the coarse source subsample (rtr_params.coarse_stride) */
 } rtr_result;
 
 /* Phase timings of the last align, from HIP events on the handle's stream

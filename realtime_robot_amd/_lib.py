@@ -32,13 +32,13 @@ class Params(C.Structure):
                 ("anderson_memory", C.c_int), ("max_correspondence_distance", C.c_double),
                 ("transformation_epsilon", C.c_double), ("rotation_epsilon", C.c_double),
                 ("euclidean_fitness_epsilon", C.c_double), ("cell_size", C.c_float),
-                ("cells_per_point", C.c_float)]
+                ("cells_per_point", C.c_float), ("coarse_stride", C.c_int), ("coarse_iterations", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("T", C.c_double * 16), ("mse", C.c_double), ("fitness_rmse", C.c_double),
                 ("n_corr", C.c_int64), ("iterations", C.c_int), ("converged", C.c_int), ("state", C.c_int),
-                ("reserved0", C.c_int)]
+                ("coarse_iterations", C.c_int)]
 
 
 class Timing(C.Structure):

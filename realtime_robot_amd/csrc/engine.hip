@@ -219,9 +219,10 @@ int Engine::set_problems(int n, const int* src_id, const int* tgt_id) {
 int Engine::set_params(const rtr_params& p) {
   if (p.max_iterations < 1 || p.max_iterations > 100000 || !(p.max_correspondence_distance > 0) ||
       (p.mode != RTR_MODE_POINT_TO_POINT && p.mode != RTR_MODE_POINT_TO_PLANE) || p.anderson_memory < 0 ||
-      p.anderson_memory > kAndersonMax) {
-    set_error("invalid parameters (max_iterations %d, max distance %g, mode %d, anderson_memory %d)",
-              p.max_iterations, p.max_correspondence_distance, p.mode, p.anderson_memory);
+      p.anderson_memory > kAndersonMax || p.coarse_stride < 0 || p.coarse_stride > 64 || p.coarse_iterations < 0) {
+    set_error("invalid parameters (max_iterations %d, max distance %g, mode %d, anderson_memory %d, coarse %d/%d)",
+              p.max_iterations, p.max_correspondence_distance, p.mode, p.anderson_memory, p.coarse_stride,
+              p.coarse_iterations);
     return RTR_ERR_INVALID_ARG;
   }
   // the grids depend on the correspondence distance, the cell settings and
@@ -443,6 +444,8 @@ DevEngine Engine::dev_view() const {
   e.prm.eps_rot = prm_.rotation_epsilon;
   e.prm.eps_fit = prm_.euclidean_fitness_epsilon;
   e.prm.anderson = prm_.anderson_memory;
+  e.prm.coarse_stride = prm_.coarse_stride > 1 && prm_.coarse_iterations > 0 ? prm_.coarse_stride : 1;
+  e.prm.coarse_iters = prm_.coarse_iterations;
   return e;
 }
 
@@ -565,7 +568,7 @@ int Engine::align(const double* guesses, rtr_result* results) {
     r.iterations = hs[p].iter;
     r.state = hs[p].state;
     r.converged = (hs[p].state >= RTR_CONV_ITERATIONS && hs[p].state <= RTR_CONV_REL_MSE) ? 1 : 0;
-    r.reserved0 = 0;
+    r.coarse_iterations = hs[p].coarse_its;
   }
   timing_.iterations_total = it_total;
   if (hc[32] != 0) {  // the error flag (word 32)

@@ -88,6 +88,8 @@ __global__ void k_icp_init(DevEngine e) {
   S.iter = 0;
   S.state = RTR_NOT_CONVERGED;
   S.finished = 0;
+  S.stage = e.prm.coarse_stride > 1 ? 1 : 0;
+  S.coarse_its = 0;
   e.done[p] = 0;
   if (e.hist) tf_rows_f32(S.T, e.hist + (size_t)p * (e.prm.max_iter + 1) * 12);
   if (e.aa) anderson_reset(e.aa + (size_t)p * kAndersonStride);
@@ -159,6 +161,8 @@ struct IcpShared {
   float m[12];
   float anchor[3];
   uint32_t n_search;      // queries this workgroup searched
+  int stride;             // this iteration's source stride (coarse stage: > 1)
+  int stage_reset;        // the solve moved the problem to all source points
   int item;
   int run;
   int stop;
@@ -348,6 +352,7 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
   }
   __syncthreads();
   if (tid == 0) {
+    sh.stage_reset = 0;
     // all FP64 work of the solve reads and writes LDS (sh.dred, sh.ws, sh.Tinc),
     // so it adds no VGPR pressure to the search loop of this kernel
     const double* vals = sh.dred[0];
@@ -366,24 +371,49 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
       S.iter += 1;
       S.mse = mse;
       double prev = S.prev_mse;
-      const int stt = convergence(e.prm, S.iter, sh.Tinc, mse, &prev);
+      int stt;
+      if (S.stage) {
+        // coarse stage: only the iteration limit ends the registration; an
+        // increment under 1 mm and 1 mrad, the coarse limit or half of the
+        // iteration limit moves it to all source points, with the MSE
+        // history and the acceleration restarted (the iteration map changes)
+        stt = S.iter >= e.prm.max_iter ? RTR_CONV_ITERATIONS : RTR_NOT_CONVERGED;
+        const double cos_angle = 0.5 * (sh.Tinc[0] + sh.Tinc[5] + sh.Tinc[10] - 1.0);
+        const double tsq = sh.Tinc[12] * sh.Tinc[12] + sh.Tinc[13] * sh.Tinc[13] + sh.Tinc[14] * sh.Tinc[14];
+        if ((cos_angle >= 1.0 - 5e-7 && tsq <= 1e-6) || S.iter >= e.prm.coarse_iters ||
+            2 * S.iter >= e.prm.max_iter) {
+          S.stage = 0;
+          S.coarse_its = S.iter;
+          prev = DBL_MAX;
+          sh.stage_reset = 1;
+        }
+      } else {
+        stt = convergence(e.prm, S.iter, sh.Tinc, mse, &prev);
+      }
       S.prev_mse = prev;
       S.state = stt;
       fin = (stt != RTR_NOT_CONVERGED);
       const double* Tnext = sh.Tn;
       if (!fin && gaa) {
-        // the next iterate is extrapolated from the recent ICP steps; iterates
-        // are compared by the truncated energy (points without a
-        // correspondence count r_max^2), which the plain iteration never raises
-        const double N = (double)e.srcs[pr.src].n;
-        const double energy = (vals[SUM_D2] + (N - n) * (double)e.prm.r2f) / N;
-        anderson_pose_step(sh.aa, e.prm.anderson, sh.Tq, sh.Tn, energy, sh.Ta, sh.ws);
-        Tnext = sh.Ta;
+        if (sh.stage_reset) {
+          // first full-resolution iterate: the plain step, a fresh history
+          anderson_reset(sh.aa);
+        } else {
+          // the next iterate is extrapolated from the recent ICP steps; iterates
+          // are compared by the truncated energy (points without a
+          // correspondence count r_max^2), which the plain iteration never raises
+          const uint32_t ns = e.srcs[pr.src].n, st = S.stage ? (uint32_t)e.prm.coarse_stride : 1u;
+          const double N = (double)((ns + st - 1u) / st);
+          const double energy = (vals[SUM_D2] + (N - n) * (double)e.prm.r2f) / N;
+          anderson_pose_step(sh.aa, e.prm.anderson, sh.Tq, sh.Tn, energy, sh.Ta, sh.ws);
+          Tnext = sh.Ta;
+        }
       }
       for (int k = 0; k < 16; ++k) S.T[k] = Tnext[k];
       if (e.hist) tf_rows_f32(Tnext, e.hist + ((size_t)p * (e.prm.max_iter + 1) + S.iter) * 12);
     }
     if (fin) S.finished = 1;
+    if (fin && S.stage) S.coarse_its = S.iter;  // ended on the subsample (no correspondences)
     sh.fin = fin ? 1 : 0;
   }
   __syncthreads();
@@ -449,6 +479,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   // a valid bound, so a seed is only range-checked, and the first iteration
   // of a launch (slots from an earlier launch) starts without one.
   uint32_t* const seeds = !TILE && e.nn_seed ? e.nn_seed + pr.pt_off : nullptr;
+  const int stride = sh.stride;  // coarse stage: every stride-th source point
   const bool seeded = seeds && pass > 0;
   if (kCompact) {
     // Compacted walk (the throughput variant).  Queries that cannot have a
@@ -465,7 +496,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         int cx, cy, cz;
         apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
         uint8_t cls = 0, dv = 0;
-        if (query_cell(g, qx, qy, qz, cx, cy, cz)) {
+        if ((stride == 1 || (chunk + o) % (uint32_t)stride == 0u) && query_cell(g, qx, qy, qz, cx, cy, cz)) {
           const int D = e.cell_dist ? (int)e.cell_dist[td.cell_off + ((uint32_t)cz * g.dy + cy) * g.dx + cx] : 0;
           if (!(e.cell_dist && D >= td.skip_ring)) {
             cls = (uint8_t)(D + 1 >= kQueryClasses ? kQueryClasses : D + 1);
@@ -526,7 +557,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
     if (j < end) {
       const uint32_t sj = sd.off + j;
       apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
-      act = query_cell(g, qx, qy, qz, cx, cy, cz);
+      act = (stride == 1 || j % (uint32_t)stride == 0u) && query_cell(g, qx, qy, qz, cx, cy, cz);
       if (seeded) s0 = seeds[j];
     }
     // distance field: cells within D - 1 rings of the query's cell are empty
@@ -779,6 +810,7 @@ __global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH =
           if (!(wv & kFinishedBit) && (int)wv == pass) {
             run = 1;
             tf_rows_f32(e.st[p].T, sh.m);
+            sh.stride = e.st[p].stage ? e.prm.coarse_stride : 1;
             const TargetDesc& td = e.tgts[e.probs[p].tgt];
             sh.anchor[0] = td.anchor[0];
             sh.anchor[1] = td.anchor[1];

@@ -46,6 +46,8 @@ struct IcpParams {
   double eps_rot;    // rotation epsilon (<= 0: 1 - eps_T)
   double eps_fit;    // euclidean fitness epsilon
   int anderson;      // Anderson acceleration memory (0: plain iteration)
+  int coarse_stride; // > 1: iterations start on every coarse_stride-th source point
+  int coarse_iters;  // most coarse iterations
 };
 
 // Per target cloud (device).  Grid fields are written by the grid build.
@@ -91,6 +93,8 @@ struct ProblemState {
   int state;         // RTR_CONV_* / RTR_FAIL_*
   int finished;
   int gen;           // number of completed iterations visible to waiters
+  int stage;         // 1 while iterating on the coarse source subsample
+  int coarse_its;    // iterations run on it
 };
 
 }  // namespace rtr

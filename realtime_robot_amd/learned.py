@@ -382,7 +382,9 @@ class LearnedRegistration:
 
     refine_anderson > 0 runs the refine launch with Anderson acceleration of
     that memory (rtr_params.anderson_memory): an equally good minimum in
-    fewer iterations; fallback_anderson does the same for the fallback's
+    fewer iterations; refine_coarse_stride > 1 starts it on every s-th source
+    point for up to refine_coarse_iterations iterations (coarse-to-fine,
+    rtr_params.coarse_stride); fallback_anderson does the same for the fallback's
     multi-start launch (0: the plain iteration).  The fallback runs on an
     engine of its own that holds only the rejected pairs' clouds: the refine engine keeps one layout (its hipGraph replays every
     step), and the fallback launch (a few pairs x n_starts problems,
@@ -395,13 +397,16 @@ class LearnedRegistration:
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 1024,
                  max_curvature: float = 0.02, hypotheses: int = 8, tau: float = 0.1, max_pairs: int = 256,
                  refine_anderson: int = 5, fallback_item_pts: int = 256, fallback_blocks_per_cu: int = 1,
-                 fallback_anderson: int = 0, device: int = 0):
+                 fallback_anderson: int = 0, refine_coarse_stride: int = 4, refine_coarse_iterations: int = 25,
+                 device: int = 0):
         if weights is None:
             weights, _ = load_weights()
         self.params = L.Params.from_buffer_copy(params if params is not None else default_params())
         self.params.anderson_memory = 0
         self.refine_params = L.Params.from_buffer_copy(self.params)
         self.refine_params.anderson_memory = refine_anderson
+        self.refine_params.coarse_stride = refine_coarse_stride
+        self.refine_params.coarse_iterations = refine_coarse_iterations
         self._active = "refine"
         self._pool = None
         self.hypotheses = hypotheses

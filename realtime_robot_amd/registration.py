@@ -160,14 +160,17 @@ class IterativeClosestPointWithNormals(IterativeClosestPoint):
 
 def result_dict(r: L.Result) -> dict:
     return dict(T=L.rowmajor(r.T), mse=r.mse, fitness_rmse=r.fitness_rmse, n_corr=r.n_corr,
-                iterations=r.iterations, converged=bool(r.converged), state=r.state)
+                iterations=r.iterations, converged=bool(r.converged), state=r.state,
+                coarse_iterations=r.coarse_iterations)
 
 
 def make_params(max_iterations=10, max_correspondence_distance=float(np.sqrt(DBL_MAX)), transformation_epsilon=0.0,
                 euclidean_fitness_epsilon=-DBL_MAX, rotation_epsilon=0.0, mode=L.MODE_POINT_TO_POINT,
-                min_correspondences=3, cell_size=0.0, cells_per_point=0.0, anderson_memory=0) -> L.Params:
+                min_correspondences=3, cell_size=0.0, cells_per_point=0.0, anderson_memory=0, coarse_stride=0,
+                coarse_iterations=0) -> L.Params:
     """rtr_params from keyword arguments (PCL 1.8's defaults).  anderson_memory
-    > 0 turns on Anderson acceleration of the iteration (no PCL counterpart)."""
+    > 0 turns on Anderson acceleration of the iteration, coarse_stride > 1
+    coarse-to-fine iterations (neither has a PCL counterpart)."""
     p = L.default_params()
     p.max_iterations = max_iterations
     p.max_correspondence_distance = max_correspondence_distance
@@ -179,6 +182,8 @@ def make_params(max_iterations=10, max_correspondence_distance=float(np.sqrt(DBL
     p.cell_size = cell_size
     p.cells_per_point = cells_per_point
     p.anderson_memory = anderson_memory
+    p.coarse_stride = coarse_stride
+    p.coarse_iterations = coarse_iterations
     return p
 
 

@@ -358,3 +358,35 @@ def test_surplus_workgroups_leaving_early_keeps_results():
         for p in range(8):
             assert np.array_equal(res[p]["T"], ref[p]["T"]), (frac, p)
             assert res[p]["iterations"] == ref[p]["iterations"]
+
+
+def test_coarse_to_fine_ends_at_a_full_resolution_fixed_point():
+    """Coarse-to-fine iterations (rtr_params.coarse_stride): the first ones use
+    every 4th source point (at most half of the iteration limit), only
+    full-resolution ones can converge.  The last
+    iteration's correspondences cover every source point, bit-exact against
+    brute force, and one more plain FP64 iteration from the result moves it
+    by no more than the transformation criterion allows."""
+    srcs, tgts, gts = synth.config3_batch(6, seed_offset=800, ns_range=(3000, 8000), nt_range=(10000, 30000),
+                                          max_deg=10.0, max_t=0.05)
+    b = R.BatchRegistration()
+    b.set_clouds(srcs, tgts)
+    b.set_problems(np.arange(6), np.arange(6))
+    for aa in (0, 5):
+        b.set_params(R.make_params(max_iterations=200, max_correspondence_distance=0.05, transformation_epsilon=1e-8,
+                                   anderson_memory=aa, coarse_stride=4, coarse_iterations=50))
+        b.set_tuning(use_tile=0)
+        res = b.align()
+        for p in range(6):
+            r = res[p]
+            assert r["state"] in (R._lib.CONV_TRANSFORM, R._lib.CONV_ABS_MSE), (aa, p, r["state"], r["iterations"])
+            it = r["iterations"]
+            rows = b.history(p, 200)[it - 1]
+            T = np.eye(4)
+            T[:3] = rows
+            idx, d2 = b.last_correspondences(p)
+            i_ref, d_ref = oracle.nn_brute(oracle.transform(T, srcs[p]), tgts[p], 0.05)
+            assert np.array_equal(idx, i_ref) and np.array_equal(d2, d_ref), (aa, p)
+            nxt = oracle.icp(srcs[p], tgts[p], oracle.params(max_iter=1, r_max=0.05), guess=r["T"])
+            print(aa, p, it, synth.rmse_between(nxt["T"], r["T"], srcs[p]))
+            assert synth.rmse_between(nxt["T"], r["T"], srcs[p]) < 2e-4
