@@ -366,15 +366,18 @@ def test_coarse_to_fine_ends_at_a_full_resolution_fixed_point():
     full-resolution ones can converge.  The last
     iteration's correspondences cover every source point, bit-exact against
     brute force, and one more plain FP64 iteration from the result moves it
-    by no more than the transformation criterion allows."""
+    by no more than the transformation criterion allows (point-to-point,
+    plain and accelerated; point-to-plane on the 8-deep search path: its
+    correspondences checked the same way)."""
     srcs, tgts, gts = synth.config3_batch(6, seed_offset=800, ns_range=(3000, 8000), nt_range=(10000, 30000),
                                           max_deg=10.0, max_t=0.05)
     b = R.BatchRegistration()
     b.set_clouds(srcs, tgts)
+    b.compute_target_normals(20)
     b.set_problems(np.arange(6), np.arange(6))
-    for aa in (0, 5):
+    for mode, aa in ((0, 0), (0, 5), (1, 0)):
         b.set_params(R.make_params(max_iterations=200, max_correspondence_distance=0.05, transformation_epsilon=1e-8,
-                                   anderson_memory=aa, coarse_stride=4, coarse_iterations=50))
+                                   anderson_memory=aa, coarse_stride=4, coarse_iterations=50, mode=mode))
         b.set_tuning(use_tile=0)
         res = b.align()
         for p in range(6):
@@ -386,7 +389,9 @@ def test_coarse_to_fine_ends_at_a_full_resolution_fixed_point():
             T[:3] = rows
             idx, d2 = b.last_correspondences(p)
             i_ref, d_ref = oracle.nn_brute(oracle.transform(T, srcs[p]), tgts[p], 0.05)
-            assert np.array_equal(idx, i_ref) and np.array_equal(d2, d_ref), (aa, p)
-            nxt = oracle.icp(srcs[p], tgts[p], oracle.params(max_iter=1, r_max=0.05), guess=r["T"])
-            print(aa, p, it, synth.rmse_between(nxt["T"], r["T"], srcs[p]))
-            assert synth.rmse_between(nxt["T"], r["T"], srcs[p]) < 2e-4
+            assert np.array_equal(idx, i_ref) and np.array_equal(d2, d_ref), (mode, aa, p)
+            assert 0 < r["coarse_iterations"] < it
+            if mode == 0:
+                nxt = oracle.icp(srcs[p], tgts[p], oracle.params(max_iter=1, r_max=0.05), guess=r["T"])
+                print(aa, p, it, r["coarse_iterations"], synth.rmse_between(nxt["T"], r["T"], srcs[p]))
+                assert synth.rmse_between(nxt["T"], r["T"], srcs[p]) < 2e-4
