@@ -1019,8 +1019,10 @@ __device__ void jacobi3(double* a, double* V) {
 __global__ __launch_bounds__(kNrmBlock) void k_target_normals(DevEngine e, int k, float vpx,
                                                               float vpy, float vpz,
                                                               float* nrm_raw) {
-  __shared__ uint64_t s_key[kMaxK * kNrmBlock];
-  __shared__ uint32_t s_slot[kMaxK * kNrmBlock];
+  // k-NN lists in dynamic LDS sized for k (as in k_judge)
+  extern __shared__ uint64_t s_dyn[];
+  uint64_t* const s_key = s_dyn;
+  uint32_t* const s_slot = reinterpret_cast<uint32_t*>(s_dyn + (size_t)k * kNrmBlock);
   const TargetDesc td = e.tgts[blockIdx.y];
   const GridView g = grid_of(e, td);
   const uint32_t base = e.cell_start[td.cell_off];
@@ -1091,7 +1093,8 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
   unsigned bx = (unsigned)((max_tgt_pts + kNrmBlock - 1) / kNrmBlock);
   if (bx < 1) bx = 1;
   if (bx > 1024) bx = 1024;
-  hipLaunchKernelGGL(k_target_normals, dim3(bx, e.n_tgt), dim3(kNrmBlock), 0, s, e, k, vpx, vpy,
+  const size_t lds = (size_t)k * kNrmBlock * (sizeof(uint64_t) + sizeof(uint32_t));
+  hipLaunchKernelGGL(k_target_normals, dim3(bx, e.n_tgt), dim3(kNrmBlock), lds, s, e, k, vpx, vpy,
                      vpz, const_cast<float*>(e.tgt_nrm_raw));
 }
 
@@ -1117,9 +1120,12 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
 // out[4 (p * gridDim.y + s) + {0, 1, 2, 3}] = {sum of squared plane
 // distances, planar inliers, inliers, sampled points}, which the host adds
 // in fixed order.
+// The k-NN lists live in dynamic LDS sized for k (k * kNrmBlock keys, then
+// as many slots): at k = 20 five workgroups fit a CU instead of three.
 __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max_pts, float r2f, double max_curv, double* out) {
-  __shared__ uint64_t s_key[kMaxK * kNrmBlock];
-  __shared__ uint32_t s_slot[kMaxK * kNrmBlock];
+  extern __shared__ uint64_t s_dyn[];
+  uint64_t* const s_key = s_dyn;
+  uint32_t* const s_slot = reinterpret_cast<uint32_t*>(s_dyn + (size_t)k * kNrmBlock);
   __shared__ double s_red[3][kNrmBlock / 64];
   const int prob = blockIdx.x;
   const Problem pr = e.probs[prob];
@@ -1208,7 +1214,8 @@ void launch_judge(const DevEngine& e, int k, int max_pts, float r2f, double max_
                   hipStream_t s) {
   if (e.n_prob == 0) return;
   if (k > kMaxK) k = kMaxK;
-  hipLaunchKernelGGL(k_judge, dim3(e.n_prob, kJudgeSplit), dim3(kNrmBlock), 0, s, e, k, max_pts, r2f, max_curv,
+  const size_t lds = (size_t)k * kNrmBlock * (sizeof(uint64_t) + sizeof(uint32_t));
+  hipLaunchKernelGGL(k_judge, dim3(e.n_prob, kJudgeSplit), dim3(kNrmBlock), lds, s, e, k, max_pts, r2f, max_curv,
                      out);
 }
 
