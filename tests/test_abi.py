@@ -274,3 +274,20 @@ def test_rotation_chart_matches_quaternion_matrix():
     Rm, vb = np.empty(9), np.empty(3)
     for bad in ([0.8, 0.7, 0.0], [0.0, 0.0, 0.9995]):
         assert L.lib.rtr_selfcheck_so3(L.ptr(np.array(bad)), L.ptr(Rm), L.ptr(vb)) != 0
+
+
+def test_result_conversion_matches_per_entry():
+    """The batch results' numpy conversion equals the per-entry one."""
+    from realtime_robot_amd.registration import result_dict, result_dicts
+    res = (R._lib.Result * 5)()
+    rng = np.random.default_rng(3)
+    for i in range(5):
+        for k in range(16):
+            res[i].T[k] = rng.normal()
+        res[i].mse, res[i].fitness_rmse, res[i].n_corr = rng.random(), rng.random(), i * 7
+        res[i].iterations, res[i].converged, res[i].state, res[i].coarse_iterations = i + 3, i % 2, i % 4, i
+    for a, b in zip(result_dicts(res), [result_dict(r) for r in res]):
+        assert a.keys() == b.keys()
+        assert np.array_equal(a["T"], b["T"])
+        assert all(a[k] == b[k] for k in a if k != "T")
+    assert result_dicts((R._lib.Result * 0)()) == []
