@@ -174,12 +174,35 @@ __global__ void k_cloud_bbox(DevEngine e) {
     h[a] = wave_max(h[a]);
   }
   cnt = wave_sum_u32(cnt);
-  if ((threadIdx.x & 63) == 0 && cnt) {
+  // the workgroup's waves combine in LDS first: one set of atomics per
+  // workgroup instead of per wave (every workgroup of a cloud updates the
+  // same seven words, so these atomics serialise)
+  __shared__ float s_l[kBlock / 64][3], s_h[kBlock / 64][3];
+  __shared__ uint32_t s_c[kBlock / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&bb[a], ord_key(l[a]));
-      atomicMax(&bb[3 + a], ord_key(h[a]));
+      s_l[w][a] = l[a];
+      s_h[w][a] = h[a];
     }
-    atomicAdd(&bb[6], cnt);
+    s_c[w] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 1; v < kBlock / 64; ++v) {
+      for (int a = 0; a < 3; ++a) {
+        l[a] = fminf(l[a], s_l[v][a]);
+        h[a] = fmaxf(h[a], s_h[v][a]);
+      }
+      cnt += s_c[v];
+    }
+    if (cnt) {
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(&bb[a], ord_key(l[a]));
+        atomicMax(&bb[3 + a], ord_key(h[a]));
+      }
+      atomicAdd(&bb[6], cnt);
+    }
   }
 }
 
@@ -352,9 +375,9 @@ __global__ void k_tgt_dist(DevEngine e, const uint8_t* src, uint8_t* dst) {
   }
 }
 
-// workgroups per cloud of a bounding-box pass: about 8 points per thread
+// workgroups per cloud of a bounding-box pass: about 16 points per thread
 static unsigned bbox_blocks_for(int max_pts) {
-  unsigned b = (unsigned)((max_pts + 8 * kBlock - 1) / (8 * kBlock));
+  unsigned b = (unsigned)((max_pts + 16 * kBlock - 1) / (16 * kBlock));
   return b < 1 ? 1 : (b > 128 ? 128 : b);
 }
 
@@ -460,7 +483,9 @@ __global__ void k_src_scatter(DevEngine e) {
 // consecutive points stay spatially compact below the bin size.  Ranks are
 // found by counting (O(n^2 / 64) per bin, n ~ tens); a bin larger than the
 // wave's LDS tile (1024 points) is ordered by original index alone by one
-// lane.
+// lane.  Only the keys are staged (8 KB per wave, 20 waves per CU): a point is
+// read back from the unsorted input by its original index when it is written
+// to its rank.
 constexpr int kBinCap = 1024;
 
 __device__ __forceinline__ uint32_t spread3_4(uint32_t v) {
@@ -471,8 +496,8 @@ __device__ __forceinline__ uint32_t spread3_4(uint32_t v) {
 }
 
 // One wave orders one bin.
-__device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32_t total, int lane, int w,
-                                          uint64_t (*s_key)[kBinCap], float4 (*s_pt)[kBinCap]) {
+__device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32_t total, int lane,
+                                          uint64_t* s_key) {
   const uint32_t beg = e.bin_start[b];
   const uint32_t end = (b + 1 < e.n_bins_total) ? e.bin_start[b + 1] : total;
   const uint32_t n = end - beg;
@@ -522,32 +547,31 @@ __device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32
       }
       fine = spread3_4(sub[0]) | (spread3_4(sub[1]) << 1) | (spread3_4(sub[2]) << 2);
     }
-    s_key[w][i] = ((uint64_t)fine << 32) | id;
-    s_pt[w][i] = make_float4(x, y, z, __int_as_float((int)id));
+    s_key[i] = ((uint64_t)fine << 32) | id;
   }
   __builtin_amdgcn_wave_barrier();
+  const float* raw = e.src_raw + 3 * (size_t)sd.off;
   for (uint32_t i = lane; i < n; i += 64) {
-    const uint64_t k = s_key[w][i];
+    const uint64_t k = s_key[i];
     uint32_t r = 0;
-    for (uint32_t j = 0; j < n; ++j) r += (s_key[w][j] < k) ? 1u : 0u;
-    const float4 pt = s_pt[w][i];
-    e.sx[beg + r] = pt.x;
-    e.sy[beg + r] = pt.y;
-    e.sz[beg + r] = pt.z;
-    e.perm[beg + r] = (uint32_t)__float_as_int(pt.w);
+    for (uint32_t j = 0; j < n; ++j) r += (s_key[j] < k) ? 1u : 0u;
+    const uint32_t id = (uint32_t)k;
+    e.sx[beg + r] = raw[3 * (size_t)id];
+    e.sy[beg + r] = raw[3 * (size_t)id + 1];
+    e.sz[beg + r] = raw[3 * (size_t)id + 2];
+    e.perm[beg + r] = id;
   }
   __builtin_amdgcn_wave_barrier();  // the LDS slots are reused by the wave's next bin
 }
 
 constexpr uint32_t kOrderBlocks = 16384;
 
-// One wave per workgroup (24 KB of LDS each).  Waves stride over groups of 64
+// One wave per workgroup (8 KB of LDS each).  Waves stride over groups of 64
 // bins: the lanes read the 64 bins' sizes at once, and the wave then orders
 // the bins of more than one point one after another (most bins of a sparse
 // surface hold none or one, so they cost one coalesced read).
 __global__ __launch_bounds__(64) void k_src_order_bins(DevEngine e, uint32_t total) {
-  __shared__ uint64_t s_key[1][kBinCap];
-  __shared__ float4 s_pt[1][kBinCap];
+  __shared__ uint64_t s_key[kBinCap];
   const int lane = threadIdx.x & 63;
   for (uint32_t b0 = blockIdx.x * 64u; b0 < e.n_bins_total; b0 += gridDim.x * 64u) {
     const uint32_t b = b0 + (uint32_t)lane;
@@ -557,7 +581,7 @@ __global__ __launch_bounds__(64) void k_src_order_bins(DevEngine e, uint32_t tot
     while (todo) {
       const uint32_t l = (uint32_t)__builtin_ctzll(todo);
       todo &= todo - 1ull;
-      order_bin(e, b0 + l, total, lane, 0, s_key, s_pt);
+      order_bin(e, b0 + l, total, lane, s_key);
     }
   }
 }
