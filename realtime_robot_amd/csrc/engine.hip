@@ -69,8 +69,8 @@ Engine::~Engine() {
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&tgt_raw_, &tgt_nrm_raw_, &tgt_sorted_, &tgt_nrm_sorted_, &cell_count_,
-                    &cell_start_, &cell_id_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &bbox_tmp_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
-                    &bin_count_, &bin_start_, &bin_id_, &src_desc_, &prob_desc_, &state_,
+                    &cell_start_, &cell_id_, &cell_rank_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &bbox_tmp_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
+                    &bin_count_, &bin_start_, &bin_id_, &bin_rank_, &src_desc_, &prob_desc_, &state_,
                     &guesses_, &items_buf_, &partials_, &done_, &counters_, &scan_tmp_,
                     &out_i32_, &out_f32_, &out_T_, &hist_, &aa_, &nn_seed_};
   for (DevBuf* b : bufs) b->release();
@@ -145,6 +145,7 @@ int Engine::set_clouds(int n_src, const float* const* src, const size_t* ns, con
     RTR_TRY(sz_.ensure(st * 4));
     RTR_TRY(perm_.ensure(st * 4));
     RTR_TRY(bin_id_.ensure(st * 4));
+    RTR_TRY(bin_rank_.ensure(st * 4));
     RTR_TRY(bin_count_.ensure((size_t)n_bins_ * 4));
     RTR_TRY(bin_start_.ensure((size_t)n_bins_ * 4));
   }
@@ -168,6 +169,7 @@ int Engine::set_clouds(int n_src, const float* const* src, const size_t* ns, con
     RTR_HIP(hipStreamSynchronize(stream_));
     RTR_TRY(tgt_sorted_.ensure(tt * 16));
     RTR_TRY(cell_id_.ensure(tt * 4));
+    RTR_TRY(cell_rank_.ensure(tt * 4));
     normals_valid_ = false;
   }
   grid_dirty_ = true;
@@ -384,6 +386,7 @@ DevEngine Engine::dev_view() const {
   e.cell_count = cell_count_.as<uint32_t>();
   e.cell_start = cell_start_.as<uint32_t>();
   e.cell_id = cell_id_.as<uint32_t>();
+  e.cell_rank = cell_rank_.as<uint32_t>();
   e.cell_dist = cell_dist_.as<uint8_t>();
   e.cell_dist_tmp = cell_dist_tmp_.as<uint8_t>();
   e.r_max = prm_.max_correspondence_distance < 3.0e38 ? (float)prm_.max_correspondence_distance : 3.0e38f;
@@ -398,6 +401,7 @@ DevEngine Engine::dev_view() const {
   e.bin_count = bin_count_.as<uint32_t>();
   e.bin_start = bin_start_.as<uint32_t>();
   e.bin_id = bin_id_.as<uint32_t>();
+  e.bin_rank = bin_rank_.as<uint32_t>();
   e.n_bins_total = n_bins_;
   e.probs = prob_desc_.as<Problem>();
   e.n_prob = (int)probs_.size();

@@ -140,10 +140,10 @@ class Engine {
   size_t counter_words() const { return 96 + (size_t)prm_.max_iterations * kXcds * claim_stride(); }
 
   // device arena
-  DevBuf tgt_raw_, tgt_nrm_raw_, tgt_sorted_, tgt_nrm_sorted_, cell_count_, cell_start_, cell_id_;
+  DevBuf tgt_raw_, tgt_nrm_raw_, tgt_sorted_, tgt_nrm_sorted_, cell_count_, cell_start_, cell_id_, cell_rank_;
   DevBuf cell_dist_, cell_dist_tmp_;
   DevBuf tgt_desc_, bbox_tmp_;
-  DevBuf src_raw_, sx_, sy_, sz_, perm_, bin_count_, bin_start_, bin_id_, src_desc_;
+  DevBuf src_raw_, sx_, sy_, sz_, perm_, bin_count_, bin_start_, bin_id_, bin_rank_, src_desc_;
   DevBuf prob_desc_, state_, guesses_, items_buf_, partials_, done_, counters_, scan_tmp_;
   DevBuf out_i32_, out_f32_, out_T_, hist_, aa_, nn_seed_;
   // pinned host staging

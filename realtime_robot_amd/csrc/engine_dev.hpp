@@ -23,6 +23,7 @@ struct DevEngine {
   uint32_t* cell_count;
   uint32_t* cell_start;
   uint32_t* cell_id;         // per raw target point
+  uint32_t* cell_rank;       // per raw target point: its slot within its cell
   uint8_t* cell_dist;        // per cell: Chebyshev distance (in cells) to the nearest occupied cell
   uint8_t* cell_dist_tmp;    // ping-pong buffer of the separable distance transform
   float r_max;               // max correspondence distance (inf when unbounded)
@@ -38,6 +39,7 @@ struct DevEngine {
   uint32_t* bin_count;
   uint32_t* bin_start;
   uint32_t* bin_id;          // per raw source point
+  uint32_t* bin_rank;        // per raw source point: its slot within its bin
   uint32_t n_bins_total;
   // problems and the work queue
   const Problem* probs;

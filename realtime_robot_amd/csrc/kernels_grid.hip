@@ -316,7 +316,7 @@ __global__ void k_tgt_count(DevEngine e) {
     uint32_t c;
     if (cell_of(t, xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], &c)) {
       e.cell_id[t.raw_off + i] = c;
-      atomicAdd(&e.cell_count[t.cell_off + c], 1u);
+      e.cell_rank[t.raw_off + i] = atomicAdd(&e.cell_count[t.cell_off + c], 1u);
     } else {
       e.cell_id[t.raw_off + i] = 0xffffffffu;
     }
@@ -330,7 +330,8 @@ __global__ void k_tgt_scatter(DevEngine e) {
     const uint32_t c = e.cell_id[t.raw_off + i];
     if (c == 0xffffffffu) continue;
     const uint32_t slot = t.cell_off + c;
-    const uint32_t pos = e.cell_start[slot] + atomicSub(&e.cell_count[slot], 1u) - 1u;
+    // the count pass handed every point its slot within the cell
+    const uint32_t pos = e.cell_start[slot] + e.cell_rank[t.raw_off + i];
     union {
       int32_t i;
       float f;
@@ -457,7 +458,7 @@ __global__ void k_src_count(DevEngine e) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     uint32_t b = bin_of(s, xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]);
     e.bin_id[s.off + i] = b;
-    atomicAdd(&e.bin_count[s.bin_off + b], 1u);
+    e.bin_rank[s.off + i] = atomicAdd(&e.bin_count[s.bin_off + b], 1u);
   }
 }
 
@@ -468,7 +469,7 @@ __global__ void k_src_scatter(DevEngine e) {
     const uint32_t slot = s.bin_off + e.bin_id[s.off + i];
     // bin_start is global over all sources; every source point is placed, so
     // source s occupies exactly [s.off, s.off + s.n) of the sorted arrays
-    const uint32_t pos = e.bin_start[slot] + atomicSub(&e.bin_count[slot], 1u) - 1u;
+    const uint32_t pos = e.bin_start[slot] + e.bin_rank[s.off + i];
     e.sx[pos] = xyz[3 * (size_t)i];
     e.sy[pos] = xyz[3 * (size_t)i + 1];
     e.sz[pos] = xyz[3 * (size_t)i + 2];
