@@ -496,6 +496,35 @@ __device__ __forceinline__ uint32_t spread3_4(uint32_t v) {
   return r;
 }
 
+// the source owning bin b (bins of consecutive sources are consecutive)
+__device__ __forceinline__ int source_of_bin(const DevEngine& e, uint32_t b) {
+  int lo = 0, hi = e.n_src - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (e.srcs[mid].bin_off <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// (Morton code of the point inside its bin at 16 sub-cells per axis, original index)
+__device__ __forceinline__ uint64_t fine_key(const SourceDesc& sd, float x, float y, float z, uint32_t id) {
+  uint32_t fine = 0xfffu;
+  if (isfinite(x) && isfinite(y) && isfinite(z)) {
+    const int P = sd.dx;
+    const float f[3] = {(x - sd.ox) * sd.inv_h, (y - sd.oy) * sd.inv_h, (z - sd.oz) * sd.inv_h};
+    uint32_t sub[3];
+    for (int a = 0; a < 3; ++a) {
+      const int c = clampi((int)floorf(f[a]), 0, P - 1);
+      sub[a] = (uint32_t)clampi((int)floorf((f[a] - (float)c) * 16.0f), 0, 15);
+    }
+    fine = spread3_4(sub[0]) | (spread3_4(sub[1]) << 1) | (spread3_4(sub[2]) << 2);
+  }
+  return ((uint64_t)fine << 32) | id;
+}
+
 // One wave orders one bin.
 __device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32_t total, int lane,
                                           uint64_t* s_key) {
@@ -524,32 +553,8 @@ __device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32
     }
     return;
   }
-  // the source owning bin b (bins of consecutive sources are consecutive)
-  int lo = 0, hi = e.n_src - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (e.srcs[mid].bin_off <= b)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const SourceDesc sd = e.srcs[lo];
-  const int P = sd.dx;
-  for (uint32_t i = lane; i < n; i += 64) {
-    const float x = e.sx[beg + i], y = e.sy[beg + i], z = e.sz[beg + i];
-    const uint32_t id = e.perm[beg + i];
-    uint32_t fine = 0xfffu;
-    if (isfinite(x) && isfinite(y) && isfinite(z)) {
-      const float f[3] = {(x - sd.ox) * sd.inv_h, (y - sd.oy) * sd.inv_h, (z - sd.oz) * sd.inv_h};
-      uint32_t sub[3];
-      for (int a = 0; a < 3; ++a) {
-        const int c = clampi((int)floorf(f[a]), 0, P - 1);
-        sub[a] = (uint32_t)clampi((int)floorf((f[a] - (float)c) * 16.0f), 0, 15);
-      }
-      fine = spread3_4(sub[0]) | (spread3_4(sub[1]) << 1) | (spread3_4(sub[2]) << 2);
-    }
-    s_key[i] = ((uint64_t)fine << 32) | id;
-  }
+  const SourceDesc sd = e.srcs[source_of_bin(e, b)];
+  for (uint32_t i = lane; i < n; i += 64) s_key[i] = fine_key(sd, e.sx[beg + i], e.sy[beg + i], e.sz[beg + i], e.perm[beg + i]);
   __builtin_amdgcn_wave_barrier();
   const float* raw = e.src_raw + 3 * (size_t)sd.off;
   for (uint32_t i = lane; i < n; i += 64) {
@@ -565,6 +570,45 @@ __device__ __forceinline__ void order_bin(const DevEngine& e, uint32_t b, uint32
   __builtin_amdgcn_wave_barrier();  // the LDS slots are reused by the wave's next bin
 }
 
+// One wave orders the 64 consecutive bins b0 .. b0 + nb - 1 of one source
+// together, their points at positions [gbeg, gbeg + N) with N <= kBinCap:
+// each point is ranked against the keys of its own bin only, so the result is
+// the per-bin order, but the wave pays one load-rank-store chain per group
+// instead of one per bin.  st = this lane's bin start, relative to gbeg
+// (N for lanes past the group).
+__device__ __forceinline__ void order_group(const DevEngine& e, const SourceDesc& sd, uint32_t gbeg, uint32_t N,
+                                            uint32_t st, int lane, uint64_t* s_key, uint32_t* s_start,
+                                            uint8_t* s_bin) {
+  s_start[lane] = st;
+  if (lane == 0) s_start[64] = N;
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t p = lane; p < N; p += 64) {
+    // the bin holding position p: the last k with s_start[k] <= p (k < 64 as p < N)
+    int k = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+      if (s_start[k + step] <= p) k += step;
+    s_bin[p] = (uint8_t)k;
+    s_key[p] = fine_key(sd, e.sx[gbeg + p], e.sy[gbeg + p], e.sz[gbeg + p], e.perm[gbeg + p]);
+  }
+  __builtin_amdgcn_wave_barrier();
+  const float* raw = e.src_raw + 3 * (size_t)sd.off;
+  for (uint32_t p = lane; p < N; p += 64) {
+    const int k = s_bin[p];
+    const uint32_t lo = s_start[k], hi = s_start[k + 1];
+    const uint64_t key = s_key[p];
+    uint32_t r = 0;
+    for (uint32_t j = lo; j < hi; ++j) r += (s_key[j] < key) ? 1u : 0u;
+    const uint32_t id = (uint32_t)key;
+    const uint32_t pos = gbeg + lo + r;
+    e.sx[pos] = raw[3 * (size_t)id];
+    e.sy[pos] = raw[3 * (size_t)id + 1];
+    e.sz[pos] = raw[3 * (size_t)id + 2];
+    e.perm[pos] = id;
+  }
+  __builtin_amdgcn_wave_barrier();  // the LDS arrays are reused by the wave's next group
+}
+
 constexpr uint32_t kOrderBlocks = 16384;
 
 // One wave per workgroup (8 KB of LDS each).  Waves stride over groups of 64
@@ -573,12 +617,29 @@ constexpr uint32_t kOrderBlocks = 16384;
 // surface hold none or one, so they cost one coalesced read).
 __global__ __launch_bounds__(64) void k_src_order_bins(DevEngine e, uint32_t total) {
   __shared__ uint64_t s_key[kBinCap];
+  __shared__ uint32_t s_start[65];
+  __shared__ uint8_t s_bin[kBinCap];
   const int lane = threadIdx.x & 63;
   for (uint32_t b0 = blockIdx.x * 64u; b0 < e.n_bins_total; b0 += gridDim.x * 64u) {
     const uint32_t b = b0 + (uint32_t)lane;
-    uint32_t n = 0u;
-    if (b < e.n_bins_total) n = ((b + 1 < e.n_bins_total) ? e.bin_start[b + 1] : total) - e.bin_start[b];
+    const uint32_t nb = min(64u, e.n_bins_total - b0);
+    uint32_t st = 0u, n = 0u;
+    if (b < e.n_bins_total) {
+      st = e.bin_start[b];
+      n = ((b + 1 < e.n_bins_total) ? e.bin_start[b + 1] : total) - st;
+    }
     uint64_t todo = __ballot(n > 1u);
+    if (!todo) continue;
+    // the whole group at once when it fits the LDS tile and one source owns it
+    const uint32_t gbeg = __shfl(st, 0);
+    const uint32_t gend = __shfl(st + n, (int)nb - 1);
+    const int si = source_of_bin(e, b0);
+    const SourceDesc& sd = e.srcs[si];
+    if (gend - gbeg <= (uint32_t)kBinCap && b0 + nb <= sd.bin_off + sd.bin_budget) {
+      order_group(e, sd, gbeg, gend - gbeg, (uint32_t)lane < nb ? st - gbeg : gend - gbeg, lane, s_key, s_start,
+                  s_bin);
+      continue;
+    }
     while (todo) {
       const uint32_t l = (uint32_t)__builtin_ctzll(todo);
       todo &= todo - 1ull;
