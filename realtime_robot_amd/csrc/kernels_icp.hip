@@ -1142,8 +1142,6 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     const uint32_t sj = sd.off + (uint32_t)(((uint64_t)j * n) / ms);
     float qx, qy, qz;
     apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
-    const NNBest b = nn_search(g, qx, qy, qz, r2f);
-    if (b.key == ~0ull) continue;
     KnnVisitor v;
     v.pts = e.tgt_sorted;
     v.qx = qx;
@@ -1153,7 +1151,17 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     v.cnt = 0;
     v.key = s_key + threadIdx.x;
     v.slot = s_slot + threadIdx.x;
-    ring_walk(g, qx, qy, qz, FLT_MAX, v);
+    // One walk bounded by the correspondence radius visits every target point
+    // within it, so its nearest key is the exact nearest neighbour whenever
+    // one lies within the radius (the inlier test), and when the k-th key is
+    // within the radius too the list is the exact k-NN list.  Only an inlier
+    // with fewer than k target points within the radius walks again, unbounded.
+    ring_walk(g, qx, qy, qz, r2f, v);
+    if (v.cnt == 0 || !(key_d2(v.key[0]) <= r2f)) continue;
+    if (v.cnt < k || !(key_d2(v.key[(k - 1) * kNrmBlock]) <= r2f)) {
+      v.cnt = 0;
+      ring_walk(g, qx, qy, qz, FLT_MAX, v);
+    }
     cnt += 1.0;
     if (v.cnt < 3) continue;
     double mean[3] = {0, 0, 0};
