@@ -956,7 +956,7 @@ struct KnnVisitor {
   float qx, qy, qz;
   int k, cnt;
   uint64_t* key;   // LDS, stride kNrmBlock
-  uint32_t* slot;  // LDS, stride kNrmBlock
+  uint32_t* slot;  // LDS, stride kNrmBlock; nullptr keeps keys only (their low word is the point's original index)
   __device__ void scan(uint32_t b, uint32_t e) {
     for (uint32_t j = b; j < e; ++j) {
       const float4 t = pts[j];
@@ -970,11 +970,11 @@ struct KnnVisitor {
       int pos = cnt < k ? cnt++ : k - 1;
       while (pos > 0 && kk < key[(pos - 1) * kNrmBlock]) {
         key[pos * kNrmBlock] = key[(pos - 1) * kNrmBlock];
-        slot[pos * kNrmBlock] = slot[(pos - 1) * kNrmBlock];
+        if (slot) slot[pos * kNrmBlock] = slot[(pos - 1) * kNrmBlock];
         --pos;
       }
       key[pos * kNrmBlock] = kk;
-      slot[pos * kNrmBlock] = j;
+      if (slot) slot[pos * kNrmBlock] = j;
     }
   }
   __device__ float bound() const { return cnt < k ? 3.4e38f : key_d2(key[(k - 1) * kNrmBlock]); }
@@ -1120,13 +1120,13 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
 // out[4 (p * gridDim.y + s) + {0, 1, 2, 3}] = {sum of squared plane
 // distances, planar inliers, inliers, sampled points}, which the host adds
 // in fixed order.
-// The k-NN lists live in dynamic LDS sized for k (k * kNrmBlock keys, then
-// as many slots): at k = 20 five workgroups fit a CU instead of three.
+// The k-NN lists live in dynamic LDS sized for k and hold keys only (k *
+// kNrmBlock keys; a neighbour's coordinates are read back from the raw target
+// cloud by the original index in its key's low word): at k = 20 eight
+// workgroups fit a CU.
 __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max_pts, float r2f, double max_curv, double* out) {
   extern __shared__ uint64_t s_dyn[];
   uint64_t* const s_key = s_dyn;
-  uint32_t* const s_slot = reinterpret_cast<uint32_t*>(s_dyn + (size_t)k * kNrmBlock);
-  __shared__ double s_red[3][kNrmBlock / 64];
   const int prob = blockIdx.x;
   const Problem pr = e.probs[prob];
   const SourceDesc sd = e.srcs[pr.src];
@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     v.k = k;
     v.cnt = 0;
     v.key = s_key + threadIdx.x;
-    v.slot = s_slot + threadIdx.x;
+    v.slot = nullptr;
     // One walk bounded by the correspondence radius visits every target point
     // within it, so its nearest key is the exact nearest neighbour whenever
     // one lies within the radius (the inlier test), and when the k-th key is
@@ -1164,18 +1164,19 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     }
     cnt += 1.0;
     if (v.cnt < 3) continue;
+    const float* const raw = e.tgt_raw + 3 * (size_t)td.raw_off;
     double mean[3] = {0, 0, 0};
     for (int i = 0; i < v.cnt; ++i) {
-      const float4 p = e.tgt_sorted[v.slot[i * kNrmBlock]];
-      mean[0] += p.x;
-      mean[1] += p.y;
-      mean[2] += p.z;
+      const float* p = raw + 3 * (size_t)(uint32_t)v.key[i * kNrmBlock];
+      mean[0] += p[0];
+      mean[1] += p[1];
+      mean[2] += p[2];
     }
     for (int a = 0; a < 3; ++a) mean[a] /= v.cnt;
     double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < v.cnt; ++i) {
-      const float4 p = e.tgt_sorted[v.slot[i * kNrmBlock]];
-      const double d[3] = {p.x - mean[0], p.y - mean[1], p.z - mean[2]};
+      const float* p = raw + 3 * (size_t)(uint32_t)v.key[i * kNrmBlock];
+      const double d[3] = {p[0] - mean[0], p[1] - mean[1], p[2] - mean[2]};
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) C[r * 3 + c] += d[r] * d[c];
     }
@@ -1196,6 +1197,11 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     cnt += __shfl_xor(cnt, off, 64);
     pcnt += __shfl_xor(pcnt, off, 64);
   }
+  // the partials reuse the start of the list area (k >= 3 lists hold far more
+  // than 3 doubles per wave) once every thread is done with its list: no
+  // static LDS, so eight 20 KB workgroups fill a CU's 160 KB at k = 20
+  double(*s_red)[kNrmBlock / 64] = reinterpret_cast<double(*)[kNrmBlock / 64]>(s_dyn);
+  __syncthreads();
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     s_red[0][w] = ssum;
@@ -1222,7 +1228,7 @@ void launch_judge(const DevEngine& e, int k, int max_pts, float r2f, double max_
                   hipStream_t s) {
   if (e.n_prob == 0) return;
   if (k > kMaxK) k = kMaxK;
-  const size_t lds = (size_t)k * kNrmBlock * (sizeof(uint64_t) + sizeof(uint32_t));
+  const size_t lds = (size_t)k * kNrmBlock * sizeof(uint64_t);
   hipLaunchKernelGGL(k_judge, dim3(e.n_prob, kJudgeSplit), dim3(kNrmBlock), lds, s, e, k, max_pts, r2f, max_curv,
                      out);
 }
