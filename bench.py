@@ -16,8 +16,18 @@ refine of every pair's best proposal (Anderson-accelerated, memory 5, unless
 pairs the judge rejects + best-start selection.  One step of the
 plain path = 8-start ICP of every pair (starts spread over SO(3)) + best-start
 selection.  A registration counts when its pose is within 1 mm RMSE (over
-the source points) of the reference pose: the GPU ICP started at the
-generating pose, itself checked against the FP64 CPU oracle on a sample.
+the source points) of two FP64 CPU oracle poses (oracle/oracle.c orc_icp),
+both computed for every pair outside the timed region: the oracle's ICP from
+the same init (the pair's proposal, or for a rejected pair its fallback's
+best start; the plain path's best start), and the oracle's ICP from the
+generating pose (the correct registration).  The JSON line states every
+registration's error against both.
+
+Also measured: the learned step with the clouds uploaded from host memory
+inside the timed region (config.value_with_upload), batched plain
+single-start ICP on the config-3 workload (config3_plain_icp), the
+PCL-semantics CPU ICP on the host's cores (cpu_baseline) and its 1-thread
+latency next to the GPU's on the config-1 pair.
 
 Prints one JSON line.
 """
@@ -46,56 +56,110 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(srcs, tgts, starts, n_starts, budget_s):
-    """PCL-semantics CPU ICP (float kd-tree, leaf 15, one start per thread) of
-    the plain multi-start path on a bounded sample of the same pairs."""
+def host_threads():
+    """Worker threads for the CPU legs: the host's CPU share, at most 16 (one
+    GPU's share of the host, whatever nproc shows)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def fp64_icp(jobs, threads):
+    """FP64 oracle ICP (oracle/oracle.c orc_icp: exact NN, FP64 moments and
+    Umeyama SVD, PCL 1.8's stop rules) for (source, target, init) jobs, on
+    `threads` host threads; returns the final transforms."""
     import oracle
-    cores = min(16, os.cpu_count() or 1)
     p = oracle.params(max_iter=MAX_ITER, r_max=R_MAX, eps_T=EPS)
-    done, t_cpu = 0, 0.0
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda j: oracle.icp(j[0], j[1], p, guess=j[2])["T"], jobs))
+
+
+def counted(T, T64_init, T64_gt, srcs):
+    """Per registration: RMSE (over the source points) to the FP64 oracle's
+    ICP from the same init and to the FP64 oracle's ICP from the generating
+    pose; a registration counts when both are within OK_RMSE."""
+    e_init = np.array([synth_rmse(T[i], T64_init[i], srcs[i]) for i in range(len(T))])
+    e_gt = np.array([synth_rmse(T[i], T64_gt[i], srcs[i]) for i in range(len(T))])
+    return (e_init <= OK_RMSE) & (e_gt <= OK_RMSE), e_init, e_gt
+
+
+def synth_rmse(Ta, Tb, pts):
+    from realtime_robot_amd import synth
+    return synth.rmse_between(Ta, Tb, pts)
+
+
+def stats_m(e):
+    e = np.asarray(e, np.float64)
+    return {"max": float(e.max()), "median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+            "n_within_1e-5": int(np.sum(e <= 1e-5))}
+
+
+def sig(e, digits=3):
+    return [float(f"{x:.{digits}g}") for x in np.asarray(e, np.float64)]
+
+
+def cpu_baseline(srcs, tgts, starts, n_starts, budget_s, T64_gt, threads):
+    """PCL-semantics CPU ICP (oracle/oracle.c orc_pcl_icp_batch: float kd-tree
+    of leaf 15, float Umeyama, PCL 1.8's stop rules) of the plain multi-start
+    path on a bounded sample of the same pairs, one start per thread, best
+    start as on the GPU (multistart.select_best).  Registrations count on the
+    basis of `value`: within 1 mm of the FP64 oracle's ICP from the same best
+    start and of its ICP from the generating pose."""
+    import oracle
+    from realtime_robot_amd import multistart
+    p = oracle.params(max_iter=MAX_ITER, r_max=R_MAX, eps_T=EPS)
+    done, t_cpu, per_batch = 0, 0.0, max(threads // n_starts, 1)
+    results = []
     while done < len(srcs) and t_cpu < budget_s:
-        batch = list(range(done, min(done + max(cores // n_starts, 1), len(srcs))))
+        batch = list(range(done, min(done + per_batch, len(srcs))))
         S, T, G = [], [], []
         for i in batch:
-            for s in range(n_starts):
+            for k in range(n_starts):
                 S.append(srcs[i])
                 T.append(tgts[i])
-                G.append(starts[i][s])
+                G.append(starts[i][k])
         t0 = time.perf_counter()
-        oracle.pcl_icp_batch(S, T, p, G, nthreads=cores)
+        res = oracle.pcl_icp_batch(S, T, p, G, nthreads=threads)
         t_cpu += time.perf_counter() - t0
+        for j, i in enumerate(batch):
+            r = res[j * n_starts:(j + 1) * n_starts]
+            b = multistart.select_best(r)
+            results.append((i, r[b]["T"], starts[i][b]))
         done += len(batch)
-    return {"value": done / t_cpu, "unit": "registrations/s", "cores": cores, "kind": "port",
+    T64 = fp64_icp([(srcs[i], tgts[i], g) for i, _, g in results], threads)
+    ok = [synth_rmse(T, T64[j], srcs[i]) <= OK_RMSE and synth_rmse(T, T64_gt[i], srcs[i]) <= OK_RMSE
+          for j, (i, T, _) in enumerate(results)]
+    # per-pair latency, one thread: the config-1 chair pair, 1 warm-up + median of 5
+    from realtime_robot_amd import synth
+    s1, t1, _ = synth.config1_pair(0)
+    lat = []
+    for k in range(6):
+        t0 = time.perf_counter()
+        oracle.pcl_icp(s1, t1, p)
+        if k:
+            lat.append(time.perf_counter() - t0)
+    return {"value": int(sum(ok)) / t_cpu, "unit": "registrations/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "attempted_per_s": done / t_cpu, "ok": int(sum(ok)), "attempted": done,
+            "latency_ms_config1_1thread": float(np.median(lat)) * 1e3,
             "sample": f"first {done} pairs x {n_starts} starts of the same workload (plain multi-start), "
                       f"PCL-semantics float kd-tree ICP (oracle/oracle.c orc_pcl_icp_batch), one start per "
-                      f"thread, {t_cpu:.1f} s"}
-
-
-def fp64_check(srcs, tgts, gts, T_ref, inits, T_plain, T_learned, n_chk):
-    """FP64 oracle (plain PCL-semantics ICP) on a sample of pairs:
-    (a) from the generating pose, against the GPU reference pose;
-    (b) from the learned proposal, against plain GPU ICP from the same
-        proposal (numerical parity of the ICP on the learned path's inits);
-    (c) continued from the learned path's final pose (Anderson-accelerated
-        refine), against that pose: how far FP64 ICP still moves it."""
-    import oracle
-    from realtime_robot_amd import synth
-    p = oracle.params(max_iter=MAX_ITER, r_max=R_MAX, eps_T=EPS)
-    cores = min(16, os.cpu_count() or 1)
-
-    def one(i):
-        a = oracle.icp(srcs[i], tgts[i], p, guess=gts[i])["T"]
-        b = oracle.icp(srcs[i], tgts[i], p, guess=inits[i])["T"]
-        c = oracle.icp(srcs[i], tgts[i], p, guess=T_learned[i])["T"]
-        return (synth.rmse_between(T_ref[i], a, srcs[i]), synth.rmse_between(T_plain[i], b, srcs[i]),
-                synth.rmse_between(T_learned[i], c, srcs[i]))
-
-    with ThreadPoolExecutor(cores) as ex:
-        r = list(ex.map(one, range(n_chk)))
-    return {"checked_pairs": n_chk, "reference_vs_fp64_max_rmse_m": float(max(x[0] for x in r)),
-            "plain_icp_from_proposal_vs_fp64_max_rmse_m": float(max(x[1] for x in r)),
-            "learned_vs_fp64_continued_max_rmse_m": float(max(x[2] for x in r)),
-            "learned_vs_fp64_continued_median_rmse_m": float(np.median([x[2] for x in r]))}
+                      f"thread on {threads} threads (the host's share for one GPU; nproc {os.cpu_count()}), "
+                      f"{t_cpu:.1f} s; {int(sum(ok))} of {done} within 1 mm of both FP64 references; "
+                      f"latency: config-1 chair pair on 1 thread, median of 5 after 1 warm-up"}
 
 
 def tuning6(cfg):
@@ -201,8 +265,9 @@ def main():
     ap.add_argument("--tuning", type=str, default="",
                     help="fixed engine tuning 'cell_factor,cells_per_point,item_pts,blocks_per_cu,use_tile[,retire]'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--fp64-sample", type=int, default=8)
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--no-config3", dest="config3", action="store_false",
+                    help="leave out the config-3 line (batched plain single-start ICP)")
     ap.add_argument("--account-sample", type=int, default=24)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--leg", choices=["both", "refine", "learned", "plain"], default="both",
@@ -227,15 +292,13 @@ def main():
         f"{sum(len(s) for s in srcs)} source / {sum(len(t) for t in tgts)} target points")
     params = learned.default_params()
 
-    T_ref = None
+    threads = host_threads()
+    T64_gt = None
     if args.leg in ("both", "learned"):
-        # reference poses: GPU ICP from the generating pose
-        ref = R.BatchRegistration()
-        ref.set_clouds(srcs, tgts)
-        ref.set_problems(range(n), range(n))
-        ref.set_params(params)
-        T_ref = [r["T"] for r in ref.align(gts)]
-        del ref
+        # the correct registrations: the FP64 oracle's ICP from the generating pose
+        t0 = time.perf_counter()
+        T64_gt = fp64_icp([(srcs[i], tgts[i], gts[i]) for i in range(n)], threads)
+        log(f"FP64 oracle from the generating poses: {time.perf_counter() - t0:.1f} s on {threads} threads")
 
     weights, meta = learned.load_weights()
     lr = learned.LearnedRegistration(weights, params, n_starts=args.starts, hypotheses=args.hypotheses,
@@ -299,14 +362,15 @@ def main():
             out = fn()
         return (time.perf_counter() - t_s) / steps, out
 
-    def n_ok(results):
-        return np.array([synth.rmse_between(results[i]["T"], T_ref[i], srcs[i]) <= OK_RMSE for i in range(n)])
+    def n_ok_gt(results):
+        return np.array([synth.rmse_between(results[i]["T"], T64_gt[i], srcs[i]) <= OK_RMSE for i in range(n)])
 
     if args.leg == "learned":
         # learned steps alone (kernel profiling of the whole step)
         t_l, res_l = timed(lr.align, args.steps, args.warmup)
         st = lr.stats
-        log(f"learned steps done: {t_l * 1e3:.1f} ms/step, {int(n_ok(res_l).sum())}/{n} within 1 mm, "
+        log(f"learned steps done: {t_l * 1e3:.1f} ms/step, {int(n_ok_gt(res_l).sum())}/{n} within 1 mm of the "
+            f"FP64 registration from the generating pose, "
             f"{st['n_fallback']} fallbacks; build {st['ms_build']:.2f} propose {st['ms_propose']:.2f} refine "
             f"{st['ms_refine']:.2f} judge {st['ms_judge']:.2f} fallback {st['ms_fallback']:.2f} ms")
         return
@@ -335,13 +399,19 @@ def main():
 
     t_l, res_l = timed(lr.align, args.steps, args.warmup)
     st = dict(lr.stats)
-    ok_l = n_ok(res_l)
     t_p, res_p = timed(ms.align, args.steps, args.warmup)
-    ok_p = n_ok(res_p)
-    value = ok_l.sum() / t_l
-    plain_value = ok_p.sum() / t_p
-    log(f"learned: {t_l * 1e3:.1f} ms/step, {ok_l.sum()}/{n} within 1 mm, {st['n_fallback']} fallbacks "
-        f"-> {value:.1f} reg/s; plain: {t_p * 1e3:.1f} ms/step, {ok_p.sum()}/{n} -> {plain_value:.1f} reg/s")
+    inits_p = ms.inits.copy()
+    log(f"learned: {t_l * 1e3:.1f} ms/step, {st['n_fallback']} fallbacks; plain: {t_p * 1e3:.1f} ms/step")
+
+    # the learned step with the clouds uploaded from host memory inside the
+    # timed region (the resident-input step above is `value`)
+    def learned_with_upload():
+        lr.set_clouds(srcs, tgts, cs, ct)
+        return lr.align()
+
+    t_lu, res_lu = timed(learned_with_upload, max(2, args.steps // 4), 1)
+    same_lu = all(np.array_equal(res_lu[i]["T"], res_l[i]["T"]) for i in range(n))
+    log(f"learned with host upload: {t_lu * 1e3:.1f} ms/step (results identical: {same_lu})")
     prop_rot = learned.rotation_error_deg(st["proposals"][:, :3, :3], np.stack(gts)[:, :3, :3])
 
     # dominant kernel: k_icp of the learned path's refine launch, HIP events on the engine's stream
@@ -356,6 +426,10 @@ def main():
     # the plain multi-start launch's search kernel on the same accounting
     roof_plain = launch_roofline(ms.reg, ms.guesses, np.repeat(np.arange(n), args.starts), srcs, tgts,
                                  args.account_sample)
+    # plain GPU ICP from the learned proposals: the parity leg (same algorithm as the FP64 oracle)
+    lr.use_params("plain")
+    res_pp = lr.reg.align(st["proposals"])
+    lr.use_params("refine")
 
     def profiled_traffic(name, tuning):
         """HBM bytes per launch from a committed rocprofv3 summary of the same
@@ -374,6 +448,39 @@ def main():
     traffic, traffic_src = profiled_traffic("k_icp_learned_refine_256", tuned_refine)
     traffic_plain, traffic_plain_src = profiled_traffic("k_icp_plain_multistart_2048", tuned)
 
+    # config 3: batched plain single-start ICP, its own line
+    c3 = config3_leg(args, params, threads, timed) if args.config3 else None
+
+    # FP64 references for every registration, outside the timed region:
+    # (a) from each pair's proposal, (b) from a rejected pair's fallback start,
+    # (c) from the plain path's best start, (d) continued from the learned pose
+    t0 = time.perf_counter()
+    fb = [int(i) for i in st["fallback_pairs"]]
+    jobs = ([(srcs[i], tgts[i], st["proposals"][i]) for i in range(n)] +
+            [(srcs[i], tgts[i], st["inits"][i]) for i in fb] +
+            [(srcs[i], tgts[i], inits_p[i]) for i in range(n)] +
+            [(srcs[i], tgts[i], res_l[i]["T"]) for i in range(n)])
+    T64 = fp64_icp(jobs, threads)
+    T64_prop, T64_fb, T64_p, T64_cont = T64[:n], T64[n:n + len(fb)], T64[n + len(fb):2 * n + len(fb)], T64[-n:]
+    log(f"FP64 oracle references: {len(jobs)} registrations in {time.perf_counter() - t0:.1f} s")
+    T64_l = list(T64_prop)
+    for k, i in enumerate(fb):
+        T64_l[i] = T64_fb[k]
+    T_l = [r["T"] for r in res_l]
+    T_p = [r["T"] for r in res_p]
+    ok_l, e_l_init, e_l_gt = counted(T_l, T64_l, T64_gt, srcs)
+    ok_p, e_p_init, e_p_gt = counted(T_p, T64_p, T64_gt, srcs)
+    value = ok_l.sum() / t_l
+    plain_value = ok_p.sum() / t_p
+    e_par = np.array([synth.rmse_between(res_pp[i]["T"], T64_prop[i], srcs[i]) for i in range(n)])
+    e_par_r = np.array([np.linalg.norm(res_pp[i]["T"][:3, :3] - T64_prop[i][:3, :3]) for i in range(n)])
+    e_par_t = np.array([np.linalg.norm(res_pp[i]["T"][:3, 3] - T64_prop[i][:3, 3]) for i in range(n)])
+    it_par = int(sum(res_pp[i]["iterations"] for i in range(n)))
+    e_cont = np.array([synth.rmse_between(T_l[i], T64_cont[i], srcs[i]) for i in range(n)])
+    log(f"learned: {ok_l.sum()}/{n} counted -> {value:.1f} reg/s; plain: {ok_p.sum()}/{n} -> {plain_value:.1f} "
+        f"reg/s; parity of plain GPU ICP from the proposals vs FP64: max rmse {e_par.max():.2e} m, rotation "
+        f"{e_par_r.max():.2e}, translation {e_par_t.max():.2e} m")
+
     out = {
         "metric": METRIC,
         "value": float(value),
@@ -385,12 +492,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 geometry, fp64 solves, bf16 network (fp32 accumulation)",
+        "dtype": "fp32 geometry, fp64 moments and solves, bf16 network (fp32 accumulation)",
         "data": "synthetic",
         "config": {"workload": f"learned pose proposal + ICP refine vs plain {args.starts}-start ICP: {n} "
                                f"asymmetric box-composite pairs (Ns logU[2k,50k], Nt logU[10k,200k], noise 1 mm, "
                                f"rotation U[0,180deg], t U[-0.1,0.1] m), r_max {R_MAX} m, <={MAX_ITER} iterations, "
-                               f"eps {EPS}; counted when within {OK_RMSE * 1e3:.0f} mm RMSE of the reference pose",
+                               f"eps {EPS}; a registration counts when within {OK_RMSE * 1e3:.0f} mm RMSE of the "
+                               f"FP64 oracle's ICP from the same init and of the FP64 oracle's ICP from the "
+                               f"generating pose",
                    "pairs": n, "hypotheses": args.hypotheses, "starts": args.starts,
                    "refine_anderson_memory": args.refine_anderson,
                    "fallback_anderson_memory": args.fallback_anderson, "judge_points": args.judge_points,
@@ -399,18 +508,29 @@ def main():
                    "tuning": list(tuned) if tuned else None,
                    "tuning_refine": list(tuned_refine) if tuned_refine else None,
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],
+                   "learned_ok_vs_same_init": int(np.sum(e_l_init <= OK_RMSE)),
+                   "learned_ok_vs_generating_pose": int(np.sum(e_l_gt <= OK_RMSE)),
                    "learned_ms_per_step": t_l * 1e3, "learned_ms_build": st["ms_build"],
                    "learned_ms_propose": st["ms_propose"], "learned_ms_propose_device": st["ms_propose_device"],
                    "learned_ms_refine": st["ms_refine"], "learned_ms_judge": st["ms_judge"],
                    "learned_ms_fallback": st["ms_fallback"],
+                   "value_with_upload": float(ok_l.sum() / t_lu), "ms_per_step_with_upload": t_lu * 1e3,
+                   "upload_bytes": int(sum(12 * (len(srcs[i]) + len(tgts[i])) for i in range(n))),
+                   "upload_results_identical": bool(same_lu),
                    "proposal_rot_err_deg_median": float(np.median(prop_rot)),
                    "proposal_rot_err_deg_p90": float(np.percentile(prop_rot, 90)),
                    "plain_value": float(plain_value), "plain_ok": int(ok_p.sum()), "plain_ms_per_step": t_p * 1e3,
+                   "plain_ok_vs_same_init": int(np.sum(e_p_init <= OK_RMSE)),
+                   "plain_ok_vs_generating_pose": int(np.sum(e_p_gt <= OK_RMSE)),
                    "plain_iterations_total": ms.iterations_total,
                    "speedup_vs_plain": float(value / plain_value) if plain_value > 0 else None,
+                   "host_threads": threads, "cpu_model": cpu_model(),
                    "weights": {k: meta[k] for k in ("base_pairs", "pairs_seen", "steps")}},
         "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": roof["achieved"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "achieved_basis": "algorithmic bytes (B_iter) / HIP-event launch time",
+                     "frac_measured_traffic": (traffic / (roof["kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
+                                               if traffic else None),
                      "kernel": "k_icp (persistent correspondence search + moments + FP64 solve), learned-path "
                                f"refine launch ({n} problems)",
                      **{k: v for k, v in roof.items() if k != "achieved"},
@@ -418,24 +538,91 @@ def main():
         "roofline_plain": {"bound": "hbm", "achieved": roof_plain["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": roof_plain["achieved"] / HBM_PEAK_GBS, "traffic": traffic_plain,
                            "traffic_source": traffic_plain_src,
+                           "achieved_basis": "algorithmic bytes (B_iter) / HIP-event launch time",
+                           "frac_measured_traffic": (traffic_plain / (roof_plain["kernel_ms"] / 1e3) / 1e9 /
+                                                     HBM_PEAK_GBS if traffic_plain else None),
                            "kernel": f"k_icp, plain multi-start launch ({n * args.starts} problems)",
                            **{k: v for k, v in roof_plain.items() if k != "achieved"}},
+        "accuracy": {
+            "reference": "FP64 CPU oracle ICP (oracle/oracle.c orc_icp) of every pair",
+            "parity_plain_gpu_icp_from_proposals_vs_fp64": {
+                "rmse_m": stats_m(e_par), "rotation_frobenius_max": float(e_par_r.max()),
+                "translation_m_max": float(e_par_t.max()), "iterations_total": it_par},
+            "learned_vs_fp64_same_init_rmse_m": stats_m(e_l_init),
+            "learned_vs_fp64_generating_pose_rmse_m": stats_m(e_l_gt),
+            "learned_vs_fp64_continued_rmse_m": stats_m(e_cont),
+            "plain_vs_fp64_same_init_rmse_m": stats_m(e_p_init),
+            "plain_vs_fp64_generating_pose_rmse_m": stats_m(e_p_gt),
+            "per_registration": {"learned_vs_fp64_same_init_m": sig(e_l_init),
+                                 "learned_vs_fp64_generating_pose_m": sig(e_l_gt),
+                                 "plain_gpu_icp_from_proposal_vs_fp64_m": sig(e_par),
+                                 "plain_vs_fp64_same_init_m": sig(e_p_init)}},
     }
+    if c3 is not None:
+        out["config3_plain_icp"] = c3
     if not args.no_cpu_baseline:
         starts = [multistart.so3_start_poses(cs[i], ct[i], args.starts) for i in range(n)]
-        out["cpu_baseline"] = cpu_baseline(srcs, tgts, starts, args.starts, args.cpu_budget)
-        rejected = set(st["fallback_pairs"].tolist())
-        # the init the learned path's final ICP ran from (fallback pairs are left out)
-        inits = [None if i in rejected else st["proposals"][i] for i in range(n)]
-        chk = [i for i in range(n) if inits[i] is not None][:args.fp64_sample]
-        # plain GPU ICP from the same proposals (the parity leg of the check)
-        lr.use_params("plain")
-        res_plain = lr.reg.align(st["proposals"])
-        acc = fp64_check([srcs[i] for i in chk], [tgts[i] for i in chk], [gts[i] for i in chk],
-                         [T_ref[i] for i in chk], [inits[i] for i in chk], [res_plain[i]["T"] for i in chk],
-                         [res_l[i]["T"] for i in chk], len(chk))
-        out["accuracy"] = acc
+        out["cpu_baseline"] = cpu_baseline(srcs, tgts, starts, args.starts, args.cpu_budget, T64_gt, threads)
+        # the GPU's single-pair latency on the same config-1 pair (1 warm-up + median of 5)
+        s1, t1, _ = synth.config1_pair(0)
+        icp1 = R.IterativeClosestPoint()
+        icp1.setInputSource(s1)
+        icp1.setInputTarget(t1)
+        icp1.setMaxCorrespondenceDistance(R_MAX)
+        icp1.setMaximumIterations(MAX_ITER)
+        icp1.setTransformationEpsilon(EPS)
+        lat = []
+        for k in range(6):
+            t0 = time.perf_counter()
+            icp1.align()
+            if k:
+                lat.append(time.perf_counter() - t0)
+        out["cpu_baseline"]["gpu_latency_ms_config1"] = float(np.median(lat)) * 1e3
     print(json.dumps(out), flush=True)
+
+
+def config3_leg(args, params, threads, timed):
+    """Batched plain ICP (256 independent pairs of random box composites, Ns
+    logU[2k,50k], Nt logU[10k,200k], 2 mm noise, source rotated U[0,30deg]
+    about a uniform axis and moved U[-0.1,0.1]^3 m): one persistent launch
+    replayed from a hipGraph, from the identity, single start, the grid build
+    included in every step, autotuned (cell size, cell budget, work items,
+    workgroups per CU, LDS tiles).  Registrations count as for `value`."""
+    import realtime_robot_amd as R
+    from realtime_robot_amd import autotune, synth
+    n = args.pairs
+    srcs, tgts, gts = synth.config3_batch(n)
+    reg = R.BatchRegistration()
+    reg.set_params(params)
+    reg.set_clouds(srcs, tgts)
+    reg.set_problems(range(n), range(n))
+    guesses = np.repeat(np.eye(4)[None], n, 0)
+    tuned, tuned_ms, _ = autotune.tune(reg, guesses, candidates=autotune.CANDIDATES, key=None, use_cache=False)
+    log(f"config 3: autotune {tuned} ({tuned_ms:.1f} ms)")
+    t3, res = timed(lambda: reg.align(guesses), args.steps, args.warmup)
+    roof = launch_roofline(reg, guesses, np.arange(n), srcs, tgts, args.account_sample)
+    T = [r["T"] for r in res]
+    T64 = fp64_icp([(srcs[i], tgts[i], np.eye(4)) for i in range(n)] +
+                   [(srcs[i], tgts[i], gts[i]) for i in range(n)], threads)
+    ok, e_init, e_gt = counted(T, T64[:n], T64[n:], srcs)
+    e_r = np.array([np.linalg.norm(T[i][:3, :3] - T64[i][:3, :3]) for i in range(n)])
+    e_t = np.array([np.linalg.norm(T[i][:3, 3] - T64[i][:3, 3]) for i in range(n)])
+    log(f"config 3: {t3 * 1e3:.1f} ms/step, {ok.sum()}/{n} counted -> {ok.sum() / t3:.1f} reg/s; vs FP64 from the "
+        f"same init: max rmse {e_init.max():.2e} m")
+    return {"workload": "config 3: 256 pairs of random box composites (Ns logU[2k,50k], Nt logU[10k,200k], noise "
+                        "2 mm, rotation U[0,30deg], t U[-0.1,0.1] m), plain single-start ICP from the identity, "
+                        f"r_max {R_MAX} m, <={MAX_ITER} iterations, eps {EPS}, one persistent launch + hipGraph, "
+                        "grid build included",
+            "value": float(ok.sum() / t3), "unit": "registrations/s", "ms_per_step": t3 * 1e3, "ok": int(ok.sum()),
+            "pairs": n, "tuning": list(tuned),
+            "iterations_total": int(sum(r["iterations"] for r in res)),
+            "parity_vs_fp64_same_init": {"rmse_m": stats_m(e_init), "rotation_frobenius_max": float(e_r.max()),
+                                         "translation_m_max": float(e_t.max())},
+            "vs_fp64_generating_pose_rmse_m": stats_m(e_gt),
+            "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": roof["achieved"] / HBM_PEAK_GBS,
+                         "achieved_basis": "algorithmic bytes (B_iter) / HIP-event launch time",
+                         **{k: v for k, v in roof.items() if k != "achieved"}}}
 
 
 if __name__ == "__main__":

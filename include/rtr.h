@@ -392,6 +392,93 @@ int rtr_gemm_bf16(int device, const float* A, int ta, const float* B, int tb, fl
                   int K, int k_split);
 
 /* ------------------------------------------------------------------ */
+/* Learned-init registration of many pairs per call: the network        */
+/* proposes an initial pose per pair, one batched ICP launch refines    */
+/* every proposal, the registration judge scores the refined poses, and */
+/* the pairs it rejects run plain multi-start ICP, whose best start     */
+/* replaces them.  Replaces the reference's coarse stage followed by    */
+/* ICP: keypoints, TDF yaw search and consensus (RealTimeRobot.cpp:      */
+/* 45-105, matching.h:122-222, function.h:35-109), then                 */
+/* pcl::IterativeClosestPoint (function.h:111-123).                     */
+/* ------------------------------------------------------------------ */
+typedef struct rtr_learned rtr_learned;
+
+typedef struct {
+  int hypotheses;              /* network orientations per pair (1..64), default 8 */
+  int n_starts;                /* fallback starts per rejected pair (1..64), default 8 */
+  float tau;                   /* truncation of the hypotheses' chamfer score, default 0.1 m */
+  double rmse_threshold;       /* judge: plane RMSE above this rejects, default 2e-3 m */
+  double min_inlier_frac;      /* judge: inlier fraction below this rejects, default 0.8 */
+  int judge_k;                 /* judge: neighbours of the PCA planes, default 20 */
+  int judge_points;            /* judge: sampled source points per pair, default 1024 */
+  double max_curvature;        /* judge: planar when surface variation <= this, default 0.02 */
+  int refine_anderson;         /* refine: Anderson memory (0..8), default 5 */
+  int refine_coarse_stride;    /* refine: coarse-to-fine stride, default 4 */
+  int refine_coarse_iterations;/* refine: most coarse iterations, default 25 */
+  int fallback_anderson;       /* fallback: Anderson memory, default 0 (plain ICP) */
+  int fallback_item_pts;       /* fallback: source points per work item, default 256 */
+  int fallback_blocks_per_cu;  /* fallback: persistent workgroups per CU, default 1 */
+} rtr_learned_config;
+
+/* Host wall-clock phases of the last call (ms) and its fallback count. */
+typedef struct {
+  int n_pairs;
+  int n_fallback;
+  float ms_propose;        /* the network (the refine grids are built beside it) */
+  float ms_build_wait;     /* wait for the grid build after the network */
+  float ms_refine;         /* batched ICP refine of every proposal */
+  float ms_judge;          /* registration judge */
+  float ms_fallback;       /* multi-start ICP of the rejected pairs */
+  float ms_total;
+  float ms_propose_device; /* device time of the network call */
+} rtr_learned_stats;
+
+int rtr_learned_config_default(rtr_learned_config* c);
+/* icp: the registration parameters (nullable: rtr_params_default); the
+ * refine adds cfg's acceleration and coarse-to-fine stage, the fallback runs
+ * the plain iteration.  cfg nullable: rtr_learned_config_default. */
+int rtr_learned_create(rtr_learned** out, int device, int max_pairs, const rtr_params* icp,
+                       const rtr_learned_config* cfg);
+int rtr_learned_destroy(rtr_learned* h);
+/* Network weights (rtr_prop_num_params FP32 values, the layout of
+ * rtr_prop_set_weights), from memory or from a raw little-endian float32 file. */
+int rtr_learned_set_weights(rtr_learned* h, const float* w);
+int rtr_learned_load_weights(rtr_learned* h, const char* path);
+/* The hypotheses' orientations and the fallback's start rotations (row-major
+ * 3x3 each; nullable: the library's default set of that size, the identity
+ * then greedy farthest-point picks over SO(3)). */
+int rtr_learned_set_rotations(rtr_learned* h, const double* hyp, int n_hyp, const double* starts, int n_starts);
+int rtr_learned_get_rotations(rtr_learned* h, double* hyp, double* starts);
+/* n <= max_pairs (source, target) pairs, packed xyz.  The handle uploads them
+ * and also keeps the pointers (the fallback reads the rejected pairs' clouds
+ * from them, as PCL's registration shares its input clouds): they must stay
+ * valid until the next rtr_learned_set_clouds or rtr_learned_destroy.
+ * Centroids (n*3 doubles, nullable: FP64 means) centre the fallback starts. */
+int rtr_learned_set_clouds(rtr_learned* h, int n, const float* const* src, const size_t* ns,
+                           const float* const* tgt, const size_t* nt, const double* src_centroids,
+                           const double* tgt_centroids);
+/* Register every pair: results (n), inits (n*16 column-major, nullable): the
+ * pose each final ICP ran from (the proposal, or a rejected pair's best
+ * start), rejected (n, nullable): 1 where the judge sent the pair to the
+ * fallback. */
+int rtr_learned_align(rtr_learned* h, rtr_result* results, double* inits, int* rejected);
+/* The same from the caller's proposals (n*16 column-major) instead of the
+ * network's. */
+int rtr_learned_align_from(rtr_learned* h, const double* proposals, rtr_result* results, double* inits,
+                           int* rejected);
+int rtr_learned_get_stats(rtr_learned* h, rtr_learned_stats* s);
+/* The last call's proposals (n*16) and judge outputs (n each, nullable). */
+int rtr_learned_get_proposals(rtr_learned* h, double* poses);
+int rtr_learned_get_judge(rtr_learned* h, double* plane_rmse, double* inlier_frac, double* planar_frac);
+/* plain = 1: the refine engine runs the plain iteration (the caller's icp
+ * parameters) until plain = 0 restores the refine's; for parity checks and
+ * tuning through the engine handle. */
+int rtr_learned_use_plain_params(rtr_learned* h, int plain);
+/* The handle's refine and fallback engines and its network (borrowed: owned
+ * and destroyed by the rtr_learned handle), for tuning and profiling. */
+int rtr_learned_batches(rtr_learned* h, rtr_batch** refine, rtr_batch** fallback, rtr_prop** proposer);
+
+/* ------------------------------------------------------------------ */
 /* Self-check hooks: run the library's own geometry code on the host.   */
 /* ------------------------------------------------------------------ */
 /* The grid NN search on the CPU over the same cell-sorted layout. */

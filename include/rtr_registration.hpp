@@ -227,6 +227,102 @@ class IterativeClosestPoint {
   bool converged_ = false;
 };
 
+// Learned-init registration of many (model, scene) pairs per call: the
+// reference's coarse alignment (keypoints, TDF yaw search, consensus:
+// RealTimeRobot.cpp:45-105) followed by its ICP (function.h:111-123), as one
+// batched pipeline (rtr_learned_*): network proposal, batched ICP refine,
+// registration judge, multi-start fallback for the rejected pairs.
+//
+//   rtr::LearnedRegistration<rtr::PointXYZ> reg(256);
+//   reg.loadWeights("proposer.f32");
+//   reg.setInputClouds(models, scans);           // vectors of PointCloud::ConstPtr
+//   reg.align();
+//   rtr::Matrix4d T = reg.getFinalTransformation(i);
+//   bool fell_back = reg.getRejected()[i];
+template <class PointT>
+class LearnedRegistration {
+ public:
+  using CloudConstPtr = std::shared_ptr<const PointCloud<PointT>>;
+
+  explicit LearnedRegistration(int max_pairs, int device = 0, const rtr_params* icp = nullptr,
+                               const rtr_learned_config* cfg = nullptr) {
+    rtr_params p;
+    if (!icp) {
+      // the batched registration workloads' ICP settings
+      check(rtr_params_default(&p));
+      p.max_iterations = 50;
+      p.max_correspondence_distance = 0.05;
+      p.transformation_epsilon = 1e-8;
+      icp = &p;
+    }
+    check(rtr_learned_create(&h_, device, max_pairs, icp, cfg));
+  }
+  ~LearnedRegistration() {
+    if (h_) rtr_learned_destroy(h_);
+  }
+  LearnedRegistration(const LearnedRegistration&) = delete;
+  LearnedRegistration& operator=(const LearnedRegistration&) = delete;
+
+  void loadWeights(const std::string& path) { check(rtr_learned_load_weights(h_, path.c_str())); }
+  void setWeights(const std::vector<float>& w) { check(rtr_learned_set_weights(h_, w.data())); }
+
+  // packed copies of the clouds are kept by this object (the library reads
+  // the rejected pairs' clouds from them)
+  void setInputClouds(const std::vector<CloudConstPtr>& sources, const std::vector<CloudConstPtr>& targets) {
+    if (sources.size() != targets.size()) throw Error(RTR_ERR_INVALID_ARG, "sources and targets must pair up");
+    const size_t n = sources.size();
+    src_.assign(n, {});
+    tgt_.assign(n, {});
+    std::vector<const float*> sp(n), tp(n);
+    std::vector<size_t> ns(n), nt(n);
+    for (size_t i = 0; i < n; ++i) {
+      pack(*sources[i], src_[i]);
+      pack(*targets[i], tgt_[i]);
+      sp[i] = src_[i].data();
+      tp[i] = tgt_[i].data();
+      ns[i] = sources[i]->size();
+      nt[i] = targets[i]->size();
+    }
+    check(rtr_learned_set_clouds(h_, (int)n, sp.data(), ns.data(), tp.data(), nt.data(), nullptr, nullptr));
+    res_.assign(n, rtr_result());
+    rej_.assign(n, 0);
+  }
+  void align() { check(rtr_learned_align(h_, res_.data(), nullptr, rej_.data())); }
+  // from the caller's initial poses instead of the network's
+  void align(const std::vector<Matrix4d>& proposals) {
+    std::vector<double> g(16 * proposals.size());
+    for (size_t i = 0; i < proposals.size(); ++i) std::memcpy(&g[16 * i], proposals[i].m, 16 * sizeof(double));
+    check(rtr_learned_align_from(h_, g.data(), res_.data(), nullptr, rej_.data()));
+  }
+  Matrix4d getFinalTransformation(size_t i) const {
+    Matrix4d T;
+    std::memcpy(T.m, res_.at(i).T, sizeof(T.m));
+    return T;
+  }
+  bool hasConverged(size_t i) const { return res_.at(i).converged != 0; }
+  const std::vector<rtr_result>& getResults() const { return res_; }
+  const std::vector<int>& getRejected() const { return rej_; }
+  rtr_learned_stats getStats() {
+    rtr_learned_stats s;
+    check(rtr_learned_get_stats(h_, &s));
+    return s;
+  }
+
+ private:
+  static void pack(const PointCloud<PointT>& c, std::vector<float>& out) {
+    out.resize(3 * c.size());
+    for (size_t i = 0; i < c.size(); ++i) {
+      out[3 * i] = c.points[i].x;
+      out[3 * i + 1] = c.points[i].y;
+      out[3 * i + 2] = c.points[i].z;
+    }
+  }
+  rtr_learned* h_ = nullptr;
+  std::vector<std::vector<float>> src_, tgt_;
+  std::vector<rtr_result> res_;
+  std::vector<int> rej_;
+};
+
 // Voxel-hash map: the merge of transformed clouds into the scene map
 // (the overlay at RealTimeRobot.cpp:104-114, as a fused map).
 class VoxelMap {

@@ -128,12 +128,14 @@ class KdTree:
             _kd_free(self._h)
             self._h = None
 
-    def nn(self, q, r_max=None, tie_lowest=True):
+    def nn(self, q, r_max=None, tie_lowest=True, bounded=False):
+        """Exact NN (ties to the lowest index, or FLANN's first found);
+        bounded: the search starts from the radius (same result)."""
         q = _pts(q)
         r2f = np.finfo(np.float32).max if r_max is None else r2_to_float(r_max * r_max)
         idx = np.empty(len(q), np.int32)
         d2 = np.empty(len(q), np.float32)
-        _kd_nn(self._h, q, len(q), r2f, int(tie_lowest), idx, d2)
+        _kd_nn(self._h, q, len(q), r2f, int(tie_lowest) | (2 if bounded else 0), idx, d2)
         return idx, d2
 
     def knn(self, q, k):

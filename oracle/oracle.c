@@ -257,7 +257,21 @@ typedef struct {
   int tie_lowest;
 } nnres;
 
-static void kd_nn_rec(const orc_kdtree* kd, int32_t id, const float* q, nnres* r) {
+/* Lower bound of d2f(q, p) for every point p of a cell, from q's per-axis
+ * offsets to the split planes that bound the cell on q's side (0 on axes
+ * with none), summed in d2f's order.  Each |off| is at most the matching
+ * |q - p| after rounding (rounding of q - x is monotone in x), so each float
+ * square is at most d2f's, and float addition of non-negative terms is
+ * monotone: the bound never exceeds a point's d2f, and the search stays
+ * exact (Arya & Mount's incremental distance, evaluated in d2f's order). */
+static inline float off_bound(const float off[3]) {
+  float s = off[0] * off[0];
+  s = s + off[1] * off[1];
+  s = s + off[2] * off[2];
+  return s;
+}
+
+static void kd_nn_rec(const orc_kdtree* kd, int32_t id, const float* q, float off[3], nnres* r) {
   const kdnode* nd = &kd->nodes[id];
   if (nd->left < 0) {
     for (int32_t i = nd->begin; i < nd->end; ++i) {
@@ -273,17 +287,35 @@ static void kd_nn_rec(const orc_kdtree* kd, int32_t id, const float* q, nnres* r
   float diff = q[nd->axis] - nd->split;
   int32_t near = diff <= 0.0f ? nd->left : nd->right;
   int32_t far = diff <= 0.0f ? nd->right : nd->left;
-  kd_nn_rec(kd, near, q, r);
-  float pd = diff * diff;
-  if (r->tie_lowest ? (pd <= r->best) : (pd < r->best)) kd_nn_rec(kd, far, q, r);
+  kd_nn_rec(kd, near, q, off, r);
+  const float saved = off[nd->axis];
+  off[nd->axis] = diff;
+  const float lb = off_bound(off);
+  if (r->tie_lowest ? (lb <= r->best) : (lb < r->best)) kd_nn_rec(kd, far, q, off, r);
+  off[nd->axis] = saved;
 }
 
-void orc_kd_nn(const orc_kdtree* kd, const float* q, int64_t nq, float r2f, int tie_lowest,
+/* flags: ORC_KD_TIE_LOWEST (ties to the lowest index; else the first found in
+ * tree order, FLANN's result set) | ORC_KD_BOUNDED (the search starts from
+ * the radius r2f instead of +inf: the same answers, since a subtree it prunes
+ * only holds points beyond r2f, which the radius test rejects anyway, and the
+ * candidates within r2f are visited in the same order; PCL's own search is
+ * unbounded, so its restatement leaves this off). */
+void orc_kd_nn(const orc_kdtree* kd, const float* q, int64_t nq, float r2f, int flags,
                int32_t* idx, float* d2) {
+  const int tie_lowest = (flags & ORC_KD_TIE_LOWEST) != 0;
+  const int bounded = (flags & ORC_KD_BOUNDED) != 0;
   for (int64_t i = 0; i < nq; ++i) {
+    /* bounded: best starts at r2f (tie rule: a point at exactly r2f still
+     * wins over the sentinel index) or just above it (strict rule) */
     nnres r = {INFINITY, -1, tie_lowest};
-    if (kd->n > 0 && finite3(q + 3 * i)) kd_nn_rec(kd, 0, q + 3 * i, &r);
-    if (r.bi >= 0 && r.best <= r2f) {
+    if (bounded) {
+      r.best = tie_lowest ? r2f : nextafterf(r2f, INFINITY);
+      r.bi = INT32_MAX;
+    }
+    float off[3] = {0.0f, 0.0f, 0.0f};
+    if (kd->n > 0 && finite3(q + 3 * i)) kd_nn_rec(kd, 0, q + 3 * i, off, &r);
+    if (r.bi >= 0 && r.bi != INT32_MAX && r.best <= r2f) {
       idx[i] = r.bi;
       d2[i] = r.best;
     } else {
@@ -649,7 +681,7 @@ int orc_icp(const float* src, int64_t ns, const float* tgt, const float* tgt_nor
   double mse = 0;
   for (;;) {
     orc_transform_f32(T, src, ns, q);
-    orc_kd_nn(kd, q, ns, r2f, 1, idx, d2);
+    orc_kd_nn(kd, q, ns, r2f, ORC_KD_TIE_LOWEST | ORC_KD_BOUNDED, idx, d2);
     ncorr = 0;
     double sd2 = 0;
     for (int64_t i = 0; i < ns; ++i)
@@ -745,7 +777,7 @@ double orc_fitness(const float* src, int64_t ns, const float* tgt, int64_t nt, c
   float* d2 = (float*)malloc((size_t)(ns > 0 ? ns : 1) * 4);
   orc_transform_f32(T, src, ns, q);
   orc_kdtree* kd = orc_kd_build(tgt, nt, 15);
-  orc_kd_nn(kd, q, ns, FLT_MAX, 1, idx, d2);
+  orc_kd_nn(kd, q, ns, FLT_MAX, ORC_KD_TIE_LOWEST, idx, d2);
   double s = 0;
   int64_t m = 0;
   for (int64_t i = 0; i < ns; ++i)
@@ -1013,7 +1045,7 @@ int orc_pcl_icp(const float* src, int64_t ns, const float* tgt, int64_t nt, cons
   int64_t ncorr = 0;
   double mse = 0;
   for (;;) {
-    orc_kd_nn(kd, w, ns, r2f, 0, idx, d2);
+    orc_kd_nn(kd, w, ns, r2f, 0, idx, d2); /* FLANN: unbounded, first found in tree order */
     ncorr = 0;
     double sd2 = 0;
     for (int64_t i = 0; i < ns; ++i)

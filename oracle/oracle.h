@@ -55,9 +55,12 @@ void orc_nn_brute(const float* q, int64_t nq, const float* t, int64_t nt, float 
 typedef struct orc_kdtree orc_kdtree;
 orc_kdtree* orc_kd_build(const float* t, int64_t nt, int leaf_size);
 void orc_kd_free(orc_kdtree* kd);
-/* exact NN.  tie_lowest=1: same result as orc_nn_brute.  tie_lowest=0: first
- * found in tree order (FLANN result-set semantics). */
-void orc_kd_nn(const orc_kdtree* kd, const float* q, int64_t nq, float r2f, int tie_lowest,
+/* exact NN.  flags: ORC_KD_TIE_LOWEST: same result as orc_nn_brute; without
+ * it: first found in tree order (FLANN result-set semantics).
+ * ORC_KD_BOUNDED: the search is bounded by r2f from the start (same result,
+ * fewer nodes visited for queries with nothing in range). */
+enum { ORC_KD_TIE_LOWEST = 1, ORC_KD_BOUNDED = 2 };
+void orc_kd_nn(const orc_kdtree* kd, const float* q, int64_t nq, float r2f, int flags,
                int32_t* idx, float* d2);
 /* exact k nearest, ordered by (d2, index) ascending; missing slots get -1/inf. */
 void orc_kd_knn(const orc_kdtree* kd, const float* q, int64_t nq, int k, int32_t* idx, float* d2);

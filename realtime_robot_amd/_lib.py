@@ -47,6 +47,21 @@ class Timing(C.Structure):
                 ("queries_searched", C.c_int64)]
 
 
+class LearnedConfig(C.Structure):
+    _fields_ = [("hypotheses", C.c_int), ("n_starts", C.c_int), ("tau", C.c_float),
+                ("rmse_threshold", C.c_double), ("min_inlier_frac", C.c_double), ("judge_k", C.c_int),
+                ("judge_points", C.c_int), ("max_curvature", C.c_double), ("refine_anderson", C.c_int),
+                ("refine_coarse_stride", C.c_int), ("refine_coarse_iterations", C.c_int),
+                ("fallback_anderson", C.c_int), ("fallback_item_pts", C.c_int),
+                ("fallback_blocks_per_cu", C.c_int)]
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [("n_pairs", C.c_int), ("n_fallback", C.c_int), ("ms_propose", C.c_float),
+                ("ms_build_wait", C.c_float), ("ms_refine", C.c_float), ("ms_judge", C.c_float),
+                ("ms_fallback", C.c_float), ("ms_total", C.c_float), ("ms_propose_device", C.c_float)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C realtime_robot_amd/csrc)")
@@ -142,6 +157,21 @@ _SIGS = {
     "rtr_prop_forward_backward": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rtr_prop_train_step": (_i, [_vp, _i, _vp, _vp, _f, C.POINTER(_f)]),
     "rtr_gemm_bf16": (_i, [_i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _i]),
+    "rtr_learned_config_default": (_i, [C.POINTER(LearnedConfig)]),
+    "rtr_learned_create": (_i, [C.POINTER(_vp), _i, _i, C.POINTER(Params), C.POINTER(LearnedConfig)]),
+    "rtr_learned_destroy": (_i, [_vp]),
+    "rtr_learned_set_weights": (_i, [_vp, _vp]),
+    "rtr_learned_load_weights": (_i, [_vp, C.c_char_p]),
+    "rtr_learned_set_rotations": (_i, [_vp, _vp, _i, _vp, _i]),
+    "rtr_learned_get_rotations": (_i, [_vp, _vp, _vp]),
+    "rtr_learned_set_clouds": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rtr_learned_align": (_i, [_vp, _vp, _vp, _vp]),
+    "rtr_learned_align_from": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "rtr_learned_get_stats": (_i, [_vp, C.POINTER(LearnedStats)]),
+    "rtr_learned_get_proposals": (_i, [_vp, _vp]),
+    "rtr_learned_get_judge": (_i, [_vp, _vp, _vp, _vp]),
+    "rtr_learned_use_plain_params": (_i, [_vp, _i]),
+    "rtr_learned_batches": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "rtr_selfcheck_nn": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp]),
     "rtr_selfcheck_nn_stats": (_i, [_vp, _sz, _vp, _sz, _f, _d, _vp, _vp, _vp, _vp]),
     "rtr_selfcheck_solve": (_i, [_i, _vp, _vp, _vp, _sz, _vp, _vp]),

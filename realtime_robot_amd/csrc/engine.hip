@@ -11,19 +11,6 @@
 
 namespace rtr {
 
-static thread_local std::string g_err;
-
-void set_error(const char* fmt, ...) {
-  char buf[1024];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_err = buf;
-}
-
-const char* last_error_str() { return g_err.c_str(); }
-
 int DevBuf::ensure(size_t n) {
   if (n == 0) n = 256;
   if (p && n <= bytes) return RTR_OK;
@@ -346,7 +333,7 @@ int Engine::layout_problems() {
   RTR_TRY(guesses_.ensure(np * 16 * sizeof(double)));
   RTR_TRY(done_.ensure(np * 4));
   RTR_TRY(items_buf_.ensure(items_.size() * sizeof(uint2)));
-  RTR_TRY(partials_.ensure(items_.size() * NSUM_MAX * 2 * sizeof(float)));
+  RTR_TRY(partials_.ensure(items_.size() * NSUM_MAX * sizeof(double)));
   RTR_TRY(counters_.ensure(counter_words() * 4));
   RTR_TRY(out_T_.ensure(16 * sizeof(double)));
   RTR_TRY(hist_.ensure(np * (size_t)(prm_.max_iterations + 1) * 12 * sizeof(float)));
@@ -417,7 +404,7 @@ DevEngine Engine::dev_view() const {
   e.deep = deep_gathers() ? 1 : 0;
   e.retire_from = tune_.retire > 0.f ? cu_count_ : 0x7fffffff;
   e.retire_active = (uint32_t)((double)tune_.retire * (double)probs_.size());
-  e.partials = partials_.as<float>();
+  e.partials = partials_.as<double>();
   uint32_t* ctr = counters_.as<uint32_t>();
   // the done count, the error flag and every claim counter on cache lines of
   // their own: spinning workgroups poll the flag while others claim

@@ -9,6 +9,7 @@
 
 #include "../../include/rtr.h"
 #include "engine_dev.hpp"
+#include "guard.hpp"
 
 namespace rtr {
 
@@ -23,12 +24,6 @@ void set_error(const char* fmt, ...);
     }                                                                                      \
   } while (0)
 
-// return the error code of a failing RTR_* call
-#define RTR_TRY(x)                   \
-  do {                               \
-    int _rc = (x);                   \
-    if (_rc != RTR_OK) return _rc;   \
-  } while (0)
 
 // Grow-only device allocation (allocations happen in set_* calls, never in a
 // captured or timed align).

@@ -57,7 +57,7 @@ struct DevEngine {
   int deep;                  // 8-deep gathers, 2 waves per SIMD (point-to-point; always for point-to-plane)
   int retire_from;           // workgroups from this index on leave at the start of an iteration ...
   uint32_t retire_active;    // ... once at most this many problems are unfinished
-  float* partials;           // n_items * NSUM_MAX * 2 floats
+  double* partials;          // n_items * NSUM_MAX FP64 moment sums
   uint32_t* claim;           // claim counter per (pass, XCD queue), claim_stride words apart
   int claim_stride;          // 32 (one 128-byte line per counter) unless the iteration limit is huge
   uint32_t xcd_off[kXcds + 1];  // item range of each XCD queue

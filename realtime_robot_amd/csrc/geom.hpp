@@ -988,37 +988,48 @@ RTR_HD void anderson_reset(double* st) {
   st[10] = 1.7976931348623157e308;  // DBL_MAX
 }
 
-// Per-correspondence contributions, relative to the anchor (float).
-RTR_HD int point_terms(int mode, float ax, float ay, float az, float bx, float by, float bz, float nx,
-                       float ny, float nz, float d2, float* out) {
-  out[SUM_N] = 1.0f;
-  out[SUM_D2] = d2;
-  if (mode == RTR_MODE_POINT_TO_POINT) {
-    out[SUM_A + 0] = ax;
-    out[SUM_A + 1] = ay;
-    out[SUM_A + 2] = az;
-    out[SUM_B + 0] = bx;
-    out[SUM_B + 1] = by;
-    out[SUM_B + 2] = bz;
-    const float a[3] = {ax, ay, az}, b[3] = {bx, by, bz};
+// One correspondence added to FP64 moment sums (layout in rtr_types.hpp).
+// q is the transformed source point, c its target point and nrm c's normal,
+// all floats, and so is the anchor (A): q - A and c - A are exact in FP64
+// (the difference of two floats needs at most 25 + the exponent gap bits),
+// every product rounds once, and the sums carry no FP32 rounding at all.
+// The moments are then those of the exact correspondences up to FP64
+// rounding, whatever order lanes, waves and work items add them in.
+template <int MODE>
+RTR_HD void accumulate_pair(float qx, float qy, float qz, float cx, float cy, float cz, float nx, float ny,
+                            float nz, float d2, double Ax, double Ay, double Az, double* acc) {
+  const double s0 = (double)qx - Ax, s1 = (double)qy - Ay, s2 = (double)qz - Az;
+  const double t0 = (double)cx - Ax, t1 = (double)cy - Ay, t2 = (double)cz - Az;
+  acc[SUM_N] += 1.0;
+  acc[SUM_D2] += (double)d2;
+  if (MODE == RTR_MODE_POINT_TO_POINT) {
+    acc[SUM_A + 0] += s0;
+    acc[SUM_A + 1] += s1;
+    acc[SUM_A + 2] += s2;
+    acc[SUM_B + 0] += t0;
+    acc[SUM_B + 1] += t1;
+    acc[SUM_B + 2] += t2;
+    const double a[3] = {s0, s1, s2}, b[3] = {t0, t1, t2};
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) out[SUM_AB + r * 3 + c] = a[r] * b[c];
-    return SUM_P2P_COUNT;
+      for (int c = 0; c < 3; ++c) acc[SUM_AB + r * 3 + c] = fma(a[r], b[c], acc[SUM_AB + r * 3 + c]);
+  } else {
+    // PCL's TransformationEstimationPointToPlaneLLS row (s x n, n) and
+    // right-hand side n . (c - q), about the anchor
+    const double n0 = nx, n1 = ny, n2 = nz;
+    const double row[6] = {n2 * s1 - n1 * s2, n0 * s2 - n2 * s0, n1 * s0 - n0 * s1, n0, n1, n2};
+    double rhs = n0 * (t0 - s0);
+    rhs = rhs + n1 * (t1 - s1);
+    rhs = rhs + n2 * (t2 - s2);
+    int k = 0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = r; c < 6; ++c, ++k) acc[SUM_ATA + k] = fma(row[r], row[c], acc[SUM_ATA + k]);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) acc[SUM_ATB + r] = fma(row[r], rhs, acc[SUM_ATB + r]);
   }
-  const float row[6] = {nz * ay - ny * az, nx * az - nz * ax, ny * ax - nx * ay, nx, ny, nz};
-  float rhs = nx * (bx - ax);
-  rhs = rhs + ny * (by - ay);
-  rhs = rhs + nz * (bz - az);
-  int k = 0;
-#pragma unroll
-  for (int r = 0; r < 6; ++r)
-#pragma unroll
-    for (int c = r; c < 6; ++c) out[SUM_ATA + k++] = row[r] * row[c];
-#pragma unroll
-  for (int r = 0; r < 6; ++r) out[SUM_ATB + r] = row[r] * rhs;
-  return SUM_P2L_COUNT;
 }
 
 }  // namespace rtr

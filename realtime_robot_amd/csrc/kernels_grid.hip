@@ -6,6 +6,7 @@
 
 #include "engine_dev.hpp"
 #include "geom.hpp"
+#include "wave.hpp"
 
 namespace rtr {
 
@@ -114,21 +115,7 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32
 // block reductions
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+// (wave reductions: DPP row operations, wave.hpp)
 
 
 // Bounding boxes of many clouds at once: each cloud is split over many
@@ -170,8 +157,8 @@ __global__ void k_cloud_bbox(DevEngine e) {
     ++cnt;
   }
   for (int a = 0; a < 3; ++a) {
-    l[a] = wave_min(l[a]);
-    h[a] = wave_max(h[a]);
+    l[a] = wave_min_f32(l[a]);
+    h[a] = wave_max_f32(h[a]);
   }
   cnt = wave_sum_u32(cnt);
   // the workgroup's waves combine in LDS first: one set of atomics per

@@ -6,8 +6,9 @@
 // (one pass = one ICP iteration of every unfinished problem); in each pass they
 // claim items from that pass's atomic counter.  An item of problem p in pass k
 // waits (bounded) until p has completed k iterations, computes its points'
-// correspondences and FP32 compensated moments, and the workgroup that
-// finishes p's last item of the pass reduces the item moments in FP64, solves
+// correspondences and FP64 moments (exact anchor-relative differences, one
+// rounding per product; wave totals from DPP row sums), and the workgroup that
+// finishes p's last item of the pass adds the item moments in FP64, solves
 // the increment, applies the convergence rules and publishes iteration k+1.
 // Problems therefore advance independently; no grid-wide barrier is needed,
 // and every wait is bounded (a timeout sets e.error and the waiters bail out).
@@ -29,6 +30,7 @@
 
 #include "engine_dev.hpp"
 #include "geom.hpp"
+#include "wave.hpp"
 
 namespace rtr {
 
@@ -40,23 +42,6 @@ __device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
 }
 __device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
-  const float y = x - c;
-  const float t = s + y;
-  c = (t - s) - y;
-  s = t;
-}
-
-// (s - c) + (s2 - c2): TwoSum of the leading parts, rounding error folded
-// into the compensation.
-__device__ __forceinline__ void comp_combine(float& s, float& c, float s2, float c2) {
-  const float S = s + s2;
-  const float bb = S - s;
-  const float err = (s - (S - bb)) + (s2 - bb);
-  c = (c + c2) - err;
-  s = S;
 }
 
 __device__ __forceinline__ GridView grid_of(const DevEngine& e, const TargetDesc& td) {
@@ -150,7 +135,7 @@ struct IcpShared {
   uint16_t qlist[kChunk]; // the chunk's queries to walk by class, each class in query order
   uint32_t qseed[kChunk]; // its search seed: the last iteration's nearest-neighbour slot (or ~0)
   int nlist;
-  float red[kWaves][NS][2];
+  double red[kWaves][NS];
   double dred[kBlock / NSUM_MAX][NSUM_MAX];
   double ws[kAndersonWs];
   double Tinc[16];
@@ -170,16 +155,6 @@ struct IcpShared {
   int fin;
 };
 
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
-}
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
   const int lane = threadIdx.x & 63;
   uint32_t inc = v;
@@ -339,8 +314,7 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
   double acc = 0.0;
   if (v < NS) {
     for (uint32_t i = grp; i < pr.n_items; i += kBlock / NSUM_MAX) {
-      const float* P = e.partials + (size_t)(pr.item_off + i) * (NSUM_MAX * 2);
-      acc += (double)P[2 * v] - (double)P[2 * v + 1];
+      acc += e.partials[(size_t)(pr.item_off + i) * NSUM_MAX + v];
     }
   }
   sh.dred[grp][v] = acc;
@@ -431,11 +405,10 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
 }
 
 // One correspondence (query q, target point c at cell-sorted slot, float d2)
-// into the lane's compensated moment sums.
-template <int MODE, int NS>
+// into the lane's FP64 moment sums.
+template <int MODE>
 __device__ __forceinline__ void add_pair(const DevEngine& e, float qx, float qy, float qz, const float4& c,
-                                         uint32_t slot, float d2, float ax, float ay, float az, float* acc_s,
-                                         float* acc_c) {
+                                         uint32_t slot, float d2, double ax, double ay, double az, double* acc) {
   float nx = 0.f, ny = 0.f, nz = 0.f;
   if (MODE == RTR_MODE_POINT_TO_PLANE) {
     const float4 nn = e.tgt_nrm_sorted[slot];
@@ -443,10 +416,7 @@ __device__ __forceinline__ void add_pair(const DevEngine& e, float qx, float qy,
     ny = nn.y;
     nz = nn.z;
   }
-  float terms[NSUM_MAX];
-  point_terms(MODE, qx - ax, qy - ay, qz - az, c.x - ax, c.y - ay, c.z - az, nx, ny, nz, d2, terms);
-#pragma unroll
-  for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+  accumulate_pair<MODE>(qx, qy, qz, c.x, c.y, c.z, nx, ny, nz, d2, ax, ay, az, acc);
 }
 
 
@@ -464,10 +434,13 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   float m[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) m[k] = sh.m[k];
-  const float ax = sh.anchor[0], ay = sh.anchor[1], az = sh.anchor[2];
-  float acc_s[NS], acc_c[NS];
+  // the anchor is the same for every lane: keep it in scalar registers
+  const double ax = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[0])));
+  const double ay = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[1])));
+  const double az = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[2])));
+  double acc[NS];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) acc_s[k] = acc_c[k] = 0.f;
+  for (int k = 0; k < NS; ++k) acc[k] = 0.0;
   uint32_t n_search = 0u;
   const uint32_t end = min(itm.y + (uint32_t)e.item_pts, sd.n);
   constexpr bool kCompact = !TILE && DEPTH == 4;
@@ -542,7 +515,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         if (seeds && vis.key != kNoKey) seeds[chunk + o] = vis.where & 0x7fffffffu;
         if (key_d2(vis.key) <= r2f) {
           const uint32_t slot = vis.where & 0x7fffffffu;
-          add_pair<MODE, NS>(e, qx, qy, qz, g.pts[slot], slot, key_d2(vis.key), ax, ay, az, acc_s, acc_c);
+          add_pair<MODE>(e, qx, qy, qz, g.pts[slot], slot, key_d2(vis.key), ax, ay, az, acc);
         }
       }
       __syncthreads();  // the next chunk rewrites the lists
@@ -580,12 +553,12 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
     bool staged = false;
     int X0 = 0, Y0 = 0, Z0 = 0, NY = 1, W1 = 1, nrows = 0;
     if (TILE && e.use_tile && __ballot(blk) != 0ull) {
-      X0 = max(wave_min_i(blk ? cx : 0x7fffffff) - 1, 0);
-      Y0 = max(wave_min_i(blk ? cy : 0x7fffffff) - 1, 0);
-      Z0 = max(wave_min_i(blk ? cz : 0x7fffffff) - 1, 0);
-      const int X1 = min(wave_max_i(blk ? cx : -1) + 1, g.dx - 1);
-      const int Y1 = min(wave_max_i(blk ? cy : -1) + 1, g.dy - 1);
-      const int Z1 = min(wave_max_i(blk ? cz : -1) + 1, g.dz - 1);
+      X0 = max(wave_min_i32(blk ? cx : 0x7fffffff) - 1, 0);
+      Y0 = max(wave_min_i32(blk ? cy : 0x7fffffff) - 1, 0);
+      Z0 = max(wave_min_i32(blk ? cz : 0x7fffffff) - 1, 0);
+      const int X1 = min(wave_max_i32(blk ? cx : -1) + 1, g.dx - 1);
+      const int Y1 = min(wave_max_i32(blk ? cy : -1) + 1, g.dy - 1);
+      const int Z1 = min(wave_max_i32(blk ? cz : -1) + 1, g.dz - 1);
       NY = Y1 - Y0 + 1;
       W1 = X1 - X0 + 2;
       nrows = NY * (Z1 - Z0 + 1);
@@ -677,49 +650,26 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
             slot = tl.row_a[r] + (vis.where - tl.row_off[r]);
           }
         }
-        float nx = 0.f, ny = 0.f, nz = 0.f;
-        if (MODE == RTR_MODE_POINT_TO_PLANE) {
-          const float4 nn = e.tgt_nrm_sorted[slot];
-          nx = nn.x;
-          ny = nn.y;
-          nz = nn.z;
-        }
-        float terms[NSUM_MAX];
-        point_terms(MODE, qx - ax, qy - ay, qz - az, c.x - ax, c.y - ay, c.z - az, nx, ny, nz,
-                    key_d2(vis.key), terms);
-#pragma unroll
-        for (int k = 0; k < NS; ++k) kahan_add(acc_s[k], acc_c[k], terms[k]);
+        add_pair<MODE>(e, qx, qy, qz, c, slot, key_d2(vis.key), ax, ay, az, acc);
       }
     }
     // the next round rewrites this wave's tile
     __builtin_amdgcn_wave_barrier();
   }
   // search statistics: per wave, then the workgroup's LDS totals
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) n_search += __shfl_xor(n_search, off, 64);
+  n_search = wave_sum_u32(n_search);
   if (lane == 0) atomicAdd(&sh.n_search, n_search);
-  // wave reduction with compensation, then across the four waves
+  // wave totals (DPP row sums), then the four waves in a fixed order
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
-    float s = acc_s[k], c = acc_c[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const float s2 = __shfl_xor(s, off, 64);
-      const float c2 = __shfl_xor(c, off, 64);
-      comp_combine(s, c, s2, c2);
-    }
-    if (lane == 0) {
-      sh.red[w][k][0] = s;
-      sh.red[w][k][1] = c;
-    }
+    const double s = wave_sum_f64(acc[k]);
+    if (lane == 0) sh.red[w][k] = s;
   }
   __syncthreads();
   if (tid < NS) {
-    float s = sh.red[0][tid][0], c = sh.red[0][tid][1];
-    for (int ww = 1; ww < kWaves; ++ww) comp_combine(s, c, sh.red[ww][tid][0], sh.red[ww][tid][1]);
-    float* P = e.partials + (size_t)it * (NSUM_MAX * 2);
-    P[2 * tid] = s;
-    P[2 * tid + 1] = c;
+    double s = sh.red[0][tid];
+    for (int ww = 1; ww < kWaves; ++ww) s += sh.red[ww][tid];
+    e.partials[(size_t)it * NSUM_MAX + tid] = s;
     __threadfence();
   }
   __syncthreads();
@@ -740,7 +690,7 @@ template <int MODE, bool TILE, int DEPTH>
 // so resident waves are what hides them.  The 8-deep variant keeps more
 // gathers in flight per wave and runs 2 waves per SIMD; it serves launches
 // tuned to 2 workgroups per CU, where the third wave would not be resident
-// anyway.  Point-to-plane carries 29 compensated sums, runs 2 waves per SIMD
+// anyway.  Point-to-plane carries 29 FP64 sums, runs 2 waves per SIMD
 // and is always 8-deep.
 __global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH == 4) ? 3 : 2) void k_icp(DevEngine e) {
   constexpr int NS = MODE == RTR_MODE_POINT_TO_POINT ? SUM_P2P_COUNT : SUM_P2L_COUNT;
@@ -1191,12 +1141,10 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     ssum += dist * dist;
     pcnt += 1.0;
   }
-  // fixed-order block reduction
-  for (int off = 32; off > 0; off >>= 1) {
-    ssum += __shfl_xor(ssum, off, 64);
-    cnt += __shfl_xor(cnt, off, 64);
-    pcnt += __shfl_xor(pcnt, off, 64);
-  }
+  // fixed-order block reduction: wave totals (DPP row sums), then the waves
+  ssum = wave_sum_f64(ssum);
+  cnt = wave_sum_f64(cnt);
+  pcnt = wave_sum_f64(pcnt);
   // the partials reuse the start of the list area (k >= 3 lists hold far more
   // than 3 doubles per wave) once every thread is done with its list: no
   // static LDS, so eight 20 KB workgroups fill a CU's 160 KB at k = 20

@@ -5,6 +5,7 @@
 // binary_compressed is rejected with RTR_ERR_FORMAT.
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -83,12 +84,18 @@ int parse_header(const std::string& buf, Header& h) {
     set_error("PCD header incomplete");
     return RTR_ERR_FORMAT;
   }
-  if (!have_points) h.points = width * height;
-  if (sizes.size() != names.size() || types.size() != names.size()) {
-    set_error("PCD header: FIELDS/SIZE/TYPE lengths differ");
-    return RTR_ERR_FORMAT;
+  if (!have_points) {
+    if (height != 0 && width > SIZE_MAX / height) {
+      set_error("PCD header: WIDTH x HEIGHT overflows");
+      return RTR_ERR_FORMAT;
+    }
+    h.points = width * height;
   }
   if (counts.empty()) counts.assign(names.size(), 1);
+  if (sizes.size() != names.size() || types.size() != names.size() || counts.size() != names.size()) {
+    set_error("PCD header: FIELDS/SIZE/TYPE/COUNT lengths differ");
+    return RTR_ERR_FORMAT;
+  }
   size_t off = 0, col = 0;
   for (size_t i = 0; i < names.size(); ++i) {
     Field f;
@@ -96,9 +103,21 @@ int parse_header(const std::string& buf, Header& h) {
     f.size = sizes[i];
     f.type = types[i];
     f.count = counts[i];
+    // PCL's scalar types: F 4 or 8 bytes, U and I 1, 2, 4 or 8 bytes
+    const bool size_ok = f.type == 'F' ? (f.size == 4 || f.size == 8)
+                                       : ((f.type == 'U' || f.type == 'I') &&
+                                          (f.size == 1 || f.size == 2 || f.size == 4 || f.size == 8));
+    if (!size_ok) {
+      set_error("PCD header: field %s has TYPE %c with SIZE %d", f.name.c_str(), f.type, f.size);
+      return RTR_ERR_FORMAT;
+    }
+    if (f.count < 1 || f.count > (1 << 20)) {
+      set_error("PCD header: field %s has COUNT %d", f.name.c_str(), f.count);
+      return RTR_ERR_FORMAT;
+    }
     f.offset = h.data == "ascii" ? col : off;
-    off += (size_t)f.size * f.count;
-    col += f.count;
+    off += (size_t)f.size * (size_t)f.count;
+    col += (size_t)f.count;
     h.fields.push_back(f);
   }
   h.record = off;
@@ -124,6 +143,11 @@ double read_scalar(const unsigned char* p, const Field& f) {
         memcpy(&v, p, 2);
         return v;
       }
+      if (f.size == 8) {
+        uint64_t v;
+        memcpy(&v, p, 8);
+        return (double)v;
+      }
       {
         uint32_t v;
         memcpy(&v, p, 4);
@@ -135,6 +159,11 @@ double read_scalar(const unsigned char* p, const Field& f) {
         int16_t v;
         memcpy(&v, p, 2);
         return v;
+      }
+      if (f.size == 8) {
+        int64_t v;
+        memcpy(&v, p, 8);
+        return (double)v;
       }
       {
         int32_t v;
@@ -159,6 +188,16 @@ int load(const char* path, std::string& buf, Header& h) {
   if (h.data != "ascii" && h.data != "binary") {
     set_error("PCD DATA %s is not supported", h.data.c_str());
     return RTR_ERR_FORMAT;
+  }
+  if (h.data == "binary") {
+    // POINTS records of `record` bytes must fit the payload (checked without
+    // forming points * record, which can overflow)
+    const size_t avail = buf.size() >= h.data_pos ? buf.size() - h.data_pos : 0;
+    if (h.record == 0 || h.points > avail / h.record) {
+      set_error("PCD binary payload truncated: %zu points of %zu bytes, %zu bytes present", h.points, h.record,
+                avail);
+      return RTR_ERR_FORMAT;
+    }
   }
   return RTR_OK;
 }
@@ -210,10 +249,6 @@ int rtr_pcd_read(const char* path, float* xyz, float* normals, size_t cap, size_
     const bool has_n = nxi >= 0 && nyi >= 0 && nzi >= 0;
     const int want[6] = {ix, iy, iz, nxi, nyi, nzi};
     if (h.data == "binary") {
-      if (h.data_pos + h.points * h.record > buf.size()) {
-        set_error("PCD binary payload truncated");
-        return RTR_ERR_FORMAT;
-      }
       const unsigned char* base = reinterpret_cast<const unsigned char*>(buf.data()) + h.data_pos;
       for (size_t i = 0; i < h.points; ++i) {
         const unsigned char* rec = base + i * h.record;

@@ -36,10 +36,11 @@ import numpy as np
 
 from . import _lib as L
 from . import multistart, synth
-from .registration import BatchRegistration, make_params
+from .registration import BatchRegistration, make_params, result_dicts
 
 KP = 1024
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "proposer.npz")
+WEIGHTS_F32 = os.path.splitext(WEIGHTS_PATH)[0] + ".f32"
 
 _SHAPES = [("W1", (16, 64)), ("b1", (64,)), ("W2", (64, 128)), ("b2", (128,)), ("W3", (128, 256)), ("b3", (256,)),
            ("W4", (512, 256)), ("b4", (256,)), ("W5", (256, 9)), ("b5", (9,))]
@@ -74,8 +75,12 @@ def init_weights(seed: int = 0) -> np.ndarray:
 
 
 def save_weights(path: str, w, meta: dict) -> None:
+    """The weights as .npz (FP32 array + JSON training record) and, beside
+    it, as a raw little-endian float32 file (same stem, .f32) for
+    rtr_learned_load_weights."""
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez(path, w=np.asarray(w, np.float32), meta=np.array(json.dumps(meta)))
+    np.asarray(w, "<f4").tofile(os.path.splitext(path)[0] + ".f32")
 
 
 def load_weights(path: str = WEIGHTS_PATH):
@@ -95,15 +100,26 @@ class Proposer:
         h = C.c_void_p()
         L.check(L.lib.rtr_prop_create(C.byref(h), int(device), int(max_pairs)))
         self._h = h
+        self._owner = None
         self.max_pairs = max_pairs
         self.n_pairs = 0
         self.set_weights(init_weights(0) if weights is None else weights)
 
+    @classmethod
+    def borrow(cls, handle, owner, max_pairs: int):
+        """A view of the network another handle owns (rtr_learned_batches)."""
+        self = cls.__new__(cls)
+        self._h = handle
+        self._owner = owner
+        self.max_pairs = max_pairs
+        self.n_pairs = 0
+        return self
+
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and getattr(self, "_owner", None) is None:
             L.lib.rtr_prop_destroy(h)
-            self._h = None
+        self._h = None
 
     def set_weights(self, w):
         w = np.ascontiguousarray(w, np.float32)
@@ -357,12 +373,17 @@ class MultiStartRegistration:
                                  for T in multistart.so3_start_poses(self.cs[i], self.ct[i], self.n_starts)])
 
     def align(self):
+        """Best start of every pair; self.inits holds the start pose each
+        pair's result was registered from."""
         res = self.reg.align(self.guesses)
         self.iterations_total = int(sum(r["iterations"] for r in res))
         out = []
+        best = np.empty(self.n, np.int64)
         for i in range(self.n):
             r = res[i * self.n_starts:(i + 1) * self.n_starts]
-            out.append(r[multistart.select_best(r)])
+            best[i] = multistart.select_best(r)
+            out.append(r[best[i]])
+        self.inits = self.guesses[np.arange(self.n) * self.n_starts + best]
         return out
 
 
@@ -373,25 +394,29 @@ def default_params():
 
 
 class LearnedRegistration:
-    """Learned proposal -> ICP refine -> judge -> multi-start fallback.
+    """Learned proposal -> ICP refine -> judge -> multi-start fallback: one
+    call of the library's rtr_learned pipeline (C++, pipeline.cpp).
 
     With hypotheses = k > 1 the network runs on the source samples rotated by
-    k fixed orientations spread over SO(3) (the first the identity) and the
-    candidate with the lowest truncated-chamfer score (rtr_prop_propose_multi)
-    is the proposal ICP refines.
+    k fixed orientations spread over SO(3) (multistart.so3_rotations, the
+    first the identity) and the candidate with the lowest truncated-chamfer
+    score is the proposal ICP refines.
 
     refine_anderson > 0 runs the refine launch with Anderson acceleration of
     that memory (rtr_params.anderson_memory): an equally good minimum in
     fewer iterations; refine_coarse_stride > 1 starts it on every s-th source
     point for up to refine_coarse_iterations iterations (coarse-to-fine,
-    rtr_params.coarse_stride); fallback_anderson does the same for the fallback's
-    multi-start launch (0: the plain iteration).  The fallback runs on an
-    engine of its own that holds only the rejected pairs' clouds: the refine engine keeps one layout (its hipGraph replays every
-    step), and the fallback launch (a few pairs x n_starts problems,
-    latency-bound) runs work items of fallback_item_pts source points on
-    fallback_blocks_per_cu persistent workgroups per CU (fewer idle
-    workgroups polling the work queue), launched directly rather than from a
-    captured hipGraph (its layout changes every call)."""
+    rtr_params.coarse_stride); fallback_anderson does the same for the
+    fallback's multi-start launch (0: the plain iteration).  The judge
+    (rtr_batch_judge) rejects a pair whose plane RMSE is above rmse_threshold
+    or whose inlier fraction is below min_inlier_frac; the rejected pairs run
+    n_starts-start ICP (multistart.so3_rotations about the centroids) on an
+    engine of their own, so the refine engine keeps one layout and replays
+    its captured graph every call.  The refine engine's grids are built on a
+    host thread while the network runs.
+
+    `reg`, `fb` and `prop` are views of the pipeline's refine engine,
+    fallback engine and network (for tuning and profiling)."""
 
     def __init__(self, weights=None, params=None, n_starts: int = 8, rmse_threshold: float = 2e-3,
                  min_inlier_frac: float = 0.8, judge_k: int = 20, judge_points: int = 1024,
@@ -402,92 +427,110 @@ class LearnedRegistration:
         if weights is None:
             weights, _ = load_weights()
         self.params = L.Params.from_buffer_copy(params if params is not None else default_params())
-        self.params.anderson_memory = 0
-        self.refine_params = L.Params.from_buffer_copy(self.params)
-        self.refine_params.anderson_memory = refine_anderson
-        self.refine_params.coarse_stride = refine_coarse_stride
-        self.refine_params.coarse_iterations = refine_coarse_iterations
-        self._active = "refine"
-        self._pool = None
+        cfg = L.LearnedConfig()
+        L.check(L.lib.rtr_learned_config_default(C.byref(cfg)))
+        cfg.hypotheses, cfg.n_starts, cfg.tau = int(hypotheses), int(n_starts), float(tau)
+        cfg.rmse_threshold, cfg.min_inlier_frac = float(rmse_threshold), float(min_inlier_frac)
+        cfg.judge_k, cfg.judge_points, cfg.max_curvature = int(judge_k), int(judge_points), float(max_curvature)
+        cfg.refine_anderson, cfg.refine_coarse_stride = int(refine_anderson), int(refine_coarse_stride)
+        cfg.refine_coarse_iterations, cfg.fallback_anderson = int(refine_coarse_iterations), int(fallback_anderson)
+        cfg.fallback_item_pts, cfg.fallback_blocks_per_cu = int(fallback_item_pts), int(fallback_blocks_per_cu)
+        self.config = cfg
+        h = C.c_void_p()
+        L.check(L.lib.rtr_learned_create(C.byref(h), int(device), int(max_pairs), C.byref(self.params),
+                                         C.byref(cfg)))
+        self._h = h
+        w = np.ascontiguousarray(weights, np.float32)
+        if len(w) != n_params():
+            raise ValueError(f"expected {n_params()} parameters, got {len(w)}")
+        L.check(L.lib.rtr_learned_set_weights(h, L.ptr(w)))
         self.hypotheses = hypotheses
-        self.rots = multistart.so3_rotations(hypotheses) if hypotheses > 1 else None
-        self.tau = tau
-        self.prop = Proposer(max_pairs=max_pairs * max(1, hypotheses), device=device, weights=weights)
-        self.reg = BatchRegistration(device)       # refine: every pair from its proposal
-        self.reg.set_params(self.refine_params)
-        self.reg.set_rebuild(False)                # one grid build per align() call
-        self.fb = BatchRegistration(device)        # fallback: the rejected pairs only
-        fb_params = L.Params.from_buffer_copy(self.params)
-        fb_params.anderson_memory = fallback_anderson
-        self.fb.set_params(fb_params)
-        self.fb.set_tuning(item_pts=fallback_item_pts, blocks_per_cu=fallback_blocks_per_cu)
-        # its layout changes every call: a captured graph would be used once
-        self.fb.set_graph(False)
         self.n_starts = n_starts
-        self.rmse_threshold = rmse_threshold
-        self.min_inlier_frac = min_inlier_frac
-        self.judge_k = judge_k
-        self.judge_points = judge_points
-        self.max_curvature = max_curvature
+        self.tau = tau
+        self.rots = multistart.so3_rotations(hypotheses) if hypotheses > 1 else np.eye(3)[None]
+        self.start_rots = multistart.so3_rotations(n_starts)
+        hr = np.ascontiguousarray(self.rots, np.float64)
+        sr = np.ascontiguousarray(self.start_rots, np.float64)
+        L.check(L.lib.rtr_learned_set_rotations(h, L.ptr(hr), len(hr), L.ptr(sr), len(sr)))
+        reg, fb, prop = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        L.check(L.lib.rtr_learned_batches(h, C.byref(reg), C.byref(fb), C.byref(prop)))
+        self.reg = BatchRegistration.borrow(reg, self)
+        self.fb = BatchRegistration.borrow(fb, self)
+        self.prop = Proposer.borrow(prop, self, max_pairs * max(1, hypotheses))
+        self._active = "refine"
+        self.n = 0
         self.stats = {}
 
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.lib.rtr_learned_destroy(h)
+        self._h = None
+
     def set_clouds(self, sources, targets, src_centroids=None, tgt_centroids=None):
-        self.n = len(sources)
-        self._sources, self._targets = list(sources), list(targets)
-        self.prop.set_clouds(sources, targets)
-        self.reg.set_clouds(sources, targets)
-        self._ids = np.arange(self.n)
-        self.reg.set_problems(self._ids, self._ids)
-        self.cs = _centroids(sources) if src_centroids is None else np.asarray(src_centroids, np.float64)
-        self.ct = _centroids(targets) if tgt_centroids is None else np.asarray(tgt_centroids, np.float64)
+        src = [L.pts3(s) for s in sources]
+        tgt = [L.pts3(t) for t in targets]
+        if len(src) != len(tgt):
+            raise ValueError("sources and targets must pair up")
+        n = len(src)
+        self.cs = _centroids(src) if src_centroids is None else np.ascontiguousarray(src_centroids, np.float64)
+        self.ct = _centroids(tgt) if tgt_centroids is None else np.ascontiguousarray(tgt_centroids, np.float64)
+        sp = (C.c_void_p * n)(*[a.ctypes.data for a in src])
+        tp = (C.c_void_p * n)(*[a.ctypes.data for a in tgt])
+        ns = (C.c_size_t * n)(*[len(a) for a in src])
+        nt = (C.c_size_t * n)(*[len(a) for a in tgt])
+        L.check(L.lib.rtr_learned_set_clouds(self._h, n, sp, ns, tp, nt, L.ptr(self.cs), L.ptr(self.ct)))
+        self._clouds = (src, tgt)          # the library reads the rejected pairs' clouds from these
+        self.n = n
+        self.reg.n_problems = n
+        self.reg._ns = [len(a) for a in src]
+        self.reg._prob_src = np.arange(n, dtype=np.int32)
+        self.reg._max_ids = (n - 1, n - 1)
+        self.prop.n_pairs = n
 
     def use_params(self, which: str):
-        """Give the refine engine the parameters of the 'refine' launch or of
-        the 'plain' iteration (the same launch unaccelerated)."""
-        if self._active != which:
-            self.reg.set_params(self.refine_params if which == "refine" else self.params)
-            self._active = which
+        """The refine engine runs the 'refine' launch or the 'plain' iteration
+        (the same launch unaccelerated, the caller's ICP parameters)."""
+        if which not in ("refine", "plain"):
+            raise ValueError(which)
+        L.check(L.lib.rtr_learned_use_plain_params(self._h, int(which == "plain")))
+        self._active = which
+
+    def _finish(self, res, inits, rej):
+        st = L.LearnedStats()
+        L.check(L.lib.rtr_learned_get_stats(self._h, C.byref(st)))
+        n = self.n
+        props = np.empty((n, 16))
+        L.check(L.lib.rtr_learned_get_proposals(self._h, L.ptr(props)))
+        rmse, frac, pfrac = np.empty(n), np.empty(n), np.empty(n)
+        L.check(L.lib.rtr_learned_get_judge(self._h, L.ptr(rmse), L.ptr(frac), L.ptr(pfrac)))
+        rowm = lambda a: np.ascontiguousarray(np.transpose(a.reshape(n, 4, 4), (0, 2, 1)))
+        self.stats = {"n_fallback": int(st.n_fallback), "fallback_pairs": np.where(rej)[0],
+                      "proposals": rowm(props), "inits": rowm(inits), "judge_rmse": rmse, "judge_frac": frac,
+                      "judge_planar_frac": pfrac,
+                      # ms_propose: the network, with the grid build running beside it;
+                      # ms_build: what is left of the build after the network
+                      "ms_build": st.ms_build_wait, "ms_propose": st.ms_propose, "ms_refine": st.ms_refine,
+                      "ms_judge": st.ms_judge, "ms_fallback": st.ms_fallback, "ms_total": st.ms_total,
+                      "ms_propose_device": st.ms_propose_device}
+        self._active = "refine"
+        return result_dicts(res)
 
     def align(self):
-        t0 = time.perf_counter()
-        # the ICP grids are built on the engine's stream while the proposal
-        # runs on the proposer's: a worker thread waits for the build (the
-        # library releases the interpreter lock in its calls)
-        if self._pool is None:
-            self._pool = ThreadPoolExecutor(1)
-        build = self._pool.submit(self.reg.build)
-        if self.hypotheses > 1:
-            _, _, poses = self.prop.propose_multi(self.rots, self.tau)
-        else:
-            poses = self.prop.propose()
-        tb = time.perf_counter()
-        build.result()
-        t1 = time.perf_counter()
-        self.use_params("refine")
-        res = self.reg.align(poses)
-        t2 = time.perf_counter()
-        rmse, frac, pfrac = self.reg.judge(self.judge_k, self.judge_points, self.max_curvature)
-        t3 = time.perf_counter()
-        fail = np.where((rmse > self.rmse_threshold) | (frac < self.min_inlier_frac))[0]
-        if len(fail):
-            # the fallback engine takes the rejected pairs' clouds (a few MB at
-            # most) and the refine engine's cell settings
-            cf, cpp = self.reg.tuning()[:2]
-            if self.fb.tuning()[:2] != (cf, cpp):
-                self.fb.set_tuning(cell_factor=cf, cells_per_point=cpp)
-            self.fb.set_clouds([self._sources[i] for i in fail], [self._targets[i] for i in fail])
-            ids = np.repeat(np.arange(len(fail)), self.n_starts)
-            self.fb.set_problems(ids, ids)
-            g = np.stack([T for i in fail for T in multistart.so3_start_poses(self.cs[i], self.ct[i], self.n_starts)])
-            res8 = self.fb.align(g)
-            for k, i in enumerate(fail):
-                r = res8[k * self.n_starts:(k + 1) * self.n_starts]
-                res[i] = r[multistart.select_best(r)]
-        t4 = time.perf_counter()
-        self.stats = {"n_fallback": int(len(fail)), "fallback_pairs": fail, "proposals": poses, "judge_rmse": rmse,
-                      # ms_propose: the proposal, with the grid build running beside it;
-                      # ms_build: what is left of the build after the proposal
-                      "judge_frac": frac, "judge_planar_frac": pfrac, "ms_build": (t1 - tb) * 1e3,
-                      "ms_propose": (tb - t0) * 1e3, "ms_refine": (t2 - t1) * 1e3, "ms_judge": (t3 - t2) * 1e3,
-                      "ms_fallback": (t4 - t3) * 1e3, "ms_propose_device": self.prop.last_ms()}
-        return res
+        """Register every pair: proposal, refine, judge, fallback."""
+        n = self.n
+        res = (L.Result * n)()
+        inits = np.empty((n, 16))
+        rej = np.zeros(n, np.int32)
+        L.check(L.lib.rtr_learned_align(self._h, res, L.ptr(inits), L.ptr(rej)))
+        return self._finish(res, inits, rej)
+
+    def align_from(self, proposals):
+        """Refine, judge and fall back from the caller's (n, 4, 4) proposals."""
+        n = self.n
+        g = np.ascontiguousarray(np.transpose(np.asarray(proposals, np.float64), (0, 2, 1))).reshape(-1)
+        res = (L.Result * n)()
+        inits = np.empty((n, 16))
+        rej = np.zeros(n, np.int32)
+        L.check(L.lib.rtr_learned_align_from(self._h, L.ptr(g), res, L.ptr(inits), L.ptr(rej)))
+        return self._finish(res, inits, rej)

@@ -88,12 +88,23 @@ def so3_start_poses(src_centroid, tgt_centroid, n_starts: int = 8):
     """Starts for arbitrary orientation: the source rotated about its
     centroid by each of so3_rotations(n_starts), centroid moved onto the
     target's."""
-    cs = np.asarray(src_centroid, np.float64)
-    ct = np.asarray(tgt_centroid, np.float64)
+    return start_poses_from(so3_rotations(n_starts), src_centroid, tgt_centroid)
+
+
+def start_poses_from(rotations, src_centroid, tgt_centroid):
+    """T = [R | ct - R cs] for each rotation, with R cs summed left to right
+    in FP64 (the order the library's fallback uses, so both give the same
+    bits)."""
+    cs = [float(v) for v in np.asarray(src_centroid, np.float64)]
+    ct = [float(v) for v in np.asarray(tgt_centroid, np.float64)]
     out = []
-    for R in so3_rotations(n_starts):
+    for R in np.asarray(rotations, np.float64):
         T = np.eye(4)
         T[:3, :3] = R
-        T[:3, 3] = ct - R @ cs
+        for r in range(3):
+            rc = float(R[r, 0]) * cs[0]
+            rc = rc + float(R[r, 1]) * cs[1]
+            rc = rc + float(R[r, 2]) * cs[2]
+            T[r, 3] = ct[r] - rc
         out.append(T)
     return out

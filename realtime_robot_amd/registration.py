@@ -209,15 +209,28 @@ class BatchRegistration:
         h = C.c_void_p()
         L.check(L.lib.rtr_batch_create(C.byref(h), device))
         self._h = h
+        self._owner = None
         self._keep = []
         self.n_problems = 0
         self._max_ids = (-1, -1)
 
+    @classmethod
+    def borrow(cls, handle, owner):
+        """A view of an engine another handle owns (rtr_learned_batches); it
+        keeps `owner` alive and never destroys the engine."""
+        self = cls.__new__(cls)
+        self._h = handle
+        self._owner = owner
+        self._keep = []
+        self.n_problems = 0
+        self._max_ids = (-1, -1)
+        return self
+
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and getattr(self, "_owner", None) is None:
             L.lib.rtr_batch_destroy(h)
-            self._h = None
+        self._h = None
 
     def set_clouds(self, sources, targets):
         src = [L.pts3(s) for s in sources]

@@ -71,6 +71,61 @@ def test_pcd_errors():
     assert e.value.code == L.ERR_IO
 
 
+_PCD_HEAD = "VERSION 0.7\nFIELDS {f}\nSIZE {s}\nTYPE {t}\nCOUNT {c}\nWIDTH {n}\nHEIGHT 1\nPOINTS {n}\nDATA {d}\n"
+
+
+@pytest.mark.parametrize("fields,sizes,types,counts,why", [
+    ("x y z", "4 4 4", "F F F", "1 1", "COUNT shorter than FIELDS"),
+    ("x y z", "4 4", "F F F", "1 1 1", "SIZE shorter than FIELDS"),
+    ("x y z", "4 4 4", "F F", "1 1 1", "TYPE shorter than FIELDS"),
+    ("x y z", "2 4 4", "F F F", "1 1 1", "F with SIZE 2"),
+    ("x y z", "4 4 3", "F F F", "1 1 1", "SIZE 3"),
+    ("x y z", "4 4 4", "F F F", "1 0 1", "COUNT 0"),
+    ("x y z", "4 4 4", "F F F", "1 -2 1", "negative COUNT"),
+    ("x y z", "4 4 -4", "F F F", "1 1 1", "negative SIZE"),
+    ("x y z", "4 4 4", "F F Q", "1 1 1", "unknown TYPE"),
+])
+def test_pcd_malformed_headers_are_rejected(tmp_path, fields, sizes, types, counts, why):
+    """Header fields that disagree, or sizes that do not match a PCL scalar
+    type, are a format error, never a read past the payload."""
+    for data in ("binary", "ascii"):
+        path = tmp_path / f"bad_{data}.pcd"
+        body = b"\x00" * 48 if data == "binary" else b"0 0 0\n0 0 0\n"
+        path.write_bytes(_PCD_HEAD.format(f=fields, s=sizes, t=types, c=counts, n=2, d=data).encode() + body)
+        with pytest.raises(R.RtrError) as e:
+            R.load_pcd(str(path))
+        assert e.value.code == L.ERR_FORMAT, why
+
+
+def test_pcd_binary_truncation_and_huge_counts(tmp_path):
+    ok = tmp_path / "ok.pcd"
+    xyz = np.arange(12, dtype=np.float32).reshape(4, 3)
+    ok.write_bytes(_PCD_HEAD.format(f="x y z", s="4 4 4", t="F F F", c="1 1 1", n=4, d="binary").encode()
+                   + xyz.tobytes())
+    got, _ = R.load_pcd(str(ok))
+    assert np.array_equal(got, xyz)
+    # 8-byte integer fields are a PCL scalar type and are read as such
+    i8 = tmp_path / "i8.pcd"
+    rec = np.zeros(2, dtype=[("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("label", "<i8")])
+    rec["x"], rec["label"] = [1.5, -2.0], [-(1 << 40), 7]
+    i8.write_bytes(_PCD_HEAD.format(f="x y z label", s="4 4 4 8", t="F F F I", c="1 1 1 1", n=2,
+                                    d="binary").encode() + rec.tobytes())
+    got, _ = R.load_pcd(str(i8))
+    assert np.array_equal(got[:, 0], [1.5, -2.0])
+    short = tmp_path / "short.pcd"
+    short.write_bytes(_PCD_HEAD.format(f="x y z", s="4 4 4", t="F F F", c="1 1 1", n=4, d="binary").encode()
+                      + xyz.tobytes()[:40])
+    with pytest.raises(R.RtrError) as e:
+        R.load_pcd(str(short))
+    assert e.value.code == L.ERR_FORMAT
+    # a point count whose payload size overflows size_t is truncation, not a wrap
+    huge = tmp_path / "huge.pcd"
+    huge.write_bytes(_PCD_HEAD.format(f="x y z", s="8 8 8", t="F F F", c="1 1 1", n=(1 << 62) + 1,
+                                      d="binary").encode() + xyz.tobytes())
+    with pytest.raises(R.RtrError):
+        R.load_pcd(str(huge))
+
+
 @pytest.mark.parametrize("cell,rmax", [(0.05, 0.08), (0.02, None), (0.3, 0.05), (0.011, 0.2), (0.004, 0.05),
                                        (0.004, None)])
 def test_host_grid_search_matches_brute_force(cell, rmax):

@@ -32,6 +32,37 @@ def test_kdtree_matches_brute_force(seed):
         assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_kdtree_bounded_search_equals_unbounded(seed):
+    """The radius-bounded search (used by the FP64 ICP) gives the unbounded
+    search's answers, in both tie modes, on ties-heavy grid data and with
+    queries straddling the radius exactly."""
+    rng = np.random.default_rng(seed)
+    t = (np.round(rng.random((4000, 3)) * 25) / 25).astype(np.float32)
+    t = np.concatenate([t, t[:500]])                       # duplicate points: exact ties
+    q = (np.round(rng.random((3000, 3)) * 50) / 50 * 1.3 - 0.15).astype(np.float32)
+    kd = oracle.KdTree(t)
+    for r in (0.02, 0.04, 0.1, 1.0):
+        for tie in (True, False):
+            i1, d1 = kd.nn(q, r, tie_lowest=tie)
+            i2, d2 = kd.nn(q, r, tie_lowest=tie, bounded=True)
+            assert np.array_equal(i1, i2) and np.array_equal(d1, d2), (r, tie)
+            if tie:
+                ib, db = oracle.nn_brute(q, t, r)
+                assert np.array_equal(ib, i2) and np.array_equal(db, d2), r
+    # a query at exactly the radius from its only neighbour is kept
+    r = 0.5
+    t1 = np.array([[0, 0, 0]], np.float32)
+    x = np.sqrt(np.float32(oracle.r2_to_float(r * r)))
+    qs = np.array([[x, 0, 0]], np.float32)
+    d_exact = np.float32(qs[0, 0] * qs[0, 0])
+    kd1 = oracle.KdTree(t1)
+    for tie in (True, False):
+        a = kd1.nn(qs, r, tie_lowest=tie)
+        b = kd1.nn(qs, r, tie_lowest=tie, bounded=True)
+        assert np.array_equal(a[0], b[0]) and (a[0][0] == 0) == (float(d_exact) <= r * r)
+
+
 def test_brute_force_is_the_textbook_argmin():
     rng = np.random.default_rng(3)
     t = rng.random((500, 3)).astype(np.float32)
