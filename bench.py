@@ -268,9 +268,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-config3", dest="config3", action="store_false",
                     help="leave out the config-3 line (batched plain single-start ICP)")
+    ap.add_argument("--no-fusion", dest="fusion", action="store_false",
+                    help="leave out the voxel-hash fusion line")
     ap.add_argument("--account-sample", type=int, default=24)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
-    ap.add_argument("--leg", choices=["both", "refine", "learned", "plain"], default="both",
+    ap.add_argument("--leg", choices=["both", "refine", "learned", "plain", "fusion"], default="both",
                     help="'refine': only the learned path's refine launches; 'learned': only learned steps; "
                          "'plain': only the plain multi-start launches (kernel profiles)")
     args = ap.parse_args()
@@ -282,6 +284,14 @@ def main():
     import realtime_robot_amd as R
     from realtime_robot_amd import _lib as L
     from realtime_robot_amd import accounting, autotune, learned, multistart, synth
+
+    if args.leg == "fusion":
+        # the fusion launches alone (kernel profiling), then the copy probe
+        out = fusion_leg(args, repeats=args.steps)
+        copy_ms = ctypes.c_float()
+        L.check(L.lib.rtr_copy_probe(0, args.copy_bytes, 3, ctypes.byref(copy_ms)))
+        log(f"fusion launches done: {out['kernel_ms']:.3f} ms; copy probe {copy_ms.value:.3f} ms")
+        return
 
     t0 = time.perf_counter()
     srcs, tgts, gts = synth.config4_batch(args.pairs)
@@ -450,6 +460,7 @@ def main():
 
     # config 3: batched plain single-start ICP, its own line
     c3 = config3_leg(args, params, threads, timed) if args.config3 else None
+    fusion = fusion_leg(args) if args.fusion else None
 
     # FP64 references for every registration, outside the timed region:
     # (a) from each pair's proposal, (b) from a rejected pair's fallback start,
@@ -560,6 +571,8 @@ def main():
     }
     if c3 is not None:
         out["config3_plain_icp"] = c3
+    if fusion is not None:
+        out["fusion"] = fusion
     if not args.no_cpu_baseline:
         starts = [multistart.so3_start_poses(cs[i], ct[i], args.starts) for i in range(n)]
         out["cpu_baseline"] = cpu_baseline(srcs, tgts, starts, args.starts, args.cpu_budget, T64_gt, threads)
@@ -579,6 +592,57 @@ def main():
                 lat.append(time.perf_counter() - t0)
         out["cpu_baseline"]["gpu_latency_ms_config1"] = float(np.median(lat)) * 1e3
     print(json.dumps(out), flush=True)
+
+
+def fusion_leg(args, frames=64, repeats=5):
+    """Voxel-hash fusion at config 5's map: `frames` depth frames of the room
+    (640x480, depth noise N(0, 0.0015 z^2)) back-projected and integrated at
+    their camera poses into a 2 cm map of 8M-voxel capacity, all frames in
+    one launch (rtr_map_integrate_frames), the map cleared before each; the
+    integration kernel timed with HIP events on the map's stream (median of
+    `repeats` after a warm-up), the points resident in HBM (the upload is
+    outside the events).  Also the per-frame launches of the streaming loop.
+    Algorithmic bytes: 52 B per point (12 B xyz read, 8 B key + 16 B payload
+    read and written)."""
+    from realtime_robot_amd import synth
+    from realtime_robot_amd.registration import VoxelMap
+    t0 = time.perf_counter()
+    depths, poses, _ = synth.config5_frames(frames, 0, 640, 480)
+    clouds = [synth.backproject(d) for d in depths]
+    npts = int(sum(len(c) for c in clouds))
+    log(f"fusion: {frames} frames, {npts} points rendered in {time.perf_counter() - t0:.1f} s")
+    m = VoxelMap(0.02, capacity=8 << 20)
+    m.set_profiling(True)
+    ms = []
+    for rep in range(repeats + 1):
+        m.clear()
+        m.integrate_frames(clouds, poses)
+        if rep:
+            ms.append(m.last_ms())
+    voxels = m.size()
+    m.clear()
+    per = []
+    for c, T in zip(clouds, poses):
+        m.integrate(c, T)
+        per.append(m.last_ms())
+    k_ms = float(np.median(ms))
+    algo = 52.0 * npts
+    traffic, src = None, None
+    prof = os.path.join(ROOT, "profiles", "k_map_integrate_64frames.json")
+    if os.path.exists(prof) and frames == 64:
+        with open(prof) as f:
+            traffic, src = float(json.load(f)["traffic_bytes"]), "profiles/k_map_integrate_64frames.md"
+    log(f"fusion: {k_ms:.3f} ms per {frames}-frame launch, {npts / k_ms / 1e6:.2f} G points/s, {voxels} voxels")
+    return {"workload": f"voxel-hash fusion: {frames} depth frames of the config-5 room (640x480, noise "
+                        f"N(0,0.0015z^2)) at their camera poses into a 2 cm map (capacity 8M voxels), one launch",
+            "points": npts, "voxels": int(voxels), "kernel_ms": k_ms, "points_per_s": npts / (k_ms / 1e3),
+            "per_frame_kernel_ms_median": float(np.median(per)), "per_frame_points_mean": npts / frames,
+            "roofline": {"bound": "hbm", "achieved": algo / (k_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": algo / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                         "traffic_source": src, "algorithmic_bytes": algo,
+                         "achieved_basis": "52 B per point / HIP-event kernel time",
+                         "frac_measured_traffic": (traffic / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None),
+                         "kernel": "k_map_integrate"}}
 
 
 def config3_leg(args, params, threads, timed):

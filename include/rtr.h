@@ -248,9 +248,19 @@ typedef struct rtr_map rtr_map;
 int rtr_map_create(rtr_map** out, int device, float voxel_size, size_t capacity);
 int rtr_map_destroy(rtr_map* m);
 int rtr_map_clear(rtr_map* m);
-/* Insert T * xyz (T nullable = identity): per voxel FP32 sum of xyz and count. */
+/* Insert T * xyz (T nullable = identity): per voxel FP32 sum of xyz and count.
+ * The merge of one transformed cloud into the map (RealTimeRobot.cpp:105-109:
+ * transformPointCloud, then the merged overlay). */
 int rtr_map_integrate(rtr_map* m, const float* xyz, size_t n, size_t stride_bytes,
                       const double T[16]);
+/* The same for n_frames clouds in one launch, frame f moved by T + 16 f
+ * (column-major; T nullable = identity for all). */
+int rtr_map_integrate_frames(rtr_map* m, int n_frames, const float* const* xyz, const size_t* n,
+                             size_t stride_bytes, const double* T);
+/* Device time of the integration kernel of the last integrate call (ms),
+ * recorded while profiling is on. */
+int rtr_map_set_profiling(rtr_map* m, int on);
+int rtr_map_last_ms(rtr_map* m, float* ms);
 int rtr_map_size(rtr_map* m, size_t* n_voxels);
 /* Copy out occupied voxels sorted by key: keys (packed 21 bits/axis, +2^20
  * bias), sums (3 per voxel) or centroids when as_centroids != 0, counts. */
@@ -291,6 +301,17 @@ int rtr_copy_probe(int device, size_t bytes, int reps, float* ms_avg);
 /* matching.h:181-187).                                                  */
 /* ------------------------------------------------------------------ */
 int rtr_tdf(int device, const int32_t* occ_xyz, size_t n_occ, const int dims[3], float trunc2, float* out);
+/* Allocation-free variant of rtr_tdf.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  occ_dev: n_occ x 3 int32 voxel coordinates
+ * and out_dev: dims[0]*dims[1]*dims[2] floats (x fastest), both device
+ * memory the caller owns; the kernels run asynchronously on `stream` (a
+ * hipStream_t; NULL: the default stream), and the call makes no allocation:
+ * the intermediate grid lives in the library's static per-device array of
+ * 2^21 floats (8 MiB), so grids above 2^21 voxels are refused
+ * (RTR_ERR_CAPACITY).  Calls that overlap on one device race on that array. */
+int rtr_tdf_noalloc(int device, const int32_t* occ_dev, size_t n_occ, const int dims[3], float trunc2,
+                    float* out_dev, void* stream);
 /* Same from points: voxel = floor((p - origin) / resolution) inside the
  * grid (the occupancy of key_point.h:287-304 and matching.h:169-186). */
 int rtr_tdf_points(int device, const float* xyz, size_t n, size_t stride_bytes, const float origin[3],

@@ -126,6 +126,9 @@ _SIGS = {
     "rtr_map_destroy": (_i, [_vp]),
     "rtr_map_clear": (_i, [_vp]),
     "rtr_map_integrate": (_i, [_vp, _vp, _sz, _sz, _vp]),
+    "rtr_map_integrate_frames": (_i, [_vp, _i, _vp, _vp, _sz, _vp]),
+    "rtr_map_set_profiling": (_i, [_vp, _i]),
+    "rtr_map_last_ms": (_i, [_vp, C.POINTER(_f)]),
     "rtr_map_size": (_i, [_vp, C.POINTER(_sz)]),
     "rtr_map_extract": (_i, [_vp, _vp, _vp, _vp, _sz, C.POINTER(_sz), _i]),
     "rtr_map_dropped": (_i, [_vp, C.POINTER(C.c_uint64)]),
@@ -136,6 +139,7 @@ _SIGS = {
     "rtr_copy_probe": (_i, [_i, _sz, _i, C.POINTER(_f)]),
     "rtr_tdf": (_i, [_i, _vp, _sz, _vp, _f, _vp]),
     "rtr_tdf_points": (_i, [_i, _vp, _sz, _sz, _vp, _f, _vp, _f, _vp]),
+    "rtr_tdf_noalloc": (_i, [_i, _vp, _sz, _vp, _f, _vp, _vp]),
     "rtr_coarse_create": (_i, [C.POINTER(_vp), _i]),
     "rtr_coarse_destroy": (_i, [_vp]),
     "rtr_coarse_set_clouds": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),

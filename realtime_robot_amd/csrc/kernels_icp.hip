@@ -420,6 +420,17 @@ __device__ __forceinline__ void add_pair(const DevEngine& e, float qx, float qy,
 }
 
 
+// The wave's moment sums (DPP row sums, then the four rows) into its LDS
+// slot, added to what earlier chunks of the item left there when `add`.
+template <int NS, class Sh>
+__device__ __forceinline__ void wave_totals(const double* acc, Sh& sh, int w, int lane, bool add) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const double s = wave_sum_f64(acc[k]);
+    if (lane == 0) sh.red[w][k] = add ? sh.red[w][k] + s : s;
+  }
+}
+
 template <int MODE, int NS, bool TILE, class TileT, int DEPTH>
 __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS, TileT>& sh) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -431,16 +442,14 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   const TargetDesc td = e.tgts[pr.tgt];
   const GridView g = grid_of(e, td);
   const float r2f = e.prm.r2f;
+  // the transform rows and the anchor are the same for every lane: keep them
+  // in scalar registers
   float m[12];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) m[k] = sh.m[k];
-  // the anchor is the same for every lane: keep it in scalar registers
+  for (int k = 0; k < 12; ++k) m[k] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.m[k])));
   const double ax = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[0])));
   const double ay = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[1])));
   const double az = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[2])));
-  double acc[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) acc[k] = 0.0;
   uint32_t n_search = 0u;
   const uint32_t end = min(itm.y + (uint32_t)e.item_pts, sd.n);
   constexpr bool kCompact = !TILE && DEPTH == 4;
@@ -513,14 +522,34 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         if (D < 2) walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
         walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, D >= 2 ? D - 1 : 1);
         if (seeds && vis.key != kNoKey) seeds[chunk + o] = vis.where & 0x7fffffffu;
-        if (key_d2(vis.key) <= r2f) {
-          const uint32_t slot = vis.where & 0x7fffffffu;
-          add_pair<MODE>(e, qx, qy, qz, g.pts[slot], slot, key_d2(vis.key), ax, ay, az, acc);
-        }
+        // the query's correspondence (its seed slot is read above by this
+        // lane only, so the slot can hold the result)
+        sh.qseed[o] = key_d2(vis.key) <= r2f ? (vis.where & 0x7fffffffu) : ~0u;
       }
+      __syncthreads();
+      // Moment phase: the walks above keep no moment sums live (their
+      // registers go to the gathers in flight); the chunk's correspondences
+      // are re-read from LDS in query order, each query's transform and
+      // squared distance recomputed exactly as the walk computed them.
+      double acc[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+      for (uint32_t o = tid; o < cn; o += kBlock) {
+        const uint32_t slot = sh.qseed[o];
+        if (slot == ~0u) continue;
+        const uint32_t sj = sd.off + chunk + o;
+        float qx, qy, qz;
+        apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+        const float4 c = g.pts[slot];
+        add_pair<MODE>(e, qx, qy, qz, c, slot, d2_f32(qx, qy, qz, c.x, c.y, c.z), ax, ay, az, acc);
+      }
+      wave_totals<NS>(acc, sh, w, lane, chunk != itm.y);
       __syncthreads();  // the next chunk rewrites the lists
     }
   }
+  double acc[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) acc[k] = 0.0;
   for (uint32_t base = itm.y + (uint32_t)w * 64u; !kCompact && base < end; base += kBlock) {
     const uint32_t j = base + lane;
     float qx = 0.f, qy = 0.f, qz = 0.f;
@@ -660,11 +689,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   n_search = wave_sum_u32(n_search);
   if (lane == 0) atomicAdd(&sh.n_search, n_search);
   // wave totals (DPP row sums), then the four waves in a fixed order
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    const double s = wave_sum_f64(acc[k]);
-    if (lane == 0) sh.red[w][k] = s;
-  }
+  if (!kCompact) wave_totals<NS>(acc, sh, w, lane, false);
   __syncthreads();
   if (tid < NS) {
     double s = sh.red[0][tid];

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/rtr.h"
@@ -127,6 +128,36 @@ static int tdf_run(int device, const int dims[3], float trunc2, float* out, cons
   return RTR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant (rtr_tdf_noalloc): the caller's device buffers hold
+// the occupied voxels and the field; the intermediate grid of the separable
+// passes is this file-scope device array, one per GPU, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel: its device
+// address, per device, comes from hipGetSymbolAddress on the first call on
+// that device.
+// ---------------------------------------------------------------------------
+constexpr size_t kTdfScratchMax = (size_t)1 << 21;  // floats: 8 MiB, grids of up to 2^21 voxels (128^3)
+__device__ __align__(256) float scratch[kTdfScratchMax];
+
+constexpr int kMaxDevices = 64;
+static float* g_scratch_dev[kMaxDevices] = {nullptr};
+static std::mutex g_scratch_mu;
+
+static int scratch_of(int device, float** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    RTR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("hipGetSymbolAddress returned no address for the TDF scratch array");
+      return RTR_ERR_DEVICE;
+    }
+    g_scratch_dev[device] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[device];
+  return RTR_OK;
+}
+
 static bool tdf_dims_ok(const int dims[3]) {
   if (!dims) return false;
   size_t n = 1;
@@ -153,6 +184,42 @@ int rtr_tdf(int device, const int32_t* occ, size_t n_occ, const int dims[3], flo
     set_error("host allocation failed");
     return RTR_ERR_CAPACITY;
   }
+}
+
+int rtr_tdf_noalloc(int device, const int32_t* occ_dev, size_t n_occ, const int dims[3], float trunc2, float* out_dev,
+                    void* stream) {
+  using namespace rtr;
+  if (!out_dev || (n_occ && !occ_dev) || !tdf_dims_ok(dims) || !(trunc2 > 0.f)) {
+    set_error("rtr_tdf_noalloc: invalid arguments");
+    return RTR_ERR_INVALID_ARG;
+  }
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  if (n > kTdfScratchMax) {
+    set_error("rtr_tdf_noalloc: a %d x %d x %d grid has %zu voxels, above the allocation-free limit of %zu",
+              dims[0], dims[1], dims[2], n, kTdfScratchMax);
+    return RTR_ERR_CAPACITY;
+  }
+  int n_dev = 0;
+  RTR_HIP(hipGetDeviceCount(&n_dev));
+  if (device < 0 || device >= n_dev || device >= kMaxDevices) {
+    set_error("rtr_tdf_noalloc: device %d not available", device);
+    return RTR_ERR_DEVICE;
+  }
+  RTR_HIP(hipSetDevice(device));
+  float* g = nullptr;
+  RTR_TRY(scratch_of(device, &g));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int dx = dims[0], dy = dims[1], dz = dims[2];
+  // every scratch voxel is written by the fill before it is read: nothing
+  // left by an earlier call on this device is used
+  hipLaunchKernelGGL(k_tdf_fill, dim3(1024), dim3(256), 0, s, g, n);
+  if (n_occ) hipLaunchKernelGGL(k_tdf_mark, dim3(256), dim3(256), 0, s, occ_dev, n_occ, dx, dy, dz, g);
+  const unsigned blocks = 2048;
+  hipLaunchKernelGGL(k_tdf_pass<0>, dim3(blocks), dim3(256), 0, s, g, out_dev, dx, dy, dz, 0.f);
+  hipLaunchKernelGGL(k_tdf_pass<1>, dim3(blocks), dim3(256), 0, s, out_dev, g, dx, dy, dz, 0.f);
+  hipLaunchKernelGGL(k_tdf_pass<2>, dim3(blocks), dim3(256), 0, s, g, out_dev, dx, dy, dz, trunc2);
+  RTR_HIP(hipGetLastError());
+  return RTR_OK;
 }
 
 int rtr_tdf_points(int device, const float* xyz, size_t n, size_t stride_bytes, const float origin[3],

@@ -9,6 +9,14 @@
 // per voxel a 64-bit key, the FP32 sum of (p - voxel corner) and a count.
 // Offsets from the corner keep the FP32 sums small, so the centroid keeps
 // full float precision however many frames land in the voxel.
+//
+// Integration kernel: each workgroup takes tiles of 512 consecutive points
+// of one frame, stages them in LDS with coalesced loads, and merges the
+// tile's points voxel by voxel in an LDS hash table first (neighbouring
+// pixels of a depth frame share voxels), so the global table sees one insert
+// and one set of atomics per distinct voxel of the tile, not per point.
+// Global probes are bounded (the table holds at most half its slots).  Many
+// frames (each with its own transform) integrate in one launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,51 +68,134 @@ RTR_HD bool voxel_key(float x, float y, float z, float inv_h, uint64_t* key, int
   return true;
 }
 
-__global__ void k_map_integrate(MapDev m, const float* xyz, uint32_t n, const float* T12,
-                                int has_T) {
-  float mt[12];
-  if (has_T)
-    for (int i = 0; i < 12; ++i) mt[i] = T12[i];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float px = xyz[3 * (size_t)i], py = xyz[3 * (size_t)i + 1], pz = xyz[3 * (size_t)i + 2];
-    if (has_T) apply_f32(mt, px, py, pz, px, py, pz);
-    uint64_t key;
-    int k3[3];
-    if (!voxel_key(px, py, pz, m.inv_h, &key, k3)) {
-      atomicAdd(&m.counters[1], 1u);
-      continue;
-    }
-    uint64_t slot = mix64(key) & m.mask;
-    bool placed = false;
-    for (uint64_t probe = 0; probe <= m.mask; ++probe) {
-      unsigned long long cur = m.keys[slot];
+constexpr int kMapBlock = 256;
+constexpr int kMapTile = 512;    // points per workgroup tile
+constexpr int kMapSlots = 1024;  // the tile's LDS hash table (at most half full)
+constexpr int kMaxProbe = 128;   // global probe bound (the table is at most half full)
+static_assert(kMapTile < kMapSlots, "every tile key must find a free LDS slot");
+
+// One frame of an integration launch: packed xyz on the device, its point
+// count, its first tile in the launch and the float rows of its transform.
+struct MapFrame {
+  const float* xyz;
+  uint32_t n;
+  uint32_t tile0;
+  float m[12];
+  int has_T;
+  int pad_;
+};
+
+// One merged voxel of a tile into the global table: (sum of corner offsets,
+// count).  False when the probe bound is reached (table too full).
+__device__ __forceinline__ bool global_add(const MapDev& m, uint64_t key, float sx, float sy, float sz, uint32_t cnt) {
+  uint64_t slot = mix64(key) & m.mask;
+  for (int probe = 0; probe < kMaxProbe; ++probe) {
+    // a stale empty read only costs a CAS; a key once written never changes
+    unsigned long long cur = m.keys[slot];
+    if (cur == kEmpty) {
+      cur = atomicCAS(&m.keys[slot], (unsigned long long)kEmpty, (unsigned long long)key);
       if (cur == kEmpty) {
-        cur = atomicCAS(&m.keys[slot], (unsigned long long)kEmpty, (unsigned long long)key);
-        if (cur == kEmpty) {
-          const uint32_t occ = atomicAdd(&m.counters[0], 1u);
-          if (occ >= m.max_voxels) atomicOr(&m.counters[2], 1u);
-          cur = key;
-        }
+        const uint32_t occ = atomicAdd(&m.counters[0], 1u);
+        if (occ >= m.max_voxels) atomicOr(&m.counters[2], 1u);
+        cur = key;
       }
-      if (cur == key) {
-        const float dx = px - (float)k3[0] * m.h;
-        const float dy = py - (float)k3[1] * m.h;
-        const float dz = pz - (float)k3[2] * m.h;
-        float* a = reinterpret_cast<float*>(&m.acc[slot]);
-        atomicAdd(a + 0, dx);
-        atomicAdd(a + 1, dy);
-        atomicAdd(a + 2, dz);
-        atomicAdd(reinterpret_cast<uint32_t*>(a + 3), 1u);
-        placed = true;
-        break;
-      }
-      slot = (slot + 1) & m.mask;
     }
-    if (!placed) {
-      atomicAdd(&m.counters[1], 1u);
-      atomicOr(&m.counters[2], 1u);
+    if (cur == key) {
+      float* a = reinterpret_cast<float*>(&m.acc[slot]);
+      atomicAdd(a + 0, sx);
+      atomicAdd(a + 1, sy);
+      atomicAdd(a + 2, sz);
+      atomicAdd(reinterpret_cast<uint32_t*>(a + 3), cnt);
+      return true;
     }
+    slot = (slot + 1) & m.mask;
   }
+  return false;
+}
+
+__global__ __launch_bounds__(kMapBlock) void k_map_integrate(MapDev m, const MapFrame* frames, int n_frames,
+                                                             uint32_t n_tiles) {
+  __shared__ float s_xyz[3 * kMapTile];
+  __shared__ unsigned long long s_key[kMapSlots];
+  __shared__ float4 s_acc[kMapSlots];
+  __shared__ int s_frame;
+  const int tid = threadIdx.x;
+  uint32_t dropped = 0, overflow = 0;
+  for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    if (tid == 0) {
+      // the frame of this tile: the last one whose first tile is <= tile
+      int lo = 0, hi = n_frames - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (frames[mid].tile0 <= tile)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      s_frame = lo;
+    }
+    for (int i = tid; i < kMapSlots; i += kMapBlock) {
+      s_key[i] = kEmpty;
+      s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    const MapFrame* f = frames + s_frame;
+    const uint32_t base = (tile - f->tile0) * (uint32_t)kMapTile;
+    const uint32_t cnt = min((uint32_t)kMapTile, f->n - base);
+    // coalesced staging: consecutive lanes load consecutive floats
+    const float* src = f->xyz + 3 * (size_t)base;
+    for (uint32_t j = tid; j < 3 * cnt; j += kMapBlock) s_xyz[j] = src[j];
+    const int has_T = f->has_T;
+    float mt[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) mt[k] = f->m[k];
+    __syncthreads();
+    for (uint32_t p = tid; p < cnt; p += kMapBlock) {
+      float px = s_xyz[3 * p], py = s_xyz[3 * p + 1], pz = s_xyz[3 * p + 2];
+      if (has_T) apply_f32(mt, px, py, pz, px, py, pz);
+      uint64_t key;
+      int k3[3];
+      if (!voxel_key(px, py, pz, m.inv_h, &key, k3)) {
+        ++dropped;
+        continue;
+      }
+      const float dx = px - (float)k3[0] * m.h;
+      const float dy = py - (float)k3[1] * m.h;
+      const float dz = pz - (float)k3[2] * m.h;
+      // at most kMapTile distinct keys per tile: a free or matching slot
+      // always exists within kMapSlots probes
+      uint32_t sl = (uint32_t)(mix64(key) & (uint64_t)(kMapSlots - 1));
+      for (int probe = 0; probe < kMapSlots; ++probe) {
+        unsigned long long cur = s_key[sl];
+        if (cur == kEmpty) {
+          cur = atomicCAS(&s_key[sl], (unsigned long long)kEmpty, (unsigned long long)key);
+          if (cur == kEmpty) cur = key;
+        }
+        if (cur == key) {
+          atomicAdd(&s_acc[sl].x, dx);
+          atomicAdd(&s_acc[sl].y, dy);
+          atomicAdd(&s_acc[sl].z, dz);
+          atomicAdd(reinterpret_cast<uint32_t*>(&s_acc[sl].w), 1u);
+          break;
+        }
+        sl = (sl + 1) & (uint32_t)(kMapSlots - 1);
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < kMapSlots; i += kMapBlock) {
+      const unsigned long long k = s_key[i];
+      if (k == kEmpty) continue;
+      const float4 a = s_acc[i];
+      const uint32_t c = __float_as_uint(a.w);
+      if (!global_add(m, k, a.x, a.y, a.z, c)) {
+        dropped += c;
+        overflow = 1;
+      }
+    }
+    __syncthreads();  // the next tile rewrites the LDS table
+  }
+  if (dropped) atomicAdd(&m.counters[1], dropped);
+  if (overflow) atomicOr(&m.counters[2], 1u);
 }
 
 __global__ void k_map_compact(MapDev m, unsigned long long* okeys, float4* oacc, uint32_t* ocount,
@@ -133,6 +224,10 @@ __global__ void k_map_clear(MapDev m) {
 struct VoxelMap {
   int device;
   hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool profiling = false;
+  float last_ms = 0.f;
+  int cu_count = 0;
   DevBuf keys, acc, counters, in, T, okeys, oacc, ocount;
   MapDev dev;
   size_t capacity = 0;
@@ -148,6 +243,9 @@ struct VoxelMap {
     }
     RTR_HIP(hipSetDevice(device));
     RTR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    RTR_HIP(hipEventCreate(&ev[0]));
+    RTR_HIP(hipEventCreate(&ev[1]));
+    RTR_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, device));
     uint64_t slots = 1024;
     while (slots < 2 * (uint64_t)cap) slots <<= 1;
     capacity = cap;
@@ -178,6 +276,8 @@ struct VoxelMap {
     if (stream) (void)hipStreamSynchronize(stream);
     DevBuf* b[] = {&keys, &acc, &counters, &in, &T, &okeys, &oacc, &ocount};
     for (DevBuf* x : b) x->release();
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
   int read_counters(uint32_t c[4]) {
@@ -185,45 +285,70 @@ struct VoxelMap {
     RTR_HIP(hipStreamSynchronize(stream));
     return RTR_OK;
   }
-  int integrate(const float* xyz, size_t n, size_t stride, const double* Tcm) {
+  // nf frames in one launch: frame f's points (packed, or with a byte
+  // stride) moved by T[16 f ..] (column-major; T or its entries nullable
+  // means identity)
+  int integrate_frames(int nf, const float* const* xyz, const size_t* n, size_t stride, const double* Tcm) {
     RTR_HIP(hipSetDevice(device));
-    if (n == 0) return RTR_OK;
-    if (n >= (1ull << 32)) {
+    if (stride == 0) stride = 12;
+    size_t total = 0, tiles = 0;
+    for (int f = 0; f < nf; ++f) {
+      total += n[f];
+      tiles += (n[f] + kMapTile - 1) / kMapTile;
+    }
+    if (total == 0) return RTR_OK;
+    if (total >= (1ull << 32) || tiles >= (1ull << 32)) {
       set_error("too many points in one call");
       return RTR_ERR_CAPACITY;
     }
+    RTR_TRY(in.ensure(total * 12));
+    RTR_TRY(T.ensure(sizeof(MapFrame) * (size_t)nf));
+    std::vector<MapFrame> fr(nf);
     std::vector<float> packed;
-    const float* src = xyz;
-    if (stride != 0 && stride != 12) {
-      packed.resize(n * 3);
-      const char* b = reinterpret_cast<const char*>(xyz);
-      for (size_t i = 0; i < n; ++i) memcpy(packed.data() + 3 * i, b + i * stride, 12);
-      src = packed.data();
+    size_t off = 0;
+    uint32_t tile0 = 0;
+    for (int f = 0; f < nf; ++f) {
+      MapFrame& d = fr[f];
+      memset(&d, 0, sizeof(d));
+      d.xyz = in.as<float>() + 3 * off;
+      d.n = (uint32_t)n[f];
+      d.tile0 = tile0;
+      d.has_T = Tcm ? 1 : 0;
+      if (Tcm) tf_rows_f32(Tcm + 16 * (size_t)f, d.m);
+      if (n[f]) {
+        const float* src = xyz[f];
+        if (stride != 12) {
+          packed.resize(n[f] * 3);
+          const char* b = reinterpret_cast<const char*>(xyz[f]);
+          for (size_t i = 0; i < n[f]; ++i) memcpy(packed.data() + 3 * i, b + i * stride, 12);
+          src = packed.data();
+        }
+        // synchronous for pageable memory: the staging buffer is reused
+        RTR_HIP(hipMemcpyAsync(in.as<float>() + 3 * off, src, n[f] * 12, hipMemcpyHostToDevice, stream));
+        if (stride != 12) RTR_HIP(hipStreamSynchronize(stream));
+      }
+      off += n[f];
+      tile0 += (uint32_t)((n[f] + kMapTile - 1) / kMapTile);
     }
-    int rc = in.ensure(n * 12);
-    if (rc) return rc;
-    RTR_HIP(hipMemcpyAsync(in.p, src, n * 12, hipMemcpyHostToDevice, stream));
-    float m[12];
-    if (Tcm) {
-      tf_rows_f32(Tcm, m);
-      RTR_HIP(hipMemcpyAsync(T.p, m, sizeof(m), hipMemcpyHostToDevice, stream));
-    }
-    uint32_t before[4];
-    rc = read_counters(before);
-    if (rc) return rc;
-    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_map_integrate, dim3(blocks), dim3(256), 0, stream, dev, in.as<float>(),
-                       (uint32_t)n, T.as<float>(), Tcm ? 1 : 0);
+    RTR_HIP(hipMemcpyAsync(T.p, fr.data(), sizeof(MapFrame) * (size_t)nf, hipMemcpyHostToDevice, stream));
+    const unsigned blocks = (unsigned)std::min<size_t>(tiles, (size_t)std::max(cu_count, 1) * 16);
+    if (profiling) RTR_HIP(hipEventRecord(ev[0], stream));
+    hipLaunchKernelGGL(k_map_integrate, dim3(blocks), dim3(kMapBlock), 0, stream, dev, T.as<MapFrame>(), nf,
+                       (uint32_t)tiles);
     RTR_HIP(hipGetLastError());
+    if (profiling) RTR_HIP(hipEventRecord(ev[1], stream));
     uint32_t after[4];
-    rc = read_counters(after);
-    if (rc) return rc;
+    RTR_TRY(read_counters(after));
+    if (profiling) RTR_HIP(hipEventElapsedTime(&last_ms, ev[0], ev[1]));
     dropped_total = after[1];
     if (after[2]) {
       set_error("voxel map capacity exceeded (%zu voxels)", capacity);
       return RTR_ERR_CAPACITY;
     }
     return RTR_OK;
+  }
+  int integrate(const float* xyz, size_t n, size_t stride, const double* Tcm) {
+    return integrate_frames(1, &xyz, &n, stride, Tcm);
   }
   int size(size_t* n) {
     uint32_t c[4];
@@ -331,6 +456,31 @@ int rtr_map_integrate(rtr_map* m, const float* xyz, size_t n, size_t stride, con
     set_error("host allocation failed");
     return RTR_ERR_CAPACITY;
   }
+}
+
+int rtr_map_integrate_frames(rtr_map* m, int n_frames, const float* const* xyz, const size_t* n, size_t stride,
+                             const double* T) {
+  if (!m || n_frames < 1 || !xyz || !n || (stride != 0 && stride < 12)) return RTR_ERR_INVALID_ARG;
+  for (int f = 0; f < n_frames; ++f)
+    if (n[f] && !xyz[f]) return RTR_ERR_INVALID_ARG;
+  try {
+    return m->m.integrate_frames(n_frames, xyz, n, stride, T);
+  } catch (...) {
+    set_error("host allocation failed");
+    return RTR_ERR_CAPACITY;
+  }
+}
+
+int rtr_map_set_profiling(rtr_map* m, int on) {
+  if (!m) return RTR_ERR_INVALID_ARG;
+  m->m.profiling = on != 0;
+  return RTR_OK;
+}
+
+int rtr_map_last_ms(rtr_map* m, float* ms) {
+  if (!m || !ms) return RTR_ERR_INVALID_ARG;
+  *ms = m->m.last_ms;
+  return RTR_OK;
 }
 
 int rtr_map_size(rtr_map* m, size_t* n) {

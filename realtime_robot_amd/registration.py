@@ -378,6 +378,26 @@ class VoxelMap:
         Tc = None if T is None else L.colmajor(T)
         L.check(L.lib.rtr_map_integrate(self._h, L.ptr(a), len(a), 12, L.ptr(Tc)))
 
+    def integrate_frames(self, clouds, poses=None):
+        """Every cloud (moved by its pose, (n, 4, 4) or None) in one launch."""
+        arrs = [L.pts3(c) for c in clouds]
+        k = len(arrs)
+        ptrs = (C.c_void_p * k)(*[a.ctypes.data for a in arrs])
+        ns = (C.c_size_t * k)(*[len(a) for a in arrs])
+        Tc = None
+        if poses is not None:
+            Tc = np.ascontiguousarray(np.transpose(np.asarray(poses, np.float64), (0, 2, 1))).reshape(-1)
+        L.check(L.lib.rtr_map_integrate_frames(self._h, k, ptrs, ns, 12, L.ptr(Tc)))
+
+    def set_profiling(self, on: bool):
+        L.check(L.lib.rtr_map_set_profiling(self._h, int(on)))
+
+    def last_ms(self) -> float:
+        """Device time of the last integration kernel (profiling on)."""
+        ms = C.c_float()
+        L.check(L.lib.rtr_map_last_ms(self._h, C.byref(ms)))
+        return ms.value
+
     def size(self) -> int:
         n = C.c_size_t()
         L.check(L.lib.rtr_map_size(self._h, C.byref(n)))

@@ -9,6 +9,8 @@ target (the pose ICP should recover).
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 
@@ -408,10 +410,26 @@ def render_depth(T_cw, boxes, width=640, height=480, fx=525.0, fy=525.0, cx=319.
     """Depth image (metres along the optical axis, 0 = no hit) of the room
     (floor + 4 walls, open ceiling) and the boxes seen from camera-to-world
     pose T_cw, with depth noise N(0, noise_k * z^2) when noise_rng is given."""
+    z = render_depth_f64(T_cw, boxes, width, height, fx, fy, cx, cy, size, max_depth)
+    if noise_rng is not None:
+        add_depth_noise(z, noise_rng, noise_k)
+    return z.reshape(height, width).astype(np.float32)
+
+
+def _rows_times(A, M):
+    """A @ M for (n, 3) A and 3x3 M, summed left to right elementwise."""
+    return A[:, 0:1] * M[0] + A[:, 1:2] * M[1] + A[:, 2:3] * M[2]
+
+
+def render_depth_f64(T_cw, boxes, width=640, height=480, fx=525.0, fy=525.0, cx=319.5, cy=239.5,
+                     size=(5.0, 5.0, 2.5), max_depth: float = 8.0):
+    """The noise-free depth of render_depth as a flat float64 array."""
     u, v = np.meshgrid(np.arange(width, dtype=np.float64), np.arange(height, dtype=np.float64))
     d_cam = np.stack([(u - cx) / fx, (v - cy) / fy, np.ones_like(u)], axis=-1).reshape(-1, 3)
     R, o = T_cw[:3, :3], T_cw[:3, 3]
-    d = d_cam @ R.T
+    # explicit products (no BLAS): the same bits on every machine and safe to
+    # call from several threads at once
+    d = _rows_times(d_cam, R.T)
     t_best = np.full(len(d), np.inf)
     with np.errstate(divide="ignore", invalid="ignore"):
         # room: floor z = 0 and the four walls, hit from inside
@@ -421,8 +439,9 @@ def render_depth(T_cw, boxes, width=640, height=480, fx=525.0, fy=525.0, cx=319.
             ok = (t > 1e-6) & np.all((p >= -1e-9) & (p <= np.array(size) + 1e-9), axis=1)
             t_best = np.where(ok & (t < t_best), t, t_best)
         for c, h, Rb in boxes:
-            ol = (o - c) @ Rb
-            dl = d @ Rb
+            oc = o - c
+            ol = oc[0] * Rb[0] + oc[1] * Rb[1] + oc[2] * Rb[2]
+            dl = _rows_times(d, Rb)
             t1 = (-h - ol) / dl
             t2 = (h - ol) / dl
             tmin = np.nanmax(np.minimum(t1, t2), axis=1)
@@ -431,10 +450,13 @@ def render_depth(T_cw, boxes, width=640, height=480, fx=525.0, fy=525.0, cx=319.
             t_best = np.where(ok & (tmin < t_best), tmin, t_best)
     z = t_best  # d_cam has unit z, so the ray parameter is the depth
     z[~np.isfinite(z) | (z > max_depth)] = 0.0
-    if noise_rng is not None:
-        hit = z > 0
-        z[hit] += noise_rng.normal(0.0, 1.0, hit.sum()) * noise_k * z[hit] ** 2
-    return z.reshape(height, width).astype(np.float32)
+    return z
+
+
+def add_depth_noise(z, rng, noise_k: float = 0.0015):
+    """z += N(0, noise_k z^2) on the hit pixels (z > 0), in place, in pixel order."""
+    hit = z > 0
+    z[hit] += rng.normal(0.0, 1.0, hit.sum()) * noise_k * z[hit] ** 2
 
 
 def backproject(depth, fx=525.0, fy=525.0, cx=319.5, cy=239.5):
@@ -449,10 +471,22 @@ def config5_frames(n_frames: int = 64, seed: int = 0, width: int = 640, height: 
     """Streaming workload: depth frames of the config-2 room along camera_path,
     depth noise N(0, 0.0015 z^2).  Returns (depth images, camera poses,
     intrinsics scale) with intrinsics scaled to the resolution."""
+    from concurrent.futures import ThreadPoolExecutor
     boxes = room_boxes(seed)
     poses = camera_path(n_frames, seed)
     s = width / 640.0
     rng = rng_for(5, 1000 + seed)
-    depths = [render_depth(T, boxes, width, height, 525.0 * s, 525.0 * s, (width - 1) / 2.0, (height - 1) / 2.0,
-                           noise_rng=rng) for T in poses]
+
+    def clean(T):
+        # noise-free float64 depth (the noise is drawn below, frame by frame,
+        # from one generator: the same draws as rendering the frames in order)
+        return render_depth_f64(T, boxes, width, height, 525.0 * s, 525.0 * s, (width - 1) / 2.0,
+                                (height - 1) / 2.0)
+
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        zs = list(ex.map(clean, poses))
+    depths = []
+    for z in zs:
+        add_depth_noise(z, rng)
+        depths.append(z.reshape(height, width).astype(np.float32))
     return np.stack(depths), poses, s

@@ -153,3 +153,42 @@ def test_fallback_rescues_injected_wrong_proposals(pairs, pipeline):
     assert np.array_equal(st["fallback_pairs"], fail) and np.array_equal(st["inits"], inits)
     print(f"injected {inject}: rejected {st['fallback_pairs'].tolist()}, fallback {st['ms_fallback']:.1f} ms",
           flush=True)
+
+
+def test_cpp_learned_registration_drop_in(tmp_path, pairs):
+    """The C++ class (rtr::LearnedRegistration, library defaults: its own
+    rotation sets, FP64 centroids) on PCD files, weights from the raw float32
+    file: the registrations it accepts land within 1 mm of the FP64 oracle's
+    registration from the generating pose for most pairs, as the Python
+    pipeline's do."""
+    import os
+    import subprocess
+
+    import realtime_robot_amd as R
+    from realtime_robot_amd import _lib as L
+    srcs, tgts, gts = pairs[0][:8], pairs[1][:8], pairs[2][:8]
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "learned_batch"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "learned_batch.cpp"), "-o", str(exe),
+                    "-L", os.path.dirname(L.LIB_PATH), "-lrtr", "-Wl,-rpath," + os.path.dirname(L.LIB_PATH)],
+                   check=True)
+    args = [str(exe), learned.WEIGHTS_F32]
+    clouds = []
+    for i in range(len(srcs)):
+        for name, c in (("s", srcs[i]), ("t", tgts[i])):
+            path = tmp_path / f"{name}{i}.pcd"
+            R.save_pcd_ascii(str(path), c)
+            args.append(str(path))
+        clouds.append((R.load_pcd(str(tmp_path / f"s{i}.pcd"))[0], R.load_pcd(str(tmp_path / f"t{i}.pcd"))[0]))
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("pair ")]
+    assert len(lines) == len(srcs)
+    ref = _fp64_from((list(c[0] for c in clouds), list(c[1] for c in clouds)), range(len(srcs)), gts)
+    ok = 0
+    for i, ln in enumerate(lines):
+        T = np.array([float(v) for v in ln[7:23]]).reshape(4, 4).T
+        ok += synth.rmse_between(T, ref[i], clouds[i][0]) <= 1e-3
+    print(r.stdout.splitlines()[-1], f"; {ok}/{len(srcs)} within 1 mm", flush=True)
+    assert ok >= len(srcs) - 2

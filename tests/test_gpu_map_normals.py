@@ -85,6 +85,36 @@ def test_voxel_map_matches_oracle():
     assert np.abs(c3 - cen_ref).max() < 2e-6
 
 
+def test_voxel_map_frames_in_one_launch_match_the_oracle():
+    """Several clouds with their own transforms in one launch (the batched
+    fusion): keys and counts exact, centroids within 2e-6 of the FP64
+    oracle; ragged sizes around the 512-point tiles, an empty frame, and a
+    tile whose points all share one voxel (the LDS merge at its densest)."""
+    rng = np.random.default_rng(3)
+    base = read_xyz(golden("ground.pcd"))
+    clouds = [base[:511], base[:513], np.zeros((0, 3), np.float32), base,
+              np.full((1500, 3), 0.0101, np.float32) + rng.normal(0, 1e-4, (1500, 3)).astype(np.float32)]
+    poses = [synth.random_rigid(rng, 30.0, 0.3) for _ in clouds]
+    m = R.VoxelMap(0.02, capacity=1 << 18)
+    m.integrate_frames(clouds, poses)
+    keys, sums, counts = m.extract(centroids=False)
+    allp = np.concatenate([oracle.transform(T, c) for T, c in zip(poses, clouds) if len(c)])
+    inv = np.float32(1.0 / np.float64(np.float32(0.02)))
+    k2, s2, c2, dropped = oracle.voxel_accumulate(allp, inv)
+    assert dropped == 0 and m.dropped() == 0
+    assert np.array_equal(keys, k2) and np.array_equal(counts.astype(np.int64), c2)
+    assert counts.sum() == sum(len(c) for c in clouds)
+    cen = sums.astype(np.float64) / counts[:, None]
+    assert np.abs(cen - s2 / c2[:, None]).max() < 2e-6
+    # the same clouds one call each give the same voxels
+    m2 = R.VoxelMap(0.02, capacity=1 << 18)
+    for T, c in zip(poses, clouds):
+        m2.integrate(c, T)
+    k3, s3, c3 = m2.extract(centroids=False)
+    assert np.array_equal(keys, k3) and np.array_equal(counts, c3)
+    assert np.abs(s3.astype(np.float64) / c3[:, None] - cen).max() < 2e-6
+
+
 def test_voxel_map_capacity_is_an_error_not_a_corruption():
     rng = np.random.default_rng(1)
     pts = (rng.random((20000, 3)) * 10).astype(np.float32)
