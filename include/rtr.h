@@ -216,6 +216,13 @@ int rtr_batch_get_grid(rtr_batch* b, int tgt, float* cell, int dims[3], float an
  * (12 floats per entry): entry i for iteration i, entry `iterations` the final
  * one.  *n_out = iterations + 1. */
 int rtr_batch_get_history(rtr_batch* b, int prob, float* rows12, int cap, int* n_out);
+/* Launch timeline (diagnostic; off by default): with on = 1, each align
+ * records, per problem and iteration, the device wall clock at the start and
+ * at the end of the iteration's solve (the last item of the iteration done;
+ * the next iteration published).  rtr_batch_get_timeline copies problem
+ * `prob`'s 2 * iterations stamps (ticks of *tick_ns nanoseconds). */
+int rtr_batch_set_timeline(rtr_batch* b, int on);
+int rtr_batch_get_timeline(rtr_batch* b, int prob, uint64_t* stamps, int cap, int* n_out, double* tick_ns);
 /* Correspondences of the last iteration the latest align ran for problem
  * `prob`, as its search found them (parity hook; the per-iteration
  * correspondences PCL's ICP keeps, function.h:117): idx[i] the nearest target

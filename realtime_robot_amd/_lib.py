@@ -120,6 +120,8 @@ _SIGS = {
     "rtr_batch_get_grid": (_i, [_vp, _i, _vp, _vp, _vp]),
     "rtr_batch_fitness": (_i, [_vp, _d, _vp]),
     "rtr_batch_get_history": (_i, [_vp, _i, _vp, _i, C.POINTER(_i)]),
+    "rtr_batch_set_timeline": (_i, [_vp, _i]),
+    "rtr_batch_get_timeline": (_i, [_vp, _i, _vp, _i, C.POINTER(_i), C.POINTER(_d)]),
     "rtr_batch_get_last_correspondences": (_i, [_vp, _i, _vp, _vp]),
     "rtr_batch_judge": (_i, [_vp, _i, _i, _d, _vp, _vp, _vp]),
     "rtr_map_create": (_i, [C.POINTER(_vp), _i, _f, _sz]),

@@ -23,6 +23,8 @@ CACHE = os.environ.get("RTR_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "realtime_robot_amd", "tune.json"))
 
 # (cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile[, retire])
+# blocks_per_cu also picks the search kernel: 1-2 the 8-deep one (2 waves per
+# SIMD), 4 the one built for 4 waves per SIMD, else 3 waves per SIMD
 # retire: workgroups beyond one per CU leave once fewer than this fraction of
 # the problems are unfinished (0: never; rtr_batch_set_retire)
 DEFAULT = (0.25, 2.0, 1024, 0, 0)
@@ -35,6 +37,14 @@ CANDIDATES = [
     (0.25, 2.0, 1024, 2, 0),
     (0.35, 2.0, 1024, 0, 0, 0.25),
     (0.35, 2.0, 1024, 0, 0, 0.5),
+    # 4 workgroups per CU: the search kernel compiled for 4 waves per SIMD
+    (0.35, 2.0, 1024, 4, 0),
+    (0.25, 2.0, 1024, 4, 0),
+    (0.35, 2.0, 1024, 4, 0, 0.5),
+    # LDS candidate tiles (each wave stages its queries' 3x3x3 blocks)
+    (0.35, 2.0, 1024, 0, 1),
+    (0.25, 2.0, 512, 0, 1),
+    (0.35, 2.0, 512, 2, 1),
 ]
 # Few problems per launch (one refine per pair, a small fallback set): the
 # launch is bound by each problem's per-iteration latency, so smaller work
@@ -63,6 +73,10 @@ LATENCY_CANDIDATES = CANDIDATES + [
     (0.25, 2.0, 512, 0, 0, 0.6),
     (0.25, 2.0, 256, 0, 0, 0.5),
     (0.25, 2.0, 1024, 0, 0, 0.5),
+    (0.25, 2.0, 1024, 4, 0, 0.5),
+    (0.25, 2.0, 512, 4, 0, 0.5),
+    (0.25, 2.0, 512, 0, 1, 0.5),
+    (0.25, 2.0, 256, 2, 1, 0.5),
 ]
 
 

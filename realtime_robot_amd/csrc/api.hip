@@ -471,6 +471,20 @@ int rtr_batch_get_grid(rtr_batch* b, int tgt, float* cell, int dims[3], float an
   RTR_GUARD_END
 }
 
+int rtr_batch_set_timeline(rtr_batch* b, int on) {
+  if (!b) return RTR_ERR_INVALID_ARG;
+  b->eng.set_timeline(on != 0);
+  b->eng.set_params(b->eng.params());
+  return RTR_OK;
+}
+
+int rtr_batch_get_timeline(rtr_batch* b, int prob, uint64_t* stamps, int cap, int* n_out, double* tick_ns) {
+  RTR_GUARD_BEGIN
+  if (!b || !stamps || !n_out || !tick_ns) return RTR_ERR_INVALID_ARG;
+  return b->eng.timeline(prob, stamps, cap, n_out, tick_ns);
+  RTR_GUARD_END
+}
+
 int rtr_batch_get_history(rtr_batch* b, int prob, float* rows12, int cap, int* n_out) {
   RTR_GUARD_BEGIN
   if (!b || !rows12 || !n_out) return RTR_ERR_INVALID_ARG;

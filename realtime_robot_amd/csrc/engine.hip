@@ -59,7 +59,7 @@ Engine::~Engine() {
                     &cell_start_, &cell_id_, &cell_rank_, &cell_dist_, &cell_dist_tmp_, &tgt_desc_, &bbox_tmp_, &src_raw_, &sx_, &sy_, &sz_, &perm_,
                     &bin_count_, &bin_start_, &bin_id_, &bin_rank_, &src_desc_, &prob_desc_, &state_,
                     &guesses_, &items_buf_, &partials_, &done_, &counters_, &scan_tmp_,
-                    &out_i32_, &out_f32_, &out_T_, &hist_, &aa_, &nn_seed_};
+                    &out_i32_, &out_f32_, &out_T_, &hist_, &aa_, &nn_seed_, &tline_};
   for (DevBuf* b : bufs) b->release();
   if (pin_) (void)hipHostFree(pin_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -337,6 +337,7 @@ int Engine::layout_problems() {
   RTR_TRY(counters_.ensure(counter_words() * 4));
   RTR_TRY(out_T_.ensure(16 * sizeof(double)));
   RTR_TRY(hist_.ensure(np * (size_t)(prm_.max_iterations + 1) * 12 * sizeof(float)));
+  if (timeline_) RTR_TRY(tline_.ensure(np * (size_t)prm_.max_iterations * 2 * sizeof(uint64_t)));
   if (prm_.anderson_memory > 0) RTR_TRY(aa_.ensure(np * (size_t)kAndersonStride * sizeof(double)));
   // search seeds: one slot per problem and source point (pt_off is 32-bit)
   if (pt_total_ > 0 && pt_total_ <= 0xffffffffull) RTR_TRY(nn_seed_.ensure(pt_total_ * 4));
@@ -395,13 +396,15 @@ DevEngine Engine::dev_view() const {
   e.st = state_.as<ProblemState>();
   e.guesses = guesses_.as<double>();
   e.hist = hist_.as<float>();
+  e.tline = timeline_ && tline_.bytes >= probs_.size() * (size_t)prm_.max_iterations * 16 ? tline_.as<uint64_t>()
+                                                                                          : nullptr;
   e.aa = prm_.anderson_memory > 0 ? aa_.as<double>() : nullptr;
   e.nn_seed = (pt_total_ > 0 && pt_total_ <= 0xffffffffull) ? nn_seed_.as<uint32_t>() : nullptr;
   e.items = items_buf_.as<uint2>();
   e.n_items = (int)items_.size();
   e.item_pts = tune_.item_pts;
   e.use_tile = tune_.use_tile;
-  e.deep = deep_gathers() ? 1 : 0;
+  e.variant = variant();
   e.retire_from = tune_.retire > 0.f ? cu_count_ : 0x7fffffff;
   e.retire_active = (uint32_t)((double)tune_.retire * (double)probs_.size());
   e.partials = partials_.as<double>();
@@ -441,7 +444,7 @@ DevEngine Engine::dev_view() const {
 }
 
 int Engine::grid_blocks() const {
-  int bpc = icp_blocks_per_cu(prm_.mode, tune_.use_tile, deep_gathers());
+  int bpc = icp_blocks_per_cu(prm_.mode, tune_.use_tile, variant());
   if (tune_.blocks_per_cu > 0 && tune_.blocks_per_cu < bpc) bpc = tune_.blocks_per_cu;
   long long b = (long long)cu_count_ * bpc;
   if (b > (long long)items_.size()) b = (long long)items_.size();
@@ -748,6 +751,36 @@ int Engine::history(int prob, float* rows12, int cap, int* n_out) {
   const size_t off = (size_t)prob * (prm_.max_iterations + 1) * 12;
   RTR_HIP(hipMemcpyAsync(rows12, hist_.as<float>() + off, (size_t)n * 12 * sizeof(float), hipMemcpyDeviceToHost,
                          stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  return RTR_OK;
+}
+
+// Wall-clock stamps of problem `prob`'s solves in the last align (timeline
+// on): 2 per iteration (the solve's start and end), in ticks of tick_ns.
+int Engine::timeline(int prob, uint64_t* stamps, int cap, int* n_out, double* tick_ns) {
+  RTR_HIP(hipSetDevice(device_));
+  if (prob < 0 || prob >= (int)probs_.size()) {
+    set_error("problem %d out of range", prob);
+    return RTR_ERR_INVALID_ARG;
+  }
+  if (!timeline_ || tline_.bytes < probs_.size() * (size_t)prm_.max_iterations * 16) {
+    set_error("the timeline is off (rtr_batch_set_timeline), or no align has run with it on");
+    return RTR_ERR_NOT_READY;
+  }
+  ProblemState st;
+  RTR_HIP(hipMemcpyAsync(&st, state_.as<ProblemState>() + prob, sizeof(st), hipMemcpyDeviceToHost, stream_));
+  RTR_HIP(hipStreamSynchronize(stream_));
+  const int n = 2 * st.iter;
+  *n_out = n;
+  if (n > cap) {
+    set_error("timeline of %d stamps, buffer holds %d", n, cap);
+    return RTR_ERR_CAPACITY;
+  }
+  int khz = 0;
+  RTR_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
+  *tick_ns = khz > 0 ? 1e6 / (double)khz : 0.0;
+  RTR_HIP(hipMemcpyAsync(stamps, tline_.as<uint64_t>() + (size_t)prob * prm_.max_iterations * 2,
+                         (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
   RTR_HIP(hipStreamSynchronize(stream_));
   return RTR_OK;
 }

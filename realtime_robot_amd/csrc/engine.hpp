@@ -81,6 +81,11 @@ class Engine {
   int judge(int k, int max_pts, double max_curv, double* plane_rmse, double* inlier_frac, double* planar_frac);
   int grid_info(int tgt, float* cell, int dims[3], float anchor[3]);
   int history(int prob, float* rows12, int cap, int* n_out);
+  void set_timeline(bool on) {
+    if (on != timeline_) graph_dirty_ = true;
+    timeline_ = on;
+  }
+  int timeline(int prob, uint64_t* stamps, int cap, int* n_out, double* tick_ns);
   int last_correspondences(int prob, int32_t* idx, float* d2);
   int target_normals(int tgt, float* out);
   const rtr_timing& timing() const { return timing_; }
@@ -108,6 +113,7 @@ class Engine {
   bool use_graph_ = true;
   bool profiling_ = false;
   bool rebuild_always_ = true;
+  bool timeline_ = false;
   bool grid_dirty_ = true;
   bool results_valid_ = false;  // states/grids hold a completed align of the current layout
   bool normals_valid_ = false;
@@ -131,7 +137,10 @@ class Engine {
   int claim_stride() const { return prm_.max_iterations <= 256 ? 32 : 1; }
   // the 8-deep search kernel (2 waves per SIMD) serves launches tuned to at
   // most 2 workgroups per CU, where a third wave would not be resident
-  bool deep_gathers() const { return tune_.blocks_per_cu == 1 || tune_.blocks_per_cu == 2; }
+  // ... and the 4-wave variant launches tuned to 4 workgroups per CU
+  int variant() const {
+    return (tune_.blocks_per_cu == 1 || tune_.blocks_per_cu == 2) ? 1 : (tune_.blocks_per_cu == 4 ? 2 : 0);
+  }
   size_t counter_words() const { return 96 + (size_t)prm_.max_iterations * kXcds * claim_stride(); }
 
   // device arena
@@ -140,7 +149,7 @@ class Engine {
   DevBuf tgt_desc_, bbox_tmp_;
   DevBuf src_raw_, sx_, sy_, sz_, perm_, bin_count_, bin_start_, bin_id_, bin_rank_, src_desc_;
   DevBuf prob_desc_, state_, guesses_, items_buf_, partials_, done_, counters_, scan_tmp_;
-  DevBuf out_i32_, out_f32_, out_T_, hist_, aa_, nn_seed_;
+  DevBuf out_i32_, out_f32_, out_T_, hist_, aa_, nn_seed_, tline_;
   // pinned host staging
   void* pin_ = nullptr;
   size_t pin_bytes_ = 0;

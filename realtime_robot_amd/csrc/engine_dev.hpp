@@ -47,6 +47,8 @@ struct DevEngine {
   ProblemState* st;
   const double* guesses;     // n_prob * 16 (column-major)
   float* hist;               // n_prob * (max_iter + 1) * 12: float rows of T used by each iteration
+  uint64_t* tline;           // n_prob * max_iter * 2 wall-clock stamps: start and end of each iteration's
+                             // solve (nullable: timeline off)
   double* aa;                // n_prob * kAndersonStride: acceleration state (null when off)
   uint32_t* nn_seed;         // per problem and sorted source point (at Problem::pt_off): cell-sorted
                              // slot of its last nearest neighbour, the next search's seed (nullable)
@@ -54,7 +56,8 @@ struct DevEngine {
   int n_items;
   int item_pts;              // points per item
   int use_tile;              // stage candidate tiles in LDS (else global gathers only)
-  int deep;                  // 8-deep gathers, 2 waves per SIMD (point-to-point; always for point-to-plane)
+  int variant;               // search kernel (point-to-point): 0 4-deep gathers at 3 waves per SIMD, 1 8-deep
+                             // at 2, 2 4-deep at 4 (point-to-plane: always 8-deep at 2)
   int retire_from;           // workgroups from this index on leave at the start of an iteration ...
   uint32_t retire_active;    // ... once at most this many problems are unfinished
   double* partials;          // n_items * NSUM_MAX FP64 moment sums
@@ -78,7 +81,7 @@ size_t scan_tmp_words(uint32_t n);
 void launch_target_grid(const DevEngine& e, int max_tgt_pts, int max_cells, hipStream_t s);
 void launch_source_sort(const DevEngine& e, int max_src_pts, uint32_t src_total, hipStream_t s);
 void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s);
-int icp_blocks_per_cu(int mode, int use_tile, int deep);
+int icp_blocks_per_cu(int mode, int use_tile, int variant);
 // correspondences / fitness / transformed output for problem set; T per
 // problem from `T` (n_prob*16 column-major, device) or from the state when null.
 void launch_correspondences(const DevEngine& e, int prob, const double* T, float r2f, int32_t* idx,

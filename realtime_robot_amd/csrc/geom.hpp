@@ -764,7 +764,7 @@ RTR_HD int convergence(const IcpParams& p, int iterations, const double* Tinc, d
 // then g_j (6), oldest first.  The update runs on one lane; every array it
 // touches is passed in (LDS on the device), and its loops are kept rolled, so
 // it adds neither private memory nor register pressure to the kernel.
-constexpr int kAndersonWs = 120;  // doubles of workspace of anderson_pose_step
+constexpr int kAndersonWs = 168;  // doubles of workspace of anderson_pose_step (the mix uses 0..71 and 120..167)
 
 // Rotations are charted by the vector part v of their unit quaternion with
 // positive scalar part (v = sin(th/2) * axis): matrix -> v needs only a square
@@ -805,28 +805,27 @@ RTR_HD bool qv_to_so3(const double* v, double* R) {
 // first.  ws: 72 doubles.  False when the least-squares system is singular.
 RTR_HD bool anderson_mix(const double* H, int k, double* x_new, double* ws) {
   const int n = k - 1;
-  double* A = ws;       // n x n
-  double* b = ws + 64;  // n
+  double* A = ws;        // n x n
+  double* b = ws + 64;   // n
+  double* D = ws + 120;  // n x 6: dF_j = f_(j+1) - f_j, f_j = g_j - x_j (each difference taken once)
+#pragma unroll 1
+  for (int j = 0; j < n; ++j)
+#pragma unroll 1
+    for (int i = 0; i < 6; ++i)
+      D[6 * j + i] = (H[12 * (j + 1) + 6 + i] - H[12 * (j + 1) + i]) - (H[12 * j + 6 + i] - H[12 * j + i]);
 #pragma unroll 1
   for (int r = 0; r < n; ++r) {
 #pragma unroll 1
     for (int c = r; c < n; ++c) {
       double s = 0.0;
 #pragma unroll 1
-      for (int i = 0; i < 6; ++i) {
-        const double dr = (H[12 * (r + 1) + 6 + i] - H[12 * (r + 1) + i]) - (H[12 * r + 6 + i] - H[12 * r + i]);
-        const double dc = (H[12 * (c + 1) + 6 + i] - H[12 * (c + 1) + i]) - (H[12 * c + 6 + i] - H[12 * c + i]);
-        s += dr * dc;
-      }
+      for (int i = 0; i < 6; ++i) s += D[6 * r + i] * D[6 * c + i];
       A[r * n + c] = s;
       A[c * n + r] = s;
     }
     double s = 0.0;
 #pragma unroll 1
-    for (int i = 0; i < 6; ++i) {
-      const double dr = (H[12 * (r + 1) + 6 + i] - H[12 * (r + 1) + i]) - (H[12 * r + 6 + i] - H[12 * r + i]);
-      s += dr * (H[12 * (k - 1) + 6 + i] - H[12 * (k - 1) + i]);
-    }
+    for (int i = 0; i < 6; ++i) s += D[6 * r + i] * (H[12 * (k - 1) + 6 + i] - H[12 * (k - 1) + i]);
     b[r] = s;
   }
   // Tikhonov term relative to the diagonal keeps near-collinear histories solvable

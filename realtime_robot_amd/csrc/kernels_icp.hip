@@ -325,6 +325,7 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
     sh.dred[0][tid] = t;
   }
   __syncthreads();
+  const uint64_t t_solve = e.tline ? wall_clock64() : 0ull;
   if (tid == 0) {
     sh.stage_reset = 0;
     // all FP64 work of the solve reads and writes LDS (sh.dred, sh.ws, sh.Tinc),
@@ -389,6 +390,11 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
     if (fin) S.finished = 1;
     if (fin && S.stage) S.coarse_its = S.iter;  // ended on the subsample (no correspondences)
     sh.fin = fin ? 1 : 0;
+    if (e.tline && S.iter >= 1 && S.iter <= e.prm.max_iter) {
+      uint64_t* ts = e.tline + ((size_t)p * e.prm.max_iter + (S.iter - 1)) * 2;
+      ts[0] = t_solve;
+      ts[1] = wall_clock64();
+    }
   }
   __syncthreads();
   if (gaa) {
@@ -521,7 +527,9 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         if (s0 - g.pt_lo < g.n_pts) vis.seed(s0);
         if (D < 2) walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
         walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, D >= 2 ? D - 1 : 1);
-        if (seeds && vis.key != kNoKey) seeds[chunk + o] = vis.where & 0x7fffffffu;
+        // a seed that is still the nearest neighbour is not written again
+        // (most queries of a converging problem keep theirs)
+        if (seeds && vis.key != kNoKey && (vis.where & 0x7fffffffu) != s0) seeds[chunk + o] = vis.where & 0x7fffffffu;
         // the query's correspondence (its seed slot is read above by this
         // lane only, so the slot can hold the result)
         sh.qseed[o] = key_d2(vis.key) <= r2f ? (vis.where & 0x7fffffffu) : ~0u;
@@ -665,7 +673,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
       }
       walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, k0);
-      if (seeds && vis.key != kNoKey) seeds[j] = vis.where & 0x7fffffffu;
+      if (seeds && vis.key != kNoKey && (vis.where & 0x7fffffffu) != s0) seeds[j] = vis.where & 0x7fffffffu;
       if (key_d2(vis.key) <= r2f) {
         float4 c;
         uint32_t slot = 0u;
@@ -709,15 +717,16 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   }
 }
 
-template <int MODE, bool TILE, int DEPTH>
+template <int MODE, bool TILE, int DEPTH, int WAVES>
 // Point-to-point with 4-deep gathers asks for at least 3 waves per SIMD
-// (<= 168 VGPRs, no spills): the gathers of the ring walk are latency-bound,
-// so resident waves are what hides them.  The 8-deep variant keeps more
-// gathers in flight per wave and runs 2 waves per SIMD; it serves launches
-// tuned to 2 workgroups per CU, where the third wave would not be resident
-// anyway.  Point-to-plane carries 29 FP64 sums, runs 2 waves per SIMD
-// and is always 8-deep.
-__global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH == 4) ? 3 : 2) void k_icp(DevEngine e) {
+// (<= 168 VGPRs): the gathers of the ring walk are latency-bound, so
+// resident waves are what hides them; a variant asks for 4 (<= 128 VGPRs,
+// its few spills outside the walks) for launches tuned to 4 workgroups per
+// CU.  The 8-deep variant keeps more gathers in flight per wave and runs 2
+// waves per SIMD; it serves launches tuned to 2 workgroups per CU, where the
+// third wave would not be resident anyway.  Point-to-plane carries 29 FP64
+// sums, runs 2 waves per SIMD and is always 8-deep.
+__global__ __launch_bounds__(kBlock, WAVES) void k_icp(DevEngine e) {
   constexpr int NS = MODE == RTR_MODE_POINT_TO_POINT ? SUM_P2P_COUNT : SUM_P2L_COUNT;
   using TileT = typename std::conditional<TILE, WaveTile, WaveTileNone>::type;
   __shared__ IcpShared<NS, TileT> sh;
@@ -806,22 +815,25 @@ __global__ __launch_bounds__(kBlock, (MODE == RTR_MODE_POINT_TO_POINT && DEPTH =
   if (tid == 0) atomicAdd(e.queries_searched, (unsigned long long)sh.n_search);
 }
 
-template <int MODE, bool TILE, int D>
+template <int MODE, bool TILE, int D, int W>
 static int blocks_per_cu_of() {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<MODE, TILE, D>, kBlock, 0) != hipSuccess || nb < 1) nb = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_icp<MODE, TILE, D, W>, kBlock, 0) != hipSuccess || nb < 1)
+    nb = 1;
   return nb;
 }
 
-int icp_blocks_per_cu(int mode, int use_tile, int deep) {
+int icp_blocks_per_cu(int mode, int use_tile, int variant) {
   if (mode == RTR_MODE_POINT_TO_PLANE)
-    return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_PLANE, true, 8>()
-                    : blocks_per_cu_of<RTR_MODE_POINT_TO_PLANE, false, 8>();
-  if (deep)
-    return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, true, 8>()
-                    : blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, false, 8>();
-  return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, true, 4>()
-                  : blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, false, 4>();
+    return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_PLANE, true, 8, 2>()
+                    : blocks_per_cu_of<RTR_MODE_POINT_TO_PLANE, false, 8, 2>();
+  if (variant == 1)
+    return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, true, 8, 2>()
+                    : blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, false, 8, 2>();
+  // (the LDS tiles leave room for 3 workgroups per CU: no 4-wave tile variant)
+  if (variant == 2 && !use_tile) return blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, false, 4, 4>();
+  return use_tile ? blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, true, 4, 3>()
+                  : blocks_per_cu_of<RTR_MODE_POINT_TO_POINT, false, 4, 3>();
 }
 
 void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s) {
@@ -829,19 +841,21 @@ void launch_icp(const DevEngine& e, int grid_blocks, hipStream_t s) {
   const dim3 g(grid_blocks), b(kBlock);
   if (e.prm.mode == RTR_MODE_POINT_TO_PLANE) {
     if (e.use_tile)
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, true, 8>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, true, 8, 2>), g, b, 0, s, e);
     else
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, false, 8>), g, b, 0, s, e);
-  } else if (e.deep) {
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_PLANE, false, 8, 2>), g, b, 0, s, e);
+  } else if (e.variant == 1) {
     if (e.use_tile)
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true, 8>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true, 8, 2>), g, b, 0, s, e);
     else
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false, 8>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false, 8, 2>), g, b, 0, s, e);
+  } else if (e.variant == 2 && !e.use_tile) {
+    hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false, 4, 4>), g, b, 0, s, e);
   } else {
     if (e.use_tile)
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true, 4>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, true, 4, 3>), g, b, 0, s, e);
     else
-      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false, 4>), g, b, 0, s, e);
+      hipLaunchKernelGGL((k_icp<RTR_MODE_POINT_TO_POINT, false, 4, 3>), g, b, 0, s, e);
   }
 }
 

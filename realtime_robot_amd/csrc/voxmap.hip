@@ -10,13 +10,17 @@
 // Offsets from the corner keep the FP32 sums small, so the centroid keeps
 // full float precision however many frames land in the voxel.
 //
-// Integration kernel: each workgroup takes tiles of 512 consecutive points
-// of one frame, stages them in LDS with coalesced loads, and merges the
-// tile's points voxel by voxel in an LDS hash table first (neighbouring
-// pixels of a depth frame share voxels), so the global table sees one insert
-// and one set of atomics per distinct voxel of the tile, not per point.
-// Global probes are bounded (the table holds at most half its slots).  Many
-// frames (each with its own transform) integrate in one launch.
+// Integration kernel: each workgroup takes a contiguous run of tiles of 512
+// consecutive points (a few image rows of a depth frame), stages each tile
+// in LDS with coalesced loads, and merges the points voxel by voxel in an LDS
+// hash table that lives across the run's tiles (neighbouring pixels, and
+// neighbouring rows, share voxels); the table is flushed into the global one
+// when it holds more than a quarter of its slots ahead of the next tile (so
+// LDS probes stay short), and at the end of the run.  The global table then
+// sees one insert and one set of atomics per distinct voxel of each flush,
+// not per point.  Global probes are bounded (the table holds at most half its
+// slots).  Many frames (each with its own transform) integrate in one
+// launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,9 +74,11 @@ RTR_HD bool voxel_key(float x, float y, float z, float inv_h, uint64_t* key, int
 
 constexpr int kMapBlock = 256;
 constexpr int kMapTile = 512;    // points per workgroup tile
-constexpr int kMapSlots = 1024;  // the tile's LDS hash table (at most half full)
+constexpr int kMapSlots = 2048;  // the run's LDS hash table
+constexpr int kMapFlushAt = kMapSlots / 2 - kMapTile;  // flush before a tile when more keys are held
+constexpr int kMapMinRun = 4;    // tiles per workgroup at least (the merging spans the run)
 constexpr int kMaxProbe = 128;   // global probe bound (the table is at most half full)
-static_assert(kMapTile < kMapSlots, "every tile key must find a free LDS slot");
+static_assert(kMapFlushAt + kMapTile < kMapSlots, "every tile key must find a free LDS slot");
 
 // One frame of an integration launch: packed xyz on the device, its point
 // count, its first tile in the launch and the float rows of its transform.
@@ -113,15 +119,47 @@ __device__ __forceinline__ bool global_add(const MapDev& m, uint64_t key, float 
   return false;
 }
 
+// The run's merged voxels into the global table; the LDS table is left
+// empty.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void map_flush(const MapDev& m, unsigned long long* s_key, float4* s_acc,
+                                          uint32_t& dropped, uint32_t& overflow) {
+  for (int i = threadIdx.x; i < kMapSlots; i += kMapBlock) {
+    const unsigned long long k = s_key[i];
+    if (k == kEmpty) continue;
+    const float4 a = s_acc[i];
+    const uint32_t c = __float_as_uint(a.w);
+    if (!global_add(m, k, a.x, a.y, a.z, c)) {
+      dropped += c;
+      overflow = 1;
+    }
+    s_key[i] = kEmpty;
+    s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
 __global__ __launch_bounds__(kMapBlock) void k_map_integrate(MapDev m, const MapFrame* frames, int n_frames,
-                                                             uint32_t n_tiles) {
+                                                             uint32_t n_tiles, uint32_t run) {
   __shared__ float s_xyz[3 * kMapTile];
   __shared__ unsigned long long s_key[kMapSlots];
   __shared__ float4 s_acc[kMapSlots];
   __shared__ int s_frame;
+  __shared__ uint32_t s_used;
   const int tid = threadIdx.x;
   uint32_t dropped = 0, overflow = 0;
-  for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  for (int i = tid; i < kMapSlots; i += kMapBlock) {
+    s_key[i] = kEmpty;
+    s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (tid == 0) s_used = 0u;
+  const uint32_t t0 = blockIdx.x * run;
+  const uint32_t t1 = min(t0 + run, n_tiles);
+  for (uint32_t tile = t0; tile < t1; ++tile) {
+    __syncthreads();  // the previous tile's inserts (and s_used) are complete
+    if (s_used > (uint32_t)kMapFlushAt) {
+      map_flush(m, s_key, s_acc, dropped, overflow);
+      __syncthreads();
+      if (tid == 0) s_used = 0u;
+    }
     if (tid == 0) {
       // the frame of this tile: the last one whose first tile is <= tile
       int lo = 0, hi = n_frames - 1;
@@ -133,10 +171,6 @@ __global__ __launch_bounds__(kMapBlock) void k_map_integrate(MapDev m, const Map
           hi = mid - 1;
       }
       s_frame = lo;
-    }
-    for (int i = tid; i < kMapSlots; i += kMapBlock) {
-      s_key[i] = kEmpty;
-      s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();
     const MapFrame* f = frames + s_frame;
@@ -162,14 +196,17 @@ __global__ __launch_bounds__(kMapBlock) void k_map_integrate(MapDev m, const Map
       const float dx = px - (float)k3[0] * m.h;
       const float dy = py - (float)k3[1] * m.h;
       const float dz = pz - (float)k3[2] * m.h;
-      // at most kMapTile distinct keys per tile: a free or matching slot
-      // always exists within kMapSlots probes
+      // the table holds at most kMapFlushAt + kMapTile < kMapSlots keys: a
+      // free or matching slot always exists within kMapSlots probes
       uint32_t sl = (uint32_t)(mix64(key) & (uint64_t)(kMapSlots - 1));
       for (int probe = 0; probe < kMapSlots; ++probe) {
         unsigned long long cur = s_key[sl];
         if (cur == kEmpty) {
           cur = atomicCAS(&s_key[sl], (unsigned long long)kEmpty, (unsigned long long)key);
-          if (cur == kEmpty) cur = key;
+          if (cur == kEmpty) {
+            atomicAdd(&s_used, 1u);
+            cur = key;
+          }
         }
         if (cur == key) {
           atomicAdd(&s_acc[sl].x, dx);
@@ -181,19 +218,9 @@ __global__ __launch_bounds__(kMapBlock) void k_map_integrate(MapDev m, const Map
         sl = (sl + 1) & (uint32_t)(kMapSlots - 1);
       }
     }
-    __syncthreads();
-    for (int i = tid; i < kMapSlots; i += kMapBlock) {
-      const unsigned long long k = s_key[i];
-      if (k == kEmpty) continue;
-      const float4 a = s_acc[i];
-      const uint32_t c = __float_as_uint(a.w);
-      if (!global_add(m, k, a.x, a.y, a.z, c)) {
-        dropped += c;
-        overflow = 1;
-      }
-    }
-    __syncthreads();  // the next tile rewrites the LDS table
   }
+  __syncthreads();
+  map_flush(m, s_key, s_acc, dropped, overflow);
   if (dropped) atomicAdd(&m.counters[1], dropped);
   if (overflow) atomicOr(&m.counters[2], 1u);
 }
@@ -331,10 +358,13 @@ struct VoxelMap {
       tile0 += (uint32_t)((n[f] + kMapTile - 1) / kMapTile);
     }
     RTR_HIP(hipMemcpyAsync(T.p, fr.data(), sizeof(MapFrame) * (size_t)nf, hipMemcpyHostToDevice, stream));
-    const unsigned blocks = (unsigned)std::min<size_t>(tiles, (size_t)std::max(cu_count, 1) * 16);
+    // contiguous runs of at least kMapMinRun tiles, at most 8 workgroups per CU
+    size_t blocks = std::min<size_t>((tiles + kMapMinRun - 1) / kMapMinRun, (size_t)std::max(cu_count, 1) * 8);
+    const uint32_t run = (uint32_t)((tiles + blocks - 1) / blocks);
+    blocks = (tiles + run - 1) / run;
     if (profiling) RTR_HIP(hipEventRecord(ev[0], stream));
-    hipLaunchKernelGGL(k_map_integrate, dim3(blocks), dim3(kMapBlock), 0, stream, dev, T.as<MapFrame>(), nf,
-                       (uint32_t)tiles);
+    hipLaunchKernelGGL(k_map_integrate, dim3((unsigned)blocks), dim3(kMapBlock), 0, stream, dev, T.as<MapFrame>(), nf,
+                       (uint32_t)tiles, run);
     RTR_HIP(hipGetLastError());
     if (profiling) RTR_HIP(hipEventRecord(ev[1], stream));
     uint32_t after[4];

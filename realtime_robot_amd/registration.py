@@ -326,6 +326,18 @@ class BatchRegistration:
         L.check(L.lib.rtr_batch_get_history(self._h, int(prob), L.ptr(buf), len(buf), C.byref(n)))
         return buf[:n.value].reshape(-1, 3, 4).copy()
 
+    def set_timeline(self, on: bool):
+        """Record each problem's solve start/end wall-clock stamps per iteration (diagnostic)."""
+        L.check(L.lib.rtr_batch_set_timeline(self._h, int(on)))
+
+    def timeline(self, prob: int, max_iterations: int = 256):
+        """(iterations, 2) solve start/end times of problem `prob` in microseconds (device wall clock)."""
+        buf = np.empty(2 * max_iterations, np.uint64)
+        n = C.c_int()
+        tick = C.c_double()
+        L.check(L.lib.rtr_batch_get_timeline(self._h, int(prob), L.ptr(buf), len(buf), C.byref(n), C.byref(tick)))
+        return buf[:n.value].reshape(-1, 2).astype(np.float64) * tick.value * 1e-3
+
     def last_correspondences(self, prob: int):
         """(idx int32, d2 float32) per source point: the correspondences the
         last iteration of problem `prob` used (-1 / inf: none within range)."""
