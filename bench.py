@@ -162,10 +162,11 @@ def cpu_baseline(srcs, tgts, starts, n_starts, budget_s, T64_gt, threads):
                       f"latency: config-1 chair pair on 1 thread, median of 5 after 1 warm-up"}
 
 
-def tuning6(cfg):
-    """A tuning as 6 rounded values (a 5-value tuning has no early exit)."""
+def tuning_key(cfg):
+    """A tuning as 7 rounded values (a shorter one has no early exit and no
+    fine items)."""
     v = [round(float(x), 4) for x in cfg]
-    return v + [0.0] * (6 - len(v))
+    return v + [0.0] * (7 - len(v))
 
 
 def prefer_profiled(reg, best, best_ms, table, name, tol=0.03):
@@ -177,11 +178,11 @@ def prefer_profiled(reg, best, best_ms, table, name, tol=0.03):
     if not os.path.exists(prof):
         return best, best_ms
     with open(prof) as f:
-        want = tuning6(json.load(f)["tuning"])
-    if tuning6(best) == want:
+        want = tuning_key(json.load(f)["tuning"])
+    if tuning_key(best) == want:
         return best, best_ms
     for cfg, ms in table:
-        if tuning6(cfg) == want and ms <= best_ms * (1 + tol):
+        if tuning_key(cfg) == want and ms <= best_ms * (1 + tol):
             autotune.apply(reg, cfg)
             log(f"autotune: {cfg} at {ms:.1f} ms ties the fastest {best} ({best_ms:.1f} ms); keeping the profiled one")
             return tuple(cfg), ms
@@ -324,7 +325,7 @@ def main():
     tuned = None
     if args.tuning:
         f = [float(v) for v in args.tuning.split(",")]
-        tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4])) + ((f[5],) if len(f) > 5 else ())
+        tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4])) + tuple(f[5:6]) + tuple(int(v) for v in f[6:7])
     elif not args.no_autotune and ms is not None:
         tuned, tuned_ms, table = autotune.tune(ms.reg, ms.guesses, candidates=autotune.LATENCY_CANDIDATES, key=None,
                                                use_cache=False)
@@ -449,7 +450,7 @@ def main():
             return None, None
         with open(prof) as f:
             pj = json.load(f)
-        same = tuning6(pj["tuning"]) == tuning6(tuning)
+        same = tuning_key(pj["tuning"]) == tuning_key(tuning)
         same = same and pj.get("refine_anderson_memory", 0) == (args.refine_anderson if "refine" in name else 0)
         same = same and pj.get("refine_coarse", [0, 0]) == (
             [args.refine_coarse_stride, args.refine_coarse_iterations] if "refine" in name else [0, 0])

@@ -190,6 +190,14 @@ int rtr_batch_set_tuning(rtr_batch* b, float cell_factor, float cells_per_point,
  * workgroups raise throughput while most problems iterate; once few remain,
  * idle workgroups polling the work queue stretch their iterations. */
 int rtr_batch_set_retire(rtr_batch* b, float below);
+/* Work-item granularity of the late iterations: from iteration `from` on
+ * (counted from 0 within one align), each problem's iteration is split into
+ * 256-point work items instead of item_pts-point ones, so the few problems
+ * still iterating at the end of a batch spread over more workgroups
+ * (0 = never, the default).  Changes the summation grouping of the moment
+ * sums (FP64), not the correspondences. */
+int rtr_batch_set_fine_items(rtr_batch* b, int from);
+int rtr_batch_get_fine_items(rtr_batch* b, int* from);
 int rtr_batch_get_retire(rtr_batch* b, float* below);
 /* The current tuning (any pointer may be null). */
 int rtr_batch_get_tuning(rtr_batch* b, float* cell_factor, float* cells_per_point, int* item_pts,

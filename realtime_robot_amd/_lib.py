@@ -110,6 +110,8 @@ _SIGS = {
     "rtr_batch_set_tuning": (_i, [_vp, _f, _f, _i, _i, _i]),
     "rtr_batch_get_tuning": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rtr_batch_set_retire": (_i, [_vp, _f]),
+    "rtr_batch_set_fine_items": (_i, [_vp, _i]),
+    "rtr_batch_get_fine_items": (_i, [_vp, _vp]),
     "rtr_batch_get_retire": (_i, [_vp, _vp]),
     "rtr_batch_set_graph": (_i, [_vp, _i]),
     "rtr_batch_set_rebuild": (_i, [_vp, _i]),

@@ -77,6 +77,13 @@ LATENCY_CANDIDATES = CANDIDATES + [
     (0.25, 2.0, 512, 4, 0, 0.5),
     (0.25, 2.0, 512, 0, 1, 0.5),
     (0.25, 2.0, 256, 2, 1, 0.5),
+    # 256-point items for the late iterations only: early ones keep large
+    # items (fewer partial sums and claims), the long tail spreads wider
+    (0.25, 2.0, 1024, 0, 0, 0.5, 6),
+    (0.25, 2.0, 1024, 0, 0, 0.5, 10),
+    (0.25, 2.0, 1024, 0, 0, 0.5, 16),
+    (0.25, 2.0, 512, 0, 0, 0.5, 10),
+    (0.25, 2.0, 1024, 4, 0, 0.5, 10),
 ]
 
 
@@ -113,7 +120,7 @@ def save_cache(cache: dict) -> None:
 
 def apply(batch, cfg) -> None:
     cf, cpp, item, bpc, tile = cfg[:5]
-    batch.set_tuning(cf, cpp, item, bpc, tile, cfg[5] if len(cfg) > 5 else 0.0)
+    batch.set_tuning(cf, cpp, item, bpc, tile, cfg[5] if len(cfg) > 5 else 0.0, int(cfg[6]) if len(cfg) > 6 else 0)
 
 
 def tune(batch, guesses=None, candidates=None, reps: int = 2, key: str | None = None, use_cache: bool = True,

@@ -408,6 +408,23 @@ int rtr_batch_get_tuning(rtr_batch* b, float* cell_factor, float* cells_per_poin
   return RTR_OK;
 }
 
+int rtr_batch_set_fine_items(rtr_batch* b, int from_iteration) {
+  RTR_GUARD_BEGIN
+  if (!b || from_iteration < 0 || from_iteration > 100000) {
+    set_error("rtr_batch_set_fine_items: the iteration must lie in [0, 100000] (got %d)", from_iteration);
+    return RTR_ERR_INVALID_ARG;
+  }
+  b->eng.tune().fine_from = from_iteration;
+  return b->eng.set_params(b->eng.params());
+  RTR_GUARD_END
+}
+
+int rtr_batch_get_fine_items(rtr_batch* b, int* from_iteration) {
+  if (!b || !from_iteration) return RTR_ERR_INVALID_ARG;
+  *from_iteration = b->eng.tune().fine_from;
+  return RTR_OK;
+}
+
 int rtr_batch_set_retire(rtr_batch* b, float below) {
   RTR_GUARD_BEGIN
   if (!b || !(below >= 0.f && below <= 1.f)) {

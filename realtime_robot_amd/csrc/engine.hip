@@ -327,6 +327,25 @@ int Engine::layout_problems() {
     }
   }
   xcd_off_[kXcds] = (uint32_t)items_.size();
+  // fine items for the late passes (the same queues, kFinePts points each):
+  // the problems still iterating then are few, and smaller items spread each
+  // iteration over more workgroups
+  for (Problem& pr : probs_) {
+    pr.fitem_off = (uint32_t)items_.size();
+    pr.n_fitems = 0;
+  }
+  for (int x = 0; x < kXcds; ++x) {
+    fxcd_off_[x] = (uint32_t)items_.size();
+    for (size_t p = 0; p < probs_.size() && tune_.fine_from > 0; ++p) {
+      Problem& pr = probs_[p];
+      if (txcd[pr.tgt] != x) continue;
+      pr.fitem_off = (uint32_t)items_.size();
+      const uint32_t n = srcs_[pr.src].n;
+      for (uint32_t b = 0; b < n; b += (uint32_t)kFinePts) items_.push_back(make_uint2((uint32_t)p, b));
+      pr.n_fitems = (n + kFinePts - 1) / kFinePts;
+    }
+  }
+  fxcd_off_[kXcds] = (uint32_t)items_.size();
   const size_t np = probs_.size();
   RTR_TRY(prob_desc_.ensure(np * sizeof(Problem)));
   RTR_TRY(state_.ensure(np * sizeof(ProblemState)));
@@ -417,6 +436,8 @@ DevEngine Engine::dev_view() const {
   e.claim = ctr + 96;
   e.claim_stride = claim_stride();
   for (int x = 0; x <= kXcds; ++x) e.xcd_off[x] = xcd_off_[x];
+  for (int x = 0; x <= kXcds; ++x) e.fxcd_off[x] = fxcd_off_[x];
+  e.fine_from = tune_.fine_from > 0 ? tune_.fine_from : 0x7fffffff;
   e.done = done_.as<uint32_t>();
   e.scan_tmp = scan_tmp_.as<uint32_t>();
   e.bbox_tmp = bbox_tmp_.as<uint32_t>();

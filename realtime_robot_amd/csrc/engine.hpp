@@ -45,6 +45,7 @@ struct TuneParams {
   int blocks_per_cu = 0;          // persistent workgroups per CU (0 = occupancy limit)
   int use_tile = 0;               // LDS candidate tiles on/off
   float retire = 0.f;             // surplus workgroups leave below this fraction of unfinished problems
+  int fine_from = 0;              // passes from this one on run 256-point work items (0: never)
 };
 
 class Engine {
@@ -132,6 +133,7 @@ class Engine {
   uint32_t n_cells_ = 0, n_bins_ = 0;
   int max_cells_ = 0;
   uint32_t xcd_off_[kXcds + 1] = {0};
+  uint32_t fxcd_off_[kXcds + 1] = {0};
   // [0] problems done, [32] error, [64..65] searches (a 64-bit count),
   // [96 + (pass * kXcds + x) * stride] claim counters
   int claim_stride() const { return prm_.max_iterations <= 256 ? 32 : 1; }

@@ -64,6 +64,8 @@ struct DevEngine {
   uint32_t* claim;           // claim counter per (pass, XCD queue), claim_stride words apart
   int claim_stride;          // 32 (one 128-byte line per counter) unless the iteration limit is huge
   uint32_t xcd_off[kXcds + 1];  // item range of each XCD queue
+  uint32_t fxcd_off[kXcds + 1]; // fine item range of each XCD queue
+  int fine_from;             // passes from this one on run the fine items (INT_MAX: never)
   uint32_t* done;            // per problem: items finished so far
   uint32_t* problems_done;
   int* error;                // nonzero: a bounded wait expired

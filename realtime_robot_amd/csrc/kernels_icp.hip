@@ -62,6 +62,19 @@ __device__ __forceinline__ GridView grid_of(const DevEngine& e, const TargetDesc
   return g;
 }
 
+// Work items of problem pr in a pass: the coarse ones (item_pts points each)
+// before e.fine_from, the fine ones (kFinePts) from then on.  The items a
+// problem completes up to and including pass k (its done count then).
+__device__ __forceinline__ bool fine_pass(const DevEngine& e, int pass) { return pass >= e.fine_from; }
+__device__ __forceinline__ uint32_t items_of(const Problem& pr, bool fine) { return fine ? pr.n_fitems : pr.n_items; }
+__device__ __forceinline__ uint32_t first_item_of(const Problem& pr, bool fine) {
+  return fine ? pr.fitem_off : pr.item_off;
+}
+__device__ __forceinline__ uint32_t done_through(const DevEngine& e, const Problem& pr, int pass) {
+  if (pass < e.fine_from) return (uint32_t)(pass + 1) * pr.n_items;
+  return (uint32_t)e.fine_from * pr.n_items + (uint32_t)(pass + 1 - e.fine_from) * pr.n_fitems;
+}
+
 __global__ void k_icp_init(DevEngine e) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= e.n_prob) return;
@@ -312,9 +325,11 @@ __device__ void solve_problem(const DevEngine& e, uint32_t p, const Problem& pr,
   if (tid < 16) sh.Tq[tid] = e.st[p].T[tid];
   if (gaa && tid < kAndersonStride) sh.aa[tid] = gaa[tid];
   double acc = 0.0;
+  const bool fine = fine_pass(e, pass);
+  const uint32_t i0 = first_item_of(pr, fine), ni = items_of(pr, fine);
   if (v < NS) {
-    for (uint32_t i = grp; i < pr.n_items; i += kBlock / NSUM_MAX) {
-      acc += e.partials[(size_t)(pr.item_off + i) * NSUM_MAX + v];
+    for (uint32_t i = grp; i < ni; i += kBlock / NSUM_MAX) {
+      acc += e.partials[(size_t)(i0 + i) * NSUM_MAX + v];
     }
   }
   sh.dred[grp][v] = acc;
@@ -457,7 +472,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   const double ay = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[1])));
   const double az = (double)__builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_int(sh.anchor[2])));
   uint32_t n_search = 0u;
-  const uint32_t end = min(itm.y + (uint32_t)e.item_pts, sd.n);
+  const uint32_t end = min(itm.y + (fine_pass(e, pass) ? (uint32_t)kFinePts : (uint32_t)e.item_pts), sd.n);
   constexpr bool kCompact = !TILE && DEPTH == 4;
   // Search seeds (global walks only): every searched query stores its
   // nearest neighbour's slot, and the next iteration starts from it.  The
@@ -708,7 +723,7 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
   __syncthreads();
   if (tid == 0) {
     const uint32_t prev = atomicAdd(&e.done[p], 1u);
-    sh.last = (prev + 1u == (uint32_t)(pass + 1) * pr.n_items);
+    sh.last = (prev + 1u == done_through(e, pr, pass));
   }
   __syncthreads();
   if (sh.last) {
@@ -750,20 +765,22 @@ __global__ __launch_bounds__(kBlock, WAVES) void k_icp(DevEngine e) {
         // own XCD's queue first (L2 locality), then steal from the others
         int it = e.n_items;
         const int x0 = (int)(blockIdx.x % kXcds);
+        const bool fine = fine_pass(e, pass);
+        const uint32_t* qoff = fine ? e.fxcd_off : e.xcd_off;
         for (int t = 0; t < kXcds; ++t) {
           const int x = (x0 + t) % kXcds;
-          const uint32_t cnt = e.xcd_off[x + 1] - e.xcd_off[x];
+          const uint32_t cnt = qoff[x + 1] - qoff[x];
           uint32_t* ctr = &e.claim[(size_t)(pass * kXcds + x) * e.claim_stride];
           if (cnt == 0u || __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
           const uint32_t c = atomicAdd(ctr, 1u);
           if (c < cnt) {
-            it = (int)(e.xcd_off[x] + c);
+            it = (int)(qoff[x] + c);
             const uint32_t p = e.items[it].x;
             if (load_acquire((const uint32_t*)&e.st[p].gen) & kFinishedBit) {
               // a finished problem: its remaining items of this pass (contiguous
               // in this queue) are skipped in one step, and the claim goes on
               const Problem& pr = e.probs[p];
-              atomicMax(ctr, pr.item_off + pr.n_items - e.xcd_off[x]);
+              atomicMax(ctr, first_item_of(pr, fine) + items_of(pr, fine) - qoff[x]);
               it = e.n_items;
               t = -1;  // rescan the queues from the own XCD's
               continue;

@@ -79,10 +79,13 @@ struct SourceDesc {
 // One registration = (source, target, initial guess).
 struct Problem {
   uint32_t src, tgt;
-  uint32_t item_off, n_items;
-  uint32_t pt_off;       // prefix sum of source sizes over problems (per-point outputs)
+  uint32_t item_off, n_items;    // its work items (item_pts source points each)
+  uint32_t pt_off;               // prefix sum of source sizes over problems (per-point outputs)
+  uint32_t fitem_off, n_fitems;  // its fine work items (kFinePts each) for the passes from fine_from on
   uint32_t pad_;
 };
+
+constexpr int kFinePts = 256;  // source points per fine work item
 
 struct ProblemState {
   double T[16];      // current estimate, column-major

@@ -276,20 +276,24 @@ class BatchRegistration:
     def build(self):
         L.check(L.lib.rtr_batch_build(self._h))
 
-    def set_tuning(self, cell_factor=0.0, cells_per_point=0.0, item_pts=0, blocks_per_cu=-1, use_tile=-1, retire=-1.0):
+    def set_tuning(self, cell_factor=0.0, cells_per_point=0.0, item_pts=0, blocks_per_cu=-1, use_tile=-1, retire=-1.0,
+                   fine_from=-1):
         L.check(L.lib.rtr_batch_set_tuning(self._h, float(cell_factor), float(cells_per_point), int(item_pts),
                                            int(blocks_per_cu), int(use_tile)))
         if retire >= 0:
             L.check(L.lib.rtr_batch_set_retire(self._h, float(retire)))
+        if fine_from >= 0:
+            L.check(L.lib.rtr_batch_set_fine_items(self._h, int(fine_from)))
 
     def tuning(self):
-        """(cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile, retire) in effect."""
+        """(cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile, retire, fine_from) in effect."""
         cf, cp, rt = C.c_float(), C.c_float(), C.c_float()
-        ip, bp, ut = C.c_int(), C.c_int(), C.c_int()
+        ip, bp, ut, ff = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         L.check(L.lib.rtr_batch_get_tuning(self._h, C.byref(cf), C.byref(cp), C.byref(ip), C.byref(bp),
                                            C.byref(ut)))
         L.check(L.lib.rtr_batch_get_retire(self._h, C.byref(rt)))
-        return (cf.value, cp.value, ip.value, bp.value, ut.value, rt.value)
+        L.check(L.lib.rtr_batch_get_fine_items(self._h, C.byref(ff)))
+        return (cf.value, cp.value, ip.value, bp.value, ut.value, rt.value, ff.value)
 
     def set_profiling(self, on: bool):
         L.check(L.lib.rtr_batch_set_profiling(self._h, int(on)))

@@ -28,11 +28,12 @@ TOL_R = 1e-5
 TOL_T = 1e-5
 N_PAIRS = 16
 
-# (cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile, retire)
+# (cell_factor, cells_per_point, item_pts, blocks_per_cu, use_tile, retire, fine_from)
 TUNINGS = {
-    "refine": (0.25, 2.0, 1024, 0, 0, 0.5),    # the bench's refine launch: compacted walk
-    "deep": (0.35, 2.0, 512, 2, 0, 0.0),       # 8-deep gathers, 2 workgroups per CU
-    "tiles": (0.25, 2.0, 256, 0, 1, 0.0),      # LDS candidate tiles, small items
+    "refine": (0.25, 2.0, 1024, 0, 0, 0.5, 0),    # compacted walk, 1024-point items
+    "fine": (0.25, 2.0, 1024, 0, 0, 0.5, 4),      # the same, 256-point items from iteration 4 on
+    "deep": (0.35, 2.0, 512, 2, 0, 0.0, 0),       # 8-deep gathers, 2 workgroups per CU
+    "tiles": (0.25, 2.0, 256, 0, 1, 0.0, 0),      # LDS candidate tiles, small items
 }
 
 
@@ -59,10 +60,14 @@ def test_plain_icp_from_proposals_matches_fp64(workload, name):
     reg.set_params(params)
     reg.set_clouds(srcs, tgts)
     reg.set_problems(range(N_PAIRS), range(N_PAIRS))
-    cf, cpp, ip, bpc, tile, retire = TUNINGS[name]
+    cf, cpp, ip, bpc, tile, retire, fine = TUNINGS[name]
     reg.set_tuning(cell_factor=cf, cells_per_point=cpp, item_pts=ip, blocks_per_cu=bpc, use_tile=tile,
-                   retire=retire)
+                   retire=retire, fine_from=fine)
+    assert reg.tuning()[6] == fine
     res = reg.align(props)
+    again = reg.align(props)
+    for i in range(N_PAIRS):
+        assert np.array_equal(res[i]["T"], again[i]["T"]) and res[i]["iterations"] == again[i]["iterations"]
     worst = []
     for i in range(N_PAIRS):
         er, et = rot_err(res[i]["T"], ref[i]["T"]), trans_err(res[i]["T"], ref[i]["T"])
