@@ -452,15 +452,30 @@ __device__ __forceinline__ void wave_totals(const double* acc, Sh& sh, int w, in
   }
 }
 
+// A copy of a wave-uniform value in scalar registers: loads of descriptors
+// through a uniform address come back in vector registers, and the pointers
+// derived from them would otherwise hold vector registers through the walk.
+template <class T>
+__device__ __forceinline__ T scalar_copy(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  T out;
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(&v);
+  uint32_t* d = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+  return out;
+}
+
 template <int MODE, int NS, bool TILE, class TileT, int DEPTH>
 __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS, TileT>& sh) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   TileT& tl = sh.tile[w];
-  const uint2 itm = e.items[it];
+  it = __builtin_amdgcn_readfirstlane(it);
+  const uint2 itm = scalar_copy(e.items[it]);
   const uint32_t p = itm.x;
-  const Problem pr = e.probs[p];
-  const SourceDesc sd = e.srcs[pr.src];
-  const TargetDesc td = e.tgts[pr.tgt];
+  const Problem pr = scalar_copy(e.probs[p]);
+  const SourceDesc sd = scalar_copy(e.srcs[pr.src]);
+  const TargetDesc td = scalar_copy(e.tgts[pr.tgt]);
   const GridView g = grid_of(e, td);
   const float r2f = e.prm.r2f;
   // the transform rows and the anchor are the same for every lane: keep them

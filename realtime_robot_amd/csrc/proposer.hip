@@ -19,6 +19,7 @@
 
 #include "../../include/rtr.h"
 #include "engine.hpp"
+#include "wave.hpp"
 
 namespace rtr {
 
@@ -327,13 +328,53 @@ __global__ __launch_bounds__(kFThreads) void k_mlp_pool(const uint16_t* x0, int 
 // One workgroup per cloud.  The cloud is first subsampled with a fixed stride
 // to at most kFpsMax points; its centroid is removed; then kP points are
 // chosen greedily, starting at subsample index 0, each time the point with
-// the largest distance to the chosen set (lowest index on ties).  Each
-// thread keeps its kFpsPer subsample points (i = tid + k * kFpsThreads) and
-// their distances in registers; the chosen point is broadcast through LDS,
-// and the chosen points are kept in LDS and written out once at the end.
+// the largest distance to the chosen set (lowest index on ties).
+//
+// The points are regrouped spatially before the selection: a counting sort
+// by 12-bit Morton cell (16^3 cells over the subsample's bounding box) puts
+// them in cell order, and each run of 64 consecutive points is a group,
+// compact in space, held in one register slot of one wave's 64 lanes.  A group is updated
+// with the newly chosen point q only if q can lower one of its distances:
+// with b the group's bounding box and M its largest distance, the update is
+// skipped when the rounded squared box distance from q is >= M.  Rounding is
+// monotone, so that bound is <= the rounded squared distance from q of every
+// point in the box: a skipped update would have left every distance as it
+// was, and the selections are exactly those of the plain O(kP * m) loop.
+// Late in the selection few groups are near q, so most of the distance work
+// is skipped.  Each group keeps its largest (distance, lowest index) as a
+// 64-bit key in lane k; the wave's best key and its coordinates go through
+// LDS to the other waves (one barrier per selection).  Where the sort puts a
+// point decides only which work is skipped, never a selection.
 constexpr int kFpsThreads = 512;
-constexpr int kFpsPer = kFpsMax / kFpsThreads;
+constexpr int kFpsPer = kFpsMax / kFpsThreads;   // register slots (groups) per lane
 constexpr int kFpsWaves = kFpsThreads / 64;
+constexpr int kFpsCells = 4096;                  // 16^3 Morton cells
+static_assert(kFpsPer == 16 && kFpsWaves == 8 && kFpsMax == 8192, "the group layout assumes 8 waves x 16 slots");
+static_assert(kFpsCells * 4 == kFpsMax * 2, "the inverse order (u16) fills the histogram's space");
+
+// A group's distances lowered by q (the group is one register slot of all
+// 64 lanes); returns the group's largest key (0: no point), the same in every
+// lane.  key = (distance bits + 1) << 32 | (8191 - index) << 16 | slot.
+// The point's subsample index comes from the inverse order in LDS (inv[j],
+// j its sorted position), read only when the group is updated.
+__device__ __forceinline__ uint64_t fps_group_update(float px, float py, float pz, float& dist, const uint16_t* inv,
+                                                     int j, uint32_t slot, float qx, float qy, float qz) {
+  // an empty asm keeps the compiler from computing every slot's distances
+  // (and reading every index) ahead of the branch that selects the slot:
+  // they are pure and would be hoisted or speculated
+  asm volatile("" : "+v"(qx), "+v"(qy), "+v"(qz), "+v"(j));
+  const uint32_t klo = ((uint32_t)(kFpsMax - 1 - (int)inv[j]) << 16) | slot;
+  const float dx = px - qx, dy = py - qy, dz = pz - qz;
+  float d = dx * dx;
+  d = d + dy * dy;
+  d = d + dz * dz;
+  dist = fminf(dist, d);  // padding slots hold -1 and keep it
+  const uint64_t key = dist >= 0.f ? ((uint64_t)(__float_as_uint(dist) + 1u) << 32) | klo : 0ull;
+  return wave_max_u64(key);
+}
+
+// cell coordinate in [0, 15] of t = (v - lo) * 16 / extent (NaN -> 0)
+__device__ __forceinline__ uint32_t fps_cell(float t) { return t >= 15.f ? 15u : (t >= 1.f ? (uint32_t)t : 0u); }
 
 __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const uint32_t* off, const uint32_t* cnt,
                                                      float* pts_out /* cloud * kP * 3 */,
@@ -344,49 +385,99 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
   const float* p = raw + 3 * (size_t)off[c];
   const int m = (int)min((uint32_t)kFpsMax, n);
   __shared__ float s_q[3];
-  // per wave: its farthest point (distance, index, coordinates); double
-  // buffered by iteration parity, so one barrier per iteration suffices
-  __shared__ float red_v[2][kFpsWaves];
-  __shared__ int red_i[2][kFpsWaves];
+  // per wave: its best key and that point's coordinates; double buffered by
+  // selection parity, so one barrier per selection suffices
+  __shared__ unsigned long long red_k[2][kFpsWaves];
   __shared__ float red_p[2][kFpsWaves][3];
   __shared__ double cen[kFpsWaves][3];
-  __shared__ float s_sel[kP][3];  // chosen points, centroid removed
-  __shared__ int s_cur[kP];       // their subsample indices
-  float px[kFpsPer], py[kFpsPer], pz[kFpsPer], dist[kFpsPer];
-  double ax = 0, ay = 0, az = 0;
+  __shared__ float s_box[kFpsWaves][6];
+  __shared__ uint32_t s_wsum[kFpsWaves];
+  __shared__ uint32_t s_hist[kFpsCells];  // the cell histogram, then the inverse order
+  __shared__ float s_sel[kP][3];    // chosen points, centroid removed
+  __shared__ int s_cur[kP];         // their subsample indices
+  uint32_t* hist = s_hist;
+  uint16_t* inv = reinterpret_cast<uint16_t*>(s_hist);  // sorted position -> subsample index
+
+  // ---- subsample i = tid + k * kFpsThreads: centroid, bounding box, cells
+  uint32_t pos[kFpsPer];
+  {
+    float px[kFpsPer], py[kFpsPer], pz[kFpsPer];
+    double ax = 0, ay = 0, az = 0;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
-  for (int k = 0; k < kFpsPer; ++k) {
-    const int i = tid + k * kFpsThreads;
-    px[k] = py[k] = pz[k] = 0.f;
-    dist[k] = -1.f;
-    if (i < m) {
-      const size_t src = (size_t)((uint64_t)i * n / (uint64_t)m);
-      px[k] = p[3 * src];
-      py[k] = p[3 * src + 1];
-      pz[k] = p[3 * src + 2];
-      dist[k] = 3.0e38f;
-      ax += px[k];
-      ay += py[k];
-      az += pz[k];
+    for (int k = 0; k < kFpsPer; ++k) {
+      const int i = tid + k * kFpsThreads;
+      px[k] = py[k] = pz[k] = 0.f;
+      if (i < m) {
+        const size_t src = (size_t)((uint64_t)i * n / (uint64_t)m);
+        px[k] = p[3 * src];
+        py[k] = p[3 * src + 1];
+        pz[k] = p[3 * src + 2];
+        ax += px[k];
+        ay += py[k];
+        az += pz[k];
+        lo[0] = fminf(lo[0], px[k]);
+        lo[1] = fminf(lo[1], py[k]);
+        lo[2] = fminf(lo[2], pz[k]);
+        hi[0] = fmaxf(hi[0], px[k]);
+        hi[1] = fmaxf(hi[1], py[k]);
+        hi[2] = fmaxf(hi[2], pz[k]);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ax += __shfl_xor(ax, o, 64);
+      ay += __shfl_xor(ay, o, 64);
+      az += __shfl_xor(az, o, 64);
+    }
+    const float b0 = wave_min_f32(lo[0]), b1 = wave_min_f32(lo[1]), b2 = wave_min_f32(lo[2]);
+    const float b3 = wave_max_f32(hi[0]), b4 = wave_max_f32(hi[1]), b5 = wave_max_f32(hi[2]);
+    if (lane == 0) {
+      cen[w][0] = ax;
+      cen[w][1] = ay;
+      cen[w][2] = az;
+      s_box[w][0] = b0;
+      s_box[w][1] = b1;
+      s_box[w][2] = b2;
+      s_box[w][3] = b3;
+      s_box[w][4] = b4;
+      s_box[w][5] = b5;
+    }
+    if (tid == 0) {  // subsample index 0 starts the selection
+      s_q[0] = px[0];
+      s_q[1] = py[0];
+      s_q[2] = pz[0];
+    }
+    for (int i = tid; i < kFpsCells; i += kFpsThreads) hist[i] = 0u;
+    __syncthreads();
+    float blo[3], sc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float l = s_box[0][a], h = s_box[0][3 + a];
+      for (int v = 1; v < kFpsWaves; ++v) {
+        l = fminf(l, s_box[v][a]);
+        h = fmaxf(h, s_box[v][3 + a]);
+      }
+      blo[a] = l;
+      sc[a] = h > l ? 16.f / (h - l) : 0.f;
+    }
+    // cell of every point and its rank within the cell (counting sort)
+#pragma unroll
+    for (int k = 0; k < kFpsPer; ++k) {
+      const int i = tid + k * kFpsThreads;
+      pos[k] = 0u;
+      if (i < m) {
+        const uint32_t cx = fps_cell((px[k] - blo[0]) * sc[0]);
+        const uint32_t cy = fps_cell((py[k] - blo[1]) * sc[1]);
+        const uint32_t cz = fps_cell((pz[k] - blo[2]) * sc[2]);
+        uint32_t code = 0u;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          code |= (((cx >> b) & 1u) << (3 * b)) | (((cy >> b) & 1u) << (3 * b + 1)) | (((cz >> b) & 1u) << (3 * b + 2));
+        pos[k] = (code << 16) | atomicAdd(&hist[code], 1u);
+      }
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    ax += __shfl_xor(ax, o, 64);
-    ay += __shfl_xor(ay, o, 64);
-    az += __shfl_xor(az, o, 64);
-  }
-  if (lane == 0) {
-    cen[w][0] = ax;
-    cen[w][1] = ay;
-    cen[w][2] = az;
-  }
-  if (tid == 0) {  // subsample index 0 starts the selection
-    s_q[0] = px[0];
-    s_q[1] = py[0];
-    s_q[2] = pz[0];
-  }
-  __syncthreads();
   double sx = 0, sy = 0, sz = 0;
   for (int k = 0; k < kFpsWaves; ++k) {
     sx += cen[k][0];
@@ -401,6 +492,88 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
     centroid[3 * c + 1] = cy;
     centroid[3 * c + 2] = cz;
   }
+  __syncthreads();
+  // exclusive scan of the histogram: 8 cells per thread, then the waves
+  {
+    uint32_t v[kFpsCells / kFpsThreads], t = 0;
+#pragma unroll
+    for (int j = 0; j < kFpsCells / kFpsThreads; ++j) {
+      v[j] = t;
+      t += hist[tid * (kFpsCells / kFpsThreads) + j];
+    }
+    uint32_t inc = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) s_wsum[w] = inc;
+    __syncthreads();
+    uint32_t base = inc - t;
+    for (int v2 = 0; v2 < w; ++v2) base += s_wsum[v2];
+#pragma unroll
+    for (int j = 0; j < kFpsCells / kFpsThreads; ++j) hist[tid * (kFpsCells / kFpsThreads) + j] = base + v[j];
+    __syncthreads();
+  }
+  // sorted position: valid points fill [0, m) in cell order; padding slot i
+  // (i >= m) stays at i
+#pragma unroll
+  for (int k = 0; k < kFpsPer; ++k) {
+    const int i = tid + k * kFpsThreads;
+    pos[k] = i < m ? hist[pos[k] >> 16] + (pos[k] & 0xffffu) : (uint32_t)i;
+  }
+  __syncthreads();  // the histogram is read: its space holds the inverse order next
+#pragma unroll
+  for (int k = 0; k < kFpsPer; ++k) inv[pos[k]] = (uint16_t)(tid + k * kFpsThreads);
+  __syncthreads();
+  // ---- the group layout: slot k of thread tid holds sorted position
+  // j = tid + 512 k, its point loaded again from the cloud
+  float px[kFpsPer], py[kFpsPer], pz[kFpsPer], dist[kFpsPer];
+  // group g = 8 k + w is slot k of wave w: sorted position j = tid + 512 k.
+  // Consecutive groups (neighbours in space) sit in different waves, so the
+  // few groups that one pick reaches are updated by several waves at once.
+#pragma unroll
+  for (int k = 0; k < kFpsPer; ++k) {
+    const int j = tid + k * kFpsThreads;
+    const int idx = inv[j];
+    px[k] = py[k] = pz[k] = 0.f;
+    dist[k] = -1.f;
+    if (idx < m) {
+      const size_t src = (size_t)((uint64_t)idx * n / (uint64_t)m);
+      px[k] = p[3 * src];
+      py[k] = p[3 * src + 1];
+      pz[k] = p[3 * src + 2];
+      dist[k] = 3.0e38f;
+    }
+  }
+  // ---- groups: bounding box and largest key, group k in lane k
+  float g_lo[3] = {0.f, 0.f, 0.f}, g_hi[3] = {0.f, 0.f, 0.f};
+  float gbest = -1.f;
+  uint64_t gkey = 0ull;
+#pragma unroll
+  for (int k = 0; k < kFpsPer; ++k) {
+    const bool v = dist[k] >= 0.f;
+    const float l0 = wave_min_f32(v ? px[k] : INFINITY), l1 = wave_min_f32(v ? py[k] : INFINITY);
+    const float l2 = wave_min_f32(v ? pz[k] : INFINITY);
+    const float h0 = wave_max_f32(v ? px[k] : -INFINITY), h1 = wave_max_f32(v ? py[k] : -INFINITY);
+    const float h2 = wave_max_f32(v ? pz[k] : -INFINITY);
+    const uint32_t klo = ((uint32_t)(kFpsMax - 1 - (int)inv[tid + k * kFpsThreads]) << 16) | (uint32_t)(k * 64 + lane);
+    const uint64_t gk = wave_max_u64(v ? ((uint64_t)(__float_as_uint(dist[k]) + 1u) << 32) | klo : 0ull);
+    if (lane == k) {
+      g_lo[0] = l0;
+      g_lo[1] = l1;
+      g_lo[2] = l2;
+      g_hi[0] = h0;
+      g_hi[1] = h1;
+      g_hi[2] = h2;
+      gkey = gk;
+      gbest = gk ? 3.0e38f : -1.f;
+    }
+  }
+  // the wave's best: (key, coordinates), recomputed when a group changes
+  uint64_t wkey = 0ull;
+  float wx = 0.f, wy = 0.f, wz = 0.f;
+
   int cur = 0;
   float qx = s_q[0], qy = s_q[1], qz = s_q[2];
   for (int s = 0; s < kP; ++s) {
@@ -411,70 +584,68 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
       s_cur[s] = cur;
     }
     if (s + 1 == kP) break;
-    float bv = -1.f, wx = 0.f, wy = 0.f, wz = 0.f;
-    int bi = 0x7fffffff;
+    // groups that q can reach: rounded squared box distance below the group's
+    // largest distance (lanes 0-15; a group of padding slots only, gbest -1,
+    // never changes)
+    const float gx = fmaxf(fmaxf(g_lo[0] - qx, qx - g_hi[0]), 0.f);
+    const float gy = fmaxf(fmaxf(g_lo[1] - qy, qy - g_hi[1]), 0.f);
+    const float gz = fmaxf(fmaxf(g_lo[2] - qz, qz - g_hi[2]), 0.f);
+    float lb = gx * gx;
+    lb = lb + gy * gy;
+    lb = lb + gz * gz;
+    const uint64_t need = __ballot(lane < kFpsPer && gbest >= 0.f && !(lb >= gbest && lb < INFINITY));
+    if (need) {
+      // uniform branches over constant slots: the distances stay in their
+      // own registers
 #pragma unroll
-    for (int k = 0; k < kFpsPer; ++k) {
-      // slots past the subsample hold -1, which fminf keeps and the strict
-      // comparison never picks, so every slot is updated without a branch
-      const float dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
-      float d = dx * dx;
-      d = d + dy * dy;
-      d = d + dz * dz;
-      dist[k] = fminf(dist[k], d);
-      // k ascending = index ascending: strict keeps the lowest (selects, no branch)
-      const bool gt = dist[k] > bv;
-      bv = gt ? dist[k] : bv;
-      bi = gt ? tid + k * kFpsThreads : bi;
-      wx = gt ? px[k] : wx;
-      wy = gt ? py[k] : wy;
-      wz = gt ? pz[k] : wz;
-    }
+      for (int k = 0; k < kFpsPer; ++k) {
+        if (need & (1ull << k)) {
+          const uint64_t key = fps_group_update(px[k], py[k], pz[k], dist[k], inv, tid + k * kFpsThreads, (uint32_t)(k * 64 + lane),
+                                                qx, qy, qz);
+          if (lane == k) {
+            gkey = key;
+            gbest = key ? __uint_as_float((uint32_t)(key >> 32) - 1u) : -1.f;
+          }
+        }
+      }
+      wkey = readlane_u64(row_max_u64(lane < kFpsPer ? gkey : 0ull), 0);
+      if (wkey) {
+        const int slot = (int)(wkey & 0xffffull);
+        const int kb = slot >> 6, lb = slot & 63;
+        // one scalar branch taken (the asm keeps the compiler from turning
+        // the branches into 48 vector selects)
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const float ov = __shfl_xor(bv, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      const bool take = (ov > bv) | ((ov == bv) & (oi < bi));
-      bv = take ? ov : bv;
-      bi = take ? oi : bi;
+        for (int k = 0; k < kFpsPer; ++k)
+          if (kb == k) {
+            asm volatile("" ::"v"(px[k]));
+            wx = readlane_f32(px[k], lb);
+            wy = readlane_f32(py[k], lb);
+            wz = readlane_f32(pz[k], lb);
+          }
+      }
     }
-    // the lane that owns the wave's winner publishes it with its coordinates
     const int buf = s & 1;
-    if (bi >= 0 && bi < m && (bi % kFpsThreads) == tid) {
-      // the wave's winner is this lane's own best
+    if (lane == 0) {
+      red_k[buf][w] = wkey;
       red_p[buf][w][0] = wx;
       red_p[buf][w][1] = wy;
       red_p[buf][w][2] = wz;
-      red_v[buf][w] = bv;
-      red_i[buf][w] = bi;
-    } else if (lane == 0 && !(bi >= 0 && bi < m)) {  // no subsample point in this wave
-      red_v[buf][w] = -1.f;
-      red_i[buf][w] = 0x7fffffff;
     }
     __syncthreads();
-    // the waves' candidates, read together, then compared without branches
-    float rv[kFpsWaves];
-    int ri[kFpsWaves];
+    uint64_t gk = red_k[buf][0];
+    int gw = 0;
 #pragma unroll
-    for (int k = 0; k < kFpsWaves; ++k) {
-      rv[k] = red_v[buf][k];
-      ri[k] = red_i[buf][k];
+    for (int v = 1; v < kFpsWaves; ++v) {
+      const uint64_t o = red_k[buf][v];
+      gw = o > gk ? v : gw;
+      gk = o > gk ? o : gk;
     }
-    float gv = rv[0];
-    int gi = ri[0], gw = 0;
-#pragma unroll
-    for (int k = 1; k < kFpsWaves; ++k) {
-      const bool take = (rv[k] > gv) | ((rv[k] == gv) & (ri[k] < gi));
-      gv = take ? rv[k] : gv;
-      gi = take ? ri[k] : gi;
-      gw = take ? k : gw;
-    }
-    if (gi >= 0 && gi < m) {
-      cur = gi;
+    if (gk) {
+      cur = kFpsMax - 1 - (int)((uint32_t)gk >> 16);
       qx = red_p[buf][gw][0];
       qy = red_p[buf][gw][1];
       qz = red_p[buf][gw][2];
-    } else {  // fewer subsample points than selections: repeat the first one
+    } else {  // no subsample point: repeat the first one
       cur = 0;
       qx = s_q[0];
       qy = s_q[1];

@@ -103,4 +103,37 @@ __device__ __forceinline__ float wave_max_f32(float v) {
   return fmaxf(fmaxf(readlane_f32(v, 0), readlane_f32(v, 16)), fmaxf(readlane_f32(v, 32), readlane_f32(v, 48)));
 }
 
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)dpp_i32<CTRL>((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)dpp_i32<CTRL>((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Every lane of each 16-lane row gets the row's largest value.
+__device__ __forceinline__ uint64_t row_max_u64(uint64_t v) {
+  uint64_t o = dpp_u64<kDppQuadXor1>(v);
+  v = o > v ? o : v;
+  o = dpp_u64<kDppQuadXor2>(v);
+  v = o > v ? o : v;
+  o = dpp_u64<kDppRowHalfMirror>(v);
+  v = o > v ? o : v;
+  o = dpp_u64<kDppRowMirror>(v);
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  v = row_max_u64(v);
+  const uint64_t a = readlane_u64(v, 0), b = readlane_u64(v, 16);
+  const uint64_t c = readlane_u64(v, 32), d = readlane_u64(v, 48);
+  const uint64_t ab = a > b ? a : b, cd = c > d ? c : d;
+  return ab > cd ? ab : cd;
+}
+
 }  // namespace rtr
