@@ -354,23 +354,38 @@ static_assert(kFpsCells * 4 == kFpsMax * 2, "the inverse order (u16) fills the h
 
 // A group's distances lowered by q (the group is one register slot of all
 // 64 lanes); returns the group's largest key (0: no point), the same in every
-// lane.  key = (distance bits + 1) << 32 | (8191 - index) << 16 | slot.
-// The point's subsample index comes from the inverse order in LDS (inv[j],
-// j its sorted position), read only when the group is updated.
+// lane.  key = (distance bits + 1) << 32 | (8191 - index) << 16 | slot.  The
+// largest distance comes from a float wave max (fmaxf returns one of its
+// operands, so it equals the winner's distance bit for bit); a ballot finds
+// the lane holding it, and only when several lanes hold it (equal distances)
+// does a second reduction pick the lowest subsample index among them.  The
+// index comes from the inverse order in LDS (inv[j], j the sorted position;
+// jw = j of lane 0 in this slot).
 __device__ __forceinline__ uint64_t fps_group_update(float px, float py, float pz, float& dist, const uint16_t* inv,
-                                                     int j, uint32_t slot, float qx, float qy, float qz) {
+                                                     int jw, int lane, int k, float qx, float qy, float qz) {
   // an empty asm keeps the compiler from computing every slot's distances
-  // (and reading every index) ahead of the branch that selects the slot:
-  // they are pure and would be hoisted or speculated
-  asm volatile("" : "+v"(qx), "+v"(qy), "+v"(qz), "+v"(j));
-  const uint32_t klo = ((uint32_t)(kFpsMax - 1 - (int)inv[j]) << 16) | slot;
+  // ahead of the branch that selects the slot: they are pure and would be
+  // hoisted or speculated
+  asm volatile("" : "+v"(qx), "+v"(qy), "+v"(qz));
   const float dx = px - qx, dy = py - qy, dz = pz - qz;
   float d = dx * dx;
   d = d + dy * dy;
   d = d + dz * dz;
   dist = fminf(dist, d);  // padding slots hold -1 and keep it
-  const uint64_t key = dist >= 0.f ? ((uint64_t)(__float_as_uint(dist) + 1u) << 32) | klo : 0ull;
-  return wave_max_u64(key);
+  const float gmax = wave_max_f32(dist);
+  if (!(gmax >= 0.f)) return 0ull;  // padding only
+  const uint64_t hit = __ballot(dist == gmax);
+  int lb, idx;
+  if (__builtin_popcountll(hit) == 1) {
+    lb = __builtin_ctzll(hit);
+    idx = inv[jw + lb];
+  } else {
+    const int v = wave_min_i32(dist == gmax ? ((int)inv[jw + lane] << 6) | lane : 0x7fffffff);
+    idx = v >> 6;
+    lb = v & 63;
+  }
+  return ((uint64_t)(__float_as_uint(gmax) + 1u) << 32) | ((uint32_t)(kFpsMax - 1 - idx) << 16) |
+         (uint32_t)(k * 64 + lb);
 }
 
 // cell coordinate in [0, 15] of t = (v - lo) * 16 / extent (NaN -> 0)
@@ -600,8 +615,8 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
 #pragma unroll
       for (int k = 0; k < kFpsPer; ++k) {
         if (need & (1ull << k)) {
-          const uint64_t key = fps_group_update(px[k], py[k], pz[k], dist[k], inv, tid + k * kFpsThreads, (uint32_t)(k * 64 + lane),
-                                                qx, qy, qz);
+          const uint64_t key =
+              fps_group_update(px[k], py[k], pz[k], dist[k], inv, w * 64 + k * kFpsThreads, lane, k, qx, qy, qz);
           if (lane == k) {
             gkey = key;
             gbest = key ? __uint_as_float((uint32_t)(key >> 32) - 1u) : -1.f;

@@ -41,14 +41,20 @@ def test_gemm_bf16_matches_numpy(M, N, K, ta, tb, ks):
 
 
 def test_fps_matches_oracle_bit_exact():
+    """Random composites of several sizes, plus two clouds full of ties: 40
+    points each repeated three times (every distance 0 once they are
+    picked), and a 12x12x12 lattice (many equal distances)."""
     rng = np.random.default_rng(5)
     boxes = synth.random_composite(rng, asymmetric=True)
     clouds = synth.sample_many([boxes] * 6, [500, 1024, 5000, 8192, 20000, 60000], rng, 0.001)
-    src, tgt = clouds[0::2], clouds[1::2]
-    prop = learned.Proposer(max_pairs=3)
+    dup = np.repeat(clouds[0][:40], 3, axis=0)
+    g = np.arange(12, dtype=np.float32) * np.float32(0.05)
+    lattice = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    src, tgt = clouds[0::2] + [dup], clouds[1::2] + [lattice]
+    prop = learned.Proposer(max_pairs=4)
     prop.set_clouds(src, tgt)
     pts, cen, sel = prop.fps()
-    order = [src[0], tgt[0], src[1], tgt[1], src[2], tgt[2]]
+    order = [src[0], tgt[0], src[1], tgt[1], src[2], tgt[2], src[3], tgt[3]]
     for c, cloud in enumerate(order):
         s_ref, c_ref, sub = oracle.fps(cloud)
         assert np.array_equal(sel[c], s_ref), f"cloud {c} ({len(cloud)} pts): selection differs"
