@@ -10,12 +10,13 @@ which no residual-based acceptance test can see:
 
   * the displacement of the source points between the two poses, projected
     on the target's normals at the reference correspondences, is at most
-    half of its RMS (measured: 26-45 %, 0.25-0.71 mm against 0.69-1.96 mm);
+    half of its RMS (measured on the 20 misses: 26-45 %, 0.33-0.71 mm
+    against 1.01-1.96 mm);
   * point-to-point ICP's own objective (mean squared distance to the nearest
     target point within r_max) is no more than 2 % higher at the learned
     pose than at the reference pose (measured: at most 1.4 % higher, and
-    lower for 7 of the 25 misses);
-  * at most 32 of 256 registrations miss (measured 25).
+    lower for 2 of the 20);
+  * at most 32 of 256 registrations miss (measured 20).
 
 The objective is flat to within about 1 % along 1-2 mm tangential slides of
 these box composites (noise 1 mm), and ICP's stop rule (transformation
