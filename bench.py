@@ -169,6 +169,17 @@ def tuning_key(cfg):
     return v + [0.0] * (7 - len(v))
 
 
+def table_summary(table):
+    """The autotuner's table in brief: candidates timed, the fastest, and the
+    fastest with LDS candidate tiles on (use_tile = 1)."""
+    if not table:
+        return None
+    rows = sorted(((list(c), round(float(ms), 3)) for c, ms in table), key=lambda r: r[1])
+    tiles = [r for r in rows if int(r[0][4]) == 1]
+    return {"candidates": len(rows), "fastest": rows[0], "fastest_tile": tiles[0] if tiles else None,
+            "slowest": rows[-1]}
+
+
 def prefer_profiled(reg, best, best_ms, table, name, tol=0.03):
     """Among candidates the autotuner timed within `tol` of its fastest (a
     statistical tie), prefer the tuning of the committed rocprofv3 summary of
@@ -323,6 +334,7 @@ def main():
         ms = learned.MultiStartRegistration(params, n_starts=args.starts)
         ms.set_clouds(srcs, tgts, cs, ct)
     tuned = None
+    table, table_r = None, None
     if args.tuning:
         f = [float(v) for v in args.tuning.split(",")]
         tuned = (f[0], f[1], int(f[2]), int(f[3]), int(f[4])) + tuple(f[5:6]) + tuple(int(v) for v in f[6:7])
@@ -519,6 +531,8 @@ def main():
                    "refine_coarse_iterations": args.refine_coarse_iterations,
                    "tuning": list(tuned) if tuned else None,
                    "tuning_refine": list(tuned_refine) if tuned_refine else None,
+                   "autotune_plain": table_summary(table),
+                   "autotune_refine": table_summary(table_r),
                    "learned_ok": int(ok_l.sum()), "learned_fallbacks": st["n_fallback"],
                    "learned_ok_vs_same_init": int(np.sum(e_l_init <= OK_RMSE)),
                    "learned_ok_vs_generating_pose": int(np.sum(e_l_gt <= OK_RMSE)),
@@ -662,7 +676,7 @@ def config3_leg(args, params, threads, timed):
     reg.set_clouds(srcs, tgts)
     reg.set_problems(range(n), range(n))
     guesses = np.repeat(np.eye(4)[None], n, 0)
-    tuned, tuned_ms, _ = autotune.tune(reg, guesses, candidates=autotune.CANDIDATES, key=None, use_cache=False)
+    tuned, tuned_ms, table3 = autotune.tune(reg, guesses, candidates=autotune.CANDIDATES, key=None, use_cache=False)
     log(f"config 3: autotune {tuned} ({tuned_ms:.1f} ms)")
     t3, res = timed(lambda: reg.align(guesses), args.steps, args.warmup)
     roof = launch_roofline(reg, guesses, np.arange(n), srcs, tgts, args.account_sample)
@@ -679,7 +693,7 @@ def config3_leg(args, params, threads, timed):
                         f"r_max {R_MAX} m, <={MAX_ITER} iterations, eps {EPS}, one persistent launch + hipGraph, "
                         "grid build included",
             "value": float(ok.sum() / t3), "unit": "registrations/s", "ms_per_step": t3 * 1e3, "ok": int(ok.sum()),
-            "pairs": n, "tuning": list(tuned),
+            "pairs": n, "tuning": list(tuned), "autotune": table_summary(table3),
             "iterations_total": int(sum(r["iterations"] for r in res)),
             "parity_vs_fp64_same_init": {"rmse_m": stats_m(e_init), "rotation_frobenius_max": float(e_r.max()),
                                          "translation_m_max": float(e_t.max())},

@@ -147,6 +147,7 @@ struct IcpShared {
   uint8_t qd[kChunk];     // its cell's distance-field value
   uint16_t qlist[kChunk]; // the chunk's queries to walk by class, each class in query order
   uint32_t qseed[kChunk]; // its search seed: the last iteration's nearest-neighbour slot (or ~0)
+  uint32_t ccount[kWaves][kQueryClasses];  // queries per class in each wave's quarter of the chunk
   int nlist;
   double red[kWaves][NS];
   double dred[kBlock / NSUM_MAX][NSUM_MAX];
@@ -526,17 +527,45 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
         sh.qseed[o] = seeded && cls ? seeds[chunk + o] : ~0u;
       }
       __syncthreads();
-      if (w == 0) {
+      {
+        // The walk list: class by class, each class in query order.  Wave w
+        // lists the classes of its quarter of the chunk (kSegs runs of 64
+        // queries): counts per class first, then each wave writes from the
+        // offset of its class in the earlier classes and earlier waves.
+        constexpr int kSegs = kChunk / 64 / kWaves;
         const uint64_t below = (1ull << lane) - 1ull;
-        uint32_t nl = 0;
-        for (int cls = 1; cls <= kQueryClasses; ++cls)
-          for (uint32_t b = 0; b < cn; b += 64) {
-            const bool f = b + lane < cn && sh.qcls[b + lane] == cls;
-            const uint64_t bal = __ballot(f);
-            if (f) sh.qlist[nl + (uint32_t)__popcll(bal & below)] = (uint16_t)(b + lane);
-            nl += (uint32_t)__popcll(bal);
+        uint8_t c[kSegs];
+#pragma unroll
+        for (int sg = 0; sg < kSegs; ++sg) {
+          const uint32_t o = (uint32_t)((w * kSegs + sg) * 64 + lane);
+          c[sg] = o < cn ? sh.qcls[o] : (uint8_t)0;
+        }
+#pragma unroll
+        for (int cls = 1; cls <= kQueryClasses; ++cls) {
+          uint32_t n = 0;
+#pragma unroll
+          for (int sg = 0; sg < kSegs; ++sg) n += (uint32_t)__popcll(__ballot(c[sg] == cls));
+          if (lane == 0) sh.ccount[w][cls - 1] = n;
+        }
+        __syncthreads();
+        uint32_t nl = 0;  // the list's length so far (classes below cls, all waves)
+#pragma unroll
+        for (int cls = 1; cls <= kQueryClasses; ++cls) {
+          uint32_t off = nl;
+          for (int v = 0; v < kWaves; ++v) {
+            const uint32_t n = sh.ccount[v][cls - 1];
+            off += v < w ? n : 0u;
+            nl += n;
           }
-        if (lane == 0) sh.nlist = (int)nl;
+#pragma unroll
+          for (int sg = 0; sg < kSegs; ++sg) {
+            const bool f = c[sg] == cls;
+            const uint64_t bal = __ballot(f);
+            if (f) sh.qlist[off + (uint32_t)__popcll(bal & below)] = (uint16_t)((w * kSegs + sg) * 64 + lane);
+            off += (uint32_t)__popcll(bal);
+          }
+        }
+        if (tid == 0) sh.nlist = (int)nl;
       }
       __syncthreads();
       const int nl = sh.nlist;
