@@ -149,6 +149,7 @@ struct IcpShared {
   uint32_t qseed[kChunk]; // its search seed: the last iteration's nearest-neighbour slot (or ~0)
   uint32_t ccount[kWaves][kQueryClasses];  // queries per class in each wave's quarter of the chunk
   int nlist;
+  int lnext;              // the walk list's next unclaimed block of 64 entries
   double red[kWaves][NS];
   double dred[kBlock / NSUM_MAX][NSUM_MAX];
   double ws[kAndersonWs];
@@ -565,33 +566,44 @@ __device__ void process_item(const DevEngine& e, int it, int pass, IcpShared<NS,
             off += (uint32_t)__popcll(bal);
           }
         }
-        if (tid == 0) sh.nlist = (int)nl;
+        if (tid == 0) {
+          sh.nlist = (int)nl;
+          sh.lnext = kBlock;
+        }
       }
       __syncthreads();
       const int nl = sh.nlist;
-      for (int base = w * 64; base < nl; base += kBlock) {
+      // Blocks of 64 list entries go to the waves as they finish (an LDS
+      // counter), so a wave that drew cheap queries takes the next block
+      // instead of idling at the chunk's closing barrier; every query's
+      // result lands in its own LDS slot, whichever wave walks it.
+      for (int base = w * 64; base < nl;) {
         const int li = base + lane;
-        if (li >= nl) continue;
-        const uint32_t o = sh.qlist[li];
-        const uint32_t sj = sd.off + chunk + o;
-        float qx, qy, qz;
-        int cx, cy, cz;
-        apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
-        query_cell(g, qx, qy, qz, cx, cy, cz);
-        const int D = sh.qd[o];
-        ++n_search;
-        GlobalVis<DEPTH> vis;
-        vis.init(g.pts, qx, qy, qz);
-        const uint32_t s0 = sh.qseed[o];
-        if (s0 - g.pt_lo < g.n_pts) vis.seed(s0);
-        if (D < 2) walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
-        walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, D >= 2 ? D - 1 : 1);
-        // a seed that is still the nearest neighbour is not written again
-        // (most queries of a converging problem keep theirs)
-        if (seeds && vis.key != kNoKey && (vis.where & 0x7fffffffu) != s0) seeds[chunk + o] = vis.where & 0x7fffffffu;
-        // the query's correspondence (its seed slot is read above by this
-        // lane only, so the slot can hold the result)
-        sh.qseed[o] = key_d2(vis.key) <= r2f ? (vis.where & 0x7fffffffu) : ~0u;
+        if (li < nl) {
+          const uint32_t o = sh.qlist[li];
+          const uint32_t sj = sd.off + chunk + o;
+          float qx, qy, qz;
+          int cx, cy, cz;
+          apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
+          query_cell(g, qx, qy, qz, cx, cy, cz);
+          const int D = sh.qd[o];
+          ++n_search;
+          GlobalVis<DEPTH> vis;
+          vis.init(g.pts, qx, qy, qz);
+          const uint32_t s0 = sh.qseed[o];
+          if (s0 - g.pt_lo < g.n_pts) vis.seed(s0);
+          if (D < 2) walk_block_global(g, qx, qy, qz, cx, cy, cz, r2f, vis);
+          walk_far(g, qx, qy, qz, cx, cy, cz, r2f, vis, D >= 2 ? D - 1 : 1);
+          // a seed that is still the nearest neighbour is not written again
+          // (most queries of a converging problem keep theirs)
+          if (seeds && vis.key != kNoKey && (vis.where & 0x7fffffffu) != s0) seeds[chunk + o] = vis.where & 0x7fffffffu;
+          // the query's correspondence (its seed slot is read above by this
+          // lane only, so the slot can hold the result)
+          sh.qseed[o] = key_d2(vis.key) <= r2f ? (vis.where & 0x7fffffffu) : ~0u;
+        }
+        int nb = 0;
+        if (lane == 0) nb = atomicAdd(&sh.lnext, 64);
+        base = __builtin_amdgcn_readfirstlane(nb);
       }
       __syncthreads();
       // Moment phase: the walks above keep no moment sums live (their
