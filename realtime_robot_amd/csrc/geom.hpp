@@ -810,7 +810,7 @@ RTR_HD bool anderson_mix(const double* H, int k, double* x_new, double* ws) {
   double* D = ws + 120;  // n x 6: dF_j = f_(j+1) - f_j, f_j = g_j - x_j (each difference taken once)
 #pragma unroll 1
   for (int j = 0; j < n; ++j)
-#pragma unroll 1
+#pragma unroll
     for (int i = 0; i < 6; ++i)
       D[6 * j + i] = (H[12 * (j + 1) + 6 + i] - H[12 * (j + 1) + i]) - (H[12 * j + 6 + i] - H[12 * j + i]);
 #pragma unroll 1
@@ -818,13 +818,13 @@ RTR_HD bool anderson_mix(const double* H, int k, double* x_new, double* ws) {
 #pragma unroll 1
     for (int c = r; c < n; ++c) {
       double s = 0.0;
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < 6; ++i) s += D[6 * r + i] * D[6 * c + i];
       A[r * n + c] = s;
       A[c * n + r] = s;
     }
     double s = 0.0;
-#pragma unroll 1
+#pragma unroll
     for (int i = 0; i < 6; ++i) s += D[6 * r + i] * (H[12 * (k - 1) + 6 + i] - H[12 * (k - 1) + i]);
     b[r] = s;
   }
@@ -880,7 +880,7 @@ RTR_HD bool anderson_mix(const double* H, int k, double* x_new, double* ws) {
 // Pose (column-major 4x4) -> x relative to the row-major base rotation Rb.
 // M: 9 doubles of workspace.
 RTR_HD bool pose_to_x(const double* T, const double* Rb, double* x, double* M) {
-#pragma unroll 1
+#pragma unroll
   for (int i = 0; i < 9; ++i) {
     const int r = i / 3, c = i % 3;  // (R Rb^T)[r][c]
     M[i] = (T[r] * Rb[c * 3] + T[4 + r] * Rb[c * 3 + 1]) + T[8 + r] * Rb[c * 3 + 2];
@@ -896,7 +896,7 @@ RTR_HD bool pose_to_x(const double* T, const double* Rb, double* x, double* M) {
 // doubles of workspace.
 RTR_HD bool x_to_pose(const double* x, const double* Rb, double* T, double* E) {
   if (!qv_to_so3(x, E)) return false;
-#pragma unroll 1
+#pragma unroll
   for (int i = 0; i < 9; ++i) {
     const int r = i / 3, c = i % 3;  // (E Rb)[r][c]
     T[c * 4 + r] = (E[r * 3] * Rb[c] + E[r * 3 + 1] * Rb[3 + c]) + E[r * 3 + 2] * Rb[6 + c];
@@ -919,7 +919,7 @@ RTR_HD int anderson_pose_step(double* st, int m, const double* Tq, const double*
                               double* ws) {
   double* x = ws + 80;  // 6
   double* M = ws + 88;  // 9
-#pragma unroll 1
+#pragma unroll
   for (int i = 0; i < 16; ++i) Tout[i] = Tg[i];
   if (m < 1) return 0;
   if (m > kAndersonMax) m = kAndersonMax;
@@ -931,19 +931,26 @@ RTR_HD int anderson_pose_step(double* st, int m, const double* Tq, const double*
     st[9] = 0.0;
     st[10] = 1.7976931348623157e308;
     if (x_to_pose(st + 12 + 12 * (k - 1) + 6, st, ws + 100, M)) {
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < 16; ++i) Tout[i] = ws[100 + i];
       return 0;
     }
     k = 0;
   }
   if (k == 0) {
-#pragma unroll 1
+#pragma unroll
     for (int i = 0; i < 9; ++i) st[i] = Tq[(i % 3) * 4 + i / 3];
   }
   if (k == m + 1) {  // drop the oldest pair
+    // loads in groups of 4, so they are in flight together
 #pragma unroll 1
-    for (int i = 12; i < 12 + 12 * m; ++i) st[i] = st[i + 12];
+    for (int i = 12; i < 12 + 12 * m; i += 4) {
+      const double t0 = st[i + 12], t1 = st[i + 13], t2 = st[i + 14], t3 = st[i + 15];
+      st[i] = t0;
+      st[i + 1] = t1;
+      st[i + 2] = t2;
+      st[i + 3] = t3;
+    }
     k = m;
   }
   double* pr = st + 12 + 12 * k;
@@ -961,7 +968,7 @@ RTR_HD int anderson_pose_step(double* st, int m, const double* Tq, const double*
   // map dominates the history) would stall the iteration; take the plain
   // step and start a new history from it
   double da = 0.0, dp = 0.0;
-#pragma unroll 1
+#pragma unroll
   for (int i = 0; i < 6; ++i) {
     da += (x[i] - pr[i]) * (x[i] - pr[i]);
     dp += (pr[6 + i] - pr[i]) * (pr[6 + i] - pr[i]);
@@ -973,11 +980,11 @@ RTR_HD int anderson_pose_step(double* st, int m, const double* Tq, const double*
   // trust region: an extrapolation beyond ten plain steps is shortened to ten
   if (da > 100.0 * dp) {
     const double f = 10.0 * sqrt(dp / da);
-#pragma unroll 1
+#pragma unroll
     for (int i = 0; i < 6; ++i) x[i] = pr[i] + f * (x[i] - pr[i]);
   }
   if (!x_to_pose(x, st, ws + 100, M)) return 0;  // outside the chart: plain step
-#pragma unroll 1
+#pragma unroll
   for (int i = 0; i < 16; ++i) Tout[i] = ws[100 + i];
   return 1;
 }
