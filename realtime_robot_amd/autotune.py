@@ -84,6 +84,9 @@ LATENCY_CANDIDATES = CANDIDATES + [
     (0.25, 2.0, 1024, 0, 0, 0.5, 16),
     (0.25, 2.0, 512, 0, 0, 0.5, 10),
     (0.25, 2.0, 1024, 4, 0, 0.5, 10),
+    (0.25, 2.0, 1024, 0, 0, 0.4, 16),
+    (0.25, 2.0, 1024, 0, 0, 0.5, 13),
+    (0.25, 2.0, 1024, 0, 0, 0.4, 13),
 ]
 
 
