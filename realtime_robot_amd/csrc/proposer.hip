@@ -904,20 +904,19 @@ __global__ void k_concat_multi(const float* G, int n, int k, uint16_t* F) {
 // moved by the candidate (R_r Q_j, t_r) and its target FPS points, both in
 // their centred frames: mean over the 1024 source points of
 // min(min_q |R s + t - q|^2, tau2).  One workgroup per candidate r = j*n + p;
-// the target points are staged in LDS.
+// the target points are staged in LDS as float4, and each thread moves its
+// four source points together through them (one LDS read per target point
+// for four distances; the distances are computed as before).
 __global__ __launch_bounds__(256) void k_score(const float* pts, const double* pose, const float* rots, int n,
                                                int k, float tau2, double* score) {
-  __shared__ float tx[kP], ty[kP], tz[kP];
+  constexpr int kPer = kP / 256;
+  __shared__ float4 tq[kP];
   __shared__ double red[4];
   const int r = blockIdx.x;
   if (r >= k * n) return;
   const int j = r / n, p = r - j * n;
   const float* tp = pts + (size_t)(2 * p + 1) * kP * 3;
-  for (int i = threadIdx.x; i < kP; i += 256) {
-    tx[i] = tp[3 * i];
-    ty[i] = tp[3 * i + 1];
-    tz[i] = tp[3 * i + 2];
-  }
+  for (int i = threadIdx.x; i < kP; i += 256) tq[i] = make_float4(tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], 0.f);
   const double* P = pose + (size_t)r * 12;
   const float* Q = rots + 9 * j;
   float M[9];
@@ -927,19 +926,27 @@ __global__ __launch_bounds__(256) void k_score(const float* pts, const double* p
   const float t0 = (float)P[9], t1 = (float)P[10], t2 = (float)P[11];
   __syncthreads();
   const float* sp = pts + (size_t)(2 * p) * kP * 3;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < kP; i += 256) {
+  float qx[kPer], qy[kPer], qz[kPer], best[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int i = threadIdx.x + u * 256;
     const float sx = sp[3 * i], sy = sp[3 * i + 1], sz = sp[3 * i + 2];
-    const float qx = M[0] * sx + M[1] * sy + M[2] * sz + t0;
-    const float qy = M[3] * sx + M[4] * sy + M[5] * sz + t1;
-    const float qz = M[6] * sx + M[7] * sy + M[8] * sz + t2;
-    float best = tau2;
-    for (int q = 0; q < kP; ++q) {
-      const float dx = qx - tx[q], dy = qy - ty[q], dz = qz - tz[q];
-      best = fminf(best, dx * dx + dy * dy + dz * dz);
-    }
-    acc += best;
+    qx[u] = M[0] * sx + M[1] * sy + M[2] * sz + t0;
+    qy[u] = M[3] * sx + M[4] * sy + M[5] * sz + t1;
+    qz[u] = M[6] * sx + M[7] * sy + M[8] * sz + t2;
+    best[u] = tau2;
   }
+  for (int q = 0; q < kP; ++q) {
+    const float4 c = tq[q];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const float dx = qx[u] - c.x, dy = qy[u] - c.y, dz = qz[u] - c.z;
+      best[u] = fminf(best[u], dx * dx + dy * dy + dz * dz);
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) acc += best[u];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
