@@ -172,11 +172,11 @@ def result_dicts(res) -> list[dict]:
         return []
     arr = np.ctypeslib.as_array(res)
     Ts = arr["T"].reshape(n, 4, 4).transpose(0, 2, 1).copy()
-    f = {k: arr[k].tolist() for k in ("mse", "fitness_rmse", "n_corr", "iterations", "converged", "state",
-                                       "coarse_iterations")}
-    return [dict(T=Ts[i], mse=f["mse"][i], fitness_rmse=f["fitness_rmse"][i], n_corr=f["n_corr"][i],
-                 iterations=f["iterations"][i], converged=bool(f["converged"][i]), state=f["state"][i],
-                 coarse_iterations=f["coarse_iterations"][i]) for i in range(n)]
+    mse, fr, nc, it, st, ci = (arr[k].tolist() for k in ("mse", "fitness_rmse", "n_corr", "iterations", "state",
+                                                           "coarse_iterations"))
+    cv = (arr["converged"] != 0).tolist()
+    return [{"T": T, "mse": a, "fitness_rmse": b, "n_corr": c, "iterations": d, "converged": e, "state": f,
+             "coarse_iterations": g} for T, a, b, c, d, e, f, g in zip(Ts, mse, fr, nc, it, cv, st, ci)]
 
 
 def make_params(max_iterations=10, max_correspondence_distance=float(np.sqrt(DBL_MAX)), transformation_epsilon=0.0,
