@@ -1013,33 +1013,48 @@ void launch_fitness_d2(const DevEngine& e, float* d2_all, hipStream_t s) {
 constexpr int kNrmBlock = 128;
 constexpr int kMaxK = 32;
 
+// Sorted insertion of the candidates into a k-long list.  Keys are unique
+// (their low word is the point's index), so the final list does not depend on
+// the order in which the candidates arrive: the scan loads four candidates
+// before inserting them one by one, keeping four loads in flight, and keeps
+// the list's last key in a register.  Initialise with cnt = 0.
 struct KnnVisitor {
   const float4* pts;
   float qx, qy, qz;
   int k, cnt;
+  uint64_t kth;    // key[k - 1] once cnt == k
   uint64_t* key;   // LDS, stride kNrmBlock
   uint32_t* slot;  // LDS, stride kNrmBlock; nullptr keeps keys only (their low word is the point's original index)
-  __device__ void scan(uint32_t b, uint32_t e) {
-    for (uint32_t j = b; j < e; ++j) {
-      const float4 t = pts[j];
-      union {
-        float f;
-        int32_t i;
-      } w;
-      w.f = t.w;
-      const uint64_t kk = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), w.i);
-      if (cnt == k && kk >= key[(k - 1) * kNrmBlock]) continue;
-      int pos = cnt < k ? cnt++ : k - 1;
-      while (pos > 0 && kk < key[(pos - 1) * kNrmBlock]) {
-        key[pos * kNrmBlock] = key[(pos - 1) * kNrmBlock];
-        if (slot) slot[pos * kNrmBlock] = slot[(pos - 1) * kNrmBlock];
-        --pos;
-      }
-      key[pos * kNrmBlock] = kk;
-      if (slot) slot[pos * kNrmBlock] = j;
+  __device__ void insert(const float4& t, uint32_t j) {
+    union {
+      float f;
+      int32_t i;
+    } w;
+    w.f = t.w;
+    const uint64_t kk = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), w.i);
+    if (cnt == k && kk >= kth) return;
+    int pos = cnt < k ? cnt++ : k - 1;
+    while (pos > 0 && kk < key[(pos - 1) * kNrmBlock]) {
+      key[pos * kNrmBlock] = key[(pos - 1) * kNrmBlock];
+      if (slot) slot[pos * kNrmBlock] = slot[(pos - 1) * kNrmBlock];
+      --pos;
     }
+    key[pos * kNrmBlock] = kk;
+    if (slot) slot[pos * kNrmBlock] = j;
+    if (cnt == k) kth = key[(k - 1) * kNrmBlock];
   }
-  __device__ float bound() const { return cnt < k ? 3.4e38f : key_d2(key[(k - 1) * kNrmBlock]); }
+  __device__ void scan(uint32_t b, uint32_t e) {
+    uint32_t j = b;
+    for (; j + 4 <= e; j += 4) {
+      float4 t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = pts[j + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) insert(t[u], j + u);
+    }
+    for (; j < e; ++j) insert(pts[j], j);
+  }
+  __device__ float bound() const { return cnt < k ? 3.4e38f : key_d2(kth); }
 };
 
 __device__ void jacobi3(double* a, double* V) {
@@ -1105,6 +1120,7 @@ __global__ __launch_bounds__(kNrmBlock) void k_target_normals(DevEngine e, int k
     double nv[3] = {__builtin_nan(""), __builtin_nan(""), __builtin_nan("")};
     if (v.cnt >= 3) {
       double mean[3] = {0, 0, 0};
+#pragma unroll 4
       for (int i = 0; i < v.cnt; ++i) {
         const float4 p = e.tgt_sorted[v.slot[i * kNrmBlock]];
         mean[0] += p.x;
@@ -1113,6 +1129,7 @@ __global__ __launch_bounds__(kNrmBlock) void k_target_normals(DevEngine e, int k
       }
       for (int a = 0; a < 3; ++a) mean[a] /= v.cnt;
       double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
       for (int i = 0; i < v.cnt; ++i) {
         const float4 p = e.tgt_sorted[v.slot[i * kNrmBlock]];
         const double d[3] = {p.x - mean[0], p.y - mean[1], p.z - mean[2]};
@@ -1228,6 +1245,7 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     if (v.cnt < 3) continue;
     const float* const raw = e.tgt_raw + 3 * (size_t)td.raw_off;
     double mean[3] = {0, 0, 0};
+#pragma unroll 4
     for (int i = 0; i < v.cnt; ++i) {
       const float* p = raw + 3 * (size_t)(uint32_t)v.key[i * kNrmBlock];
       mean[0] += p[0];
@@ -1236,6 +1254,7 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
     }
     for (int a = 0; a < 3; ++a) mean[a] /= v.cnt;
     double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
     for (int i = 0; i < v.cnt; ++i) {
       const float* p = raw + 3 * (size_t)(uint32_t)v.key[i * kNrmBlock];
       const double d[3] = {p[0] - mean[0], p[1] - mean[1], p[2] - mean[2]};
