@@ -1055,6 +1055,76 @@ struct KnnVisitor {
     for (; j < e; ++j) insert(pts[j], j);
   }
   __device__ float bound() const { return cnt < k ? 3.4e38f : key_d2(kth); }
+  // the judge's list interface: reset, size, and the keys in ascending order
+  __device__ void reset() { cnt = 0; }
+  __device__ int size() const { return cnt; }
+  __device__ uint64_t first() const { return key[0]; }
+  __device__ uint64_t last() const { return key[(k - 1) * kNrmBlock]; }
+  template <class F>
+  __device__ void for_each(F f) const {
+#pragma unroll 4
+    for (int i = 0; i < cnt; ++i) f(key[i * kNrmBlock]);
+  }
+};
+
+// The K nearest keys in registers, ascending, empty slots holding ~0 (no
+// real key reaches it: a finite or infinite float d2 has high word below
+// 0xffffffff).  A candidate below the last key goes through K compare-swap
+// steps, which leave the K smallest keys in order: no serial LDS shifts, whose
+// latency, taken by every lane of a wave at the pace of its longest shift, set
+// the walk's time.  Keys are unique, so the list does not depend on the order
+// of the candidates.
+template <int K>
+struct KnnRegList {
+  const float4* pts;
+  float qx, qy, qz;
+  uint64_t key[K];
+  __device__ void reset() {
+#pragma unroll
+    for (int u = 0; u < K; ++u) key[u] = ~0ull;
+  }
+  __device__ void insert(const float4& t) {
+    union {
+      float f;
+      int32_t i;
+    } w;
+    w.f = t.w;
+    uint64_t kk = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), w.i);
+    if (kk >= key[K - 1]) return;
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const uint64_t a = key[u];
+      const bool lt = kk < a;
+      key[u] = lt ? kk : a;
+      kk = lt ? a : kk;
+    }
+  }
+  __device__ void scan(uint32_t b, uint32_t e) {
+    uint32_t j = b;
+    for (; j + 4 <= e; j += 4) {
+      float4 t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = pts[j + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) insert(t[u]);
+    }
+    for (; j < e; ++j) insert(pts[j]);
+  }
+  __device__ float bound() const { return key[K - 1] == ~0ull ? 3.4e38f : key_d2(key[K - 1]); }
+  __device__ int size() const {
+    int c = 0;
+#pragma unroll
+    for (int u = 0; u < K; ++u) c += key[u] != ~0ull;
+    return c;
+  }
+  __device__ uint64_t first() const { return key[0]; }
+  __device__ uint64_t last() const { return key[K - 1]; }
+  template <class F>
+  __device__ void for_each(F f) const {
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (key[u] != ~0ull) f(key[u]);
+  }
 };
 
 __device__ void jacobi3(double* a, double* V) {
@@ -1199,11 +1269,16 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
 // out[4 (p * gridDim.y + s) + {0, 1, 2, 3}] = {sum of squared plane
 // distances, planar inliers, inliers, sampled points}, which the host adds
 // in fixed order.
-// The k-NN lists live in dynamic LDS sized for k and hold keys only (k *
-// kNrmBlock keys; a neighbour's coordinates are read back from the raw target
-// cloud by the original index in its key's low word): at k = 20 eight
-// workgroups fit a CU.
-__global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max_pts, float r2f, double max_curv, double* out) {
+// k = 20 (the default) keeps each k-NN list in registers (KnnRegList);
+// other k keep theirs in dynamic LDS sized for k, holding keys only (k *
+// kNrmBlock keys).  Either way a neighbour's coordinates are read back from
+// the raw target cloud by the original index in its key's low word.  Three
+// waves per SIMD: the register list costs 40 VGPRs, and holding the kernel
+// to 168 VGPRs (a few spilled) measured 0.99 ms on 256 pairs x 1024 points
+// against 1.17 ms at two waves and 1.19 ms at four (more spills).
+template <int K>
+__global__ __launch_bounds__(kNrmBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_judge(
+    DevEngine e, int k, int max_pts, float r2f, double max_curv, double* out) {
   extern __shared__ uint64_t s_dyn[];
   uint64_t* const s_key = s_dyn;
   const int prob = blockIdx.x;
@@ -1215,52 +1290,54 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
   tf_rows_f32(e.st[prob].T, m);
   const uint32_t n = sd.n;
   const uint32_t ms = n < (uint32_t)max_pts ? n : (uint32_t)max_pts;
+  const int kk = K ? K : k;
   double ssum = 0.0, cnt = 0.0, pcnt = 0.0;
   const uint32_t split = gridDim.y;
   for (uint32_t j = blockIdx.y + split * threadIdx.x; j < ms; j += split * kNrmBlock) {
     const uint32_t sj = sd.off + (uint32_t)(((uint64_t)j * n) / ms);
     float qx, qy, qz;
     apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
-    KnnVisitor v;
+    typename std::conditional<K == 0, KnnVisitor, KnnRegList<(K > 0 ? K : 1)>>::type v;
     v.pts = e.tgt_sorted;
     v.qx = qx;
     v.qy = qy;
     v.qz = qz;
-    v.k = k;
-    v.cnt = 0;
-    v.key = s_key + threadIdx.x;
-    v.slot = nullptr;
+    if constexpr (K == 0) {
+      v.k = k;
+      v.key = s_key + threadIdx.x;
+      v.slot = nullptr;
+    }
+    v.reset();
     // One walk bounded by the correspondence radius visits every target point
     // within it, so its nearest key is the exact nearest neighbour whenever
     // one lies within the radius (the inlier test), and when the k-th key is
     // within the radius too the list is the exact k-NN list.  Only an inlier
     // with fewer than k target points within the radius walks again, unbounded.
     ring_walk(g, qx, qy, qz, r2f, v);
-    if (v.cnt == 0 || !(key_d2(v.key[0]) <= r2f)) continue;
-    if (v.cnt < k || !(key_d2(v.key[(k - 1) * kNrmBlock]) <= r2f)) {
-      v.cnt = 0;
+    if (v.size() == 0 || !(key_d2(v.first()) <= r2f)) continue;
+    if (v.size() < kk || !(key_d2(v.last()) <= r2f)) {
+      v.reset();
       ring_walk(g, qx, qy, qz, FLT_MAX, v);
     }
     cnt += 1.0;
-    if (v.cnt < 3) continue;
+    const int nn = v.size();
+    if (nn < 3) continue;
     const float* const raw = e.tgt_raw + 3 * (size_t)td.raw_off;
     double mean[3] = {0, 0, 0};
-#pragma unroll 4
-    for (int i = 0; i < v.cnt; ++i) {
-      const float* p = raw + 3 * (size_t)(uint32_t)v.key[i * kNrmBlock];
+    v.for_each([&](uint64_t key) {
+      const float* p = raw + 3 * (size_t)(uint32_t)key;
       mean[0] += p[0];
       mean[1] += p[1];
       mean[2] += p[2];
-    }
-    for (int a = 0; a < 3; ++a) mean[a] /= v.cnt;
+    });
+    for (int a = 0; a < 3; ++a) mean[a] /= nn;
     double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 4
-    for (int i = 0; i < v.cnt; ++i) {
-      const float* p = raw + 3 * (size_t)(uint32_t)v.key[i * kNrmBlock];
+    v.for_each([&](uint64_t key) {
+      const float* p = raw + 3 * (size_t)(uint32_t)key;
       const double d[3] = {p[0] - mean[0], p[1] - mean[1], p[2] - mean[2]};
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) C[r * 3 + c] += d[r] * d[c];
-    }
+    });
     double V[9];
     jacobi3(C, V);
     int mi = 0;
@@ -1276,9 +1353,9 @@ __global__ __launch_bounds__(kNrmBlock) void k_judge(DevEngine e, int k, int max
   ssum = wave_sum_f64(ssum);
   cnt = wave_sum_f64(cnt);
   pcnt = wave_sum_f64(pcnt);
-  // the partials reuse the start of the list area (k >= 3 lists hold far more
-  // than 3 doubles per wave) once every thread is done with its list: no
-  // static LDS, so eight 20 KB workgroups fill a CU's 160 KB at k = 20
+  // the partials reuse the start of the dynamic LDS (for LDS lists, k >= 3
+  // lists hold far more than 3 doubles per wave) once every thread is done
+  // with its list: no static LDS
   double(*s_red)[kNrmBlock / 64] = reinterpret_cast<double(*)[kNrmBlock / 64]>(s_dyn);
   __syncthreads();
   const int w = threadIdx.x >> 6;
@@ -1307,9 +1384,15 @@ void launch_judge(const DevEngine& e, int k, int max_pts, float r2f, double max_
                   hipStream_t s) {
   if (e.n_prob == 0) return;
   if (k > kMaxK) k = kMaxK;
+  const size_t red = 3 * (kNrmBlock / 64) * sizeof(double);
+  if (k == 20) {
+    hipLaunchKernelGGL(k_judge<20>, dim3(e.n_prob, kJudgeSplit), dim3(kNrmBlock), red, s, e, k, max_pts, r2f,
+                       max_curv, out);
+    return;
+  }
   const size_t lds = (size_t)k * kNrmBlock * sizeof(uint64_t);
-  hipLaunchKernelGGL(k_judge, dim3(e.n_prob, kJudgeSplit), dim3(kNrmBlock), lds, s, e, k, max_pts, r2f, max_curv,
-                     out);
+  hipLaunchKernelGGL(k_judge<0>, dim3(e.n_prob, kJudgeSplit), dim3(kNrmBlock), lds > red ? lds : red, s, e, k,
+                     max_pts, r2f, max_curv, out);
 }
 
 }  // namespace rtr
