@@ -362,7 +362,8 @@ static_assert(kFpsCells * 4 == kFpsMax * 2, "the inverse order (u16) fills the h
 // index comes from the inverse order in LDS (inv[j], j the sorted position;
 // jw = j of lane 0 in this slot).
 __device__ __forceinline__ uint64_t fps_group_update(float px, float py, float pz, float& dist, const uint16_t* inv,
-                                                     int jw, int lane, int k, float qx, float qy, float qz) {
+                                                     int jw, int lane, int k, float qx, float qy, float qz,
+                                                     uint64_t old) {
   // an empty asm keeps the compiler from computing every slot's distances
   // ahead of the branch that selects the slot: they are pure and would be
   // hoisted or speculated
@@ -371,7 +372,13 @@ __device__ __forceinline__ uint64_t fps_group_update(float px, float py, float p
   float d = dx * dx;
   d = d + dy * dy;
   d = d + dz * dz;
-  dist = fminf(dist, d);  // padding slots hold -1 and keep it
+  const float nd = fminf(dist, d);  // padding slots hold -1 and keep it
+  const uint64_t lowered = __ballot(nd < dist);
+  dist = nd;
+  // distances only fall: while the lane holding the group's key keeps its
+  // distance, it still holds the largest one, and among equal largest
+  // distances still the smallest index, so the key stands
+  if (!((lowered >> (old & 63)) & 1ull)) return old;
   const float gmax = wave_max_f32(dist);
   if (!(gmax >= 0.f)) return 0ull;  // padding only
   const uint64_t hit = __ballot(dist == gmax);
@@ -615,8 +622,8 @@ __global__ __launch_bounds__(kFpsThreads, 4) void k_fps(const float* raw, const 
 #pragma unroll
       for (int k = 0; k < kFpsPer; ++k) {
         if (need & (1ull << k)) {
-          const uint64_t key =
-              fps_group_update(px[k], py[k], pz[k], dist[k], inv, w * 64 + k * kFpsThreads, lane, k, qx, qy, qz);
+          const uint64_t key = fps_group_update(px[k], py[k], pz[k], dist[k], inv, w * 64 + k * kFpsThreads, lane, k,
+                                                qx, qy, qz, readlane_u64(gkey, k));
           if (lane == k) {
             gkey = key;
             gbest = key ? __uint_as_float((uint32_t)(key >> 32) - 1u) : -1.f;
