@@ -294,15 +294,17 @@ __global__ __launch_bounds__(kFThreads) void k_mlp_pool(const uint16_t* x0, int 
             acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, lds4(&sh.w3t[(half * 4 + t) * 32 + lr][ks * 8 + kh]),
                                                               acc[t], 0, 0, 0);
         }
+        // the pooled value is the BF16 rounding of the largest ReLU output:
+        // rounding is monotone, so rounding the maximum once equals the
+        // maximum of the rounded outputs; fmaxf(m, v) with m >= 0 is the
+        // ReLU (a NaN v leaves m, as ReLU maps NaN to 0)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int col = (half * 4 + t) * 32 + lr;
+          const float b = sh.b3[col];
           float m = vmax[half * 4 + t];
 #pragma unroll
-          for (int j = 0; j < 16; ++j) {
-            const float v = acc[t][j] + sh.b3[col];
-            m = fmaxf(m, bf2f(f2bf(v > 0.f ? v : 0.f)));
-          }
+          for (int j = 0; j < 16; ++j) m = fmaxf(m, acc[t][j] + b);
           vmax[half * 4 + t] = m;
         }
       }
@@ -312,7 +314,8 @@ __global__ __launch_bounds__(kFThreads) void k_mlp_pool(const uint16_t* x0, int 
 #pragma unroll
     for (int t = 0; t < kC3 / 32; ++t) {
       const float o = __shfl_xor(vmax[t], 32, 64);
-      const float m = fmaxf(vmax[t], o);
+      const float mx = fmaxf(vmax[t], o);
+      const float m = bf2f(f2bf(mx > 0.f ? mx : 0.f));  // a -0 maximum becomes +0, as the ReLU gives
       if (lane < 32) atomicMax(&sh.cmax[t * 32 + lane], __float_as_uint(m));  // m >= 0: bits order as floats
     }
     __syncthreads();
