@@ -426,6 +426,11 @@ int rtr_prop_train_step(rtr_prop* h, int n_pairs, const float* xyz, const double
  * N x K when tb), K split over k_split workgroups accumulated atomically. */
 int rtr_gemm_bf16(int device, const float* A, int ta, const float* B, int tb, float* C, int M, int N,
                   int K, int k_split);
+/* The fused point MLP rounds its activations to BF16 with the hardware
+ * conversion (v_cvt_pk_bf16_f32); the layer-by-layer path and the host round
+ * in software (round to nearest even).  Counts the float bit patterns, NaNs
+ * excepted, on which the two differ: identical features need 0. */
+int rtr_bf16_conversion_check(int device, unsigned long long* mismatches);
 
 /* ------------------------------------------------------------------ */
 /* Learned-init registration of many pairs per call: the network        */

@@ -165,6 +165,7 @@ _SIGS = {
     "rtr_prop_forward_backward": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rtr_prop_train_step": (_i, [_vp, _i, _vp, _vp, _f, C.POINTER(_f)]),
     "rtr_gemm_bf16": (_i, [_i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _i]),
+    "rtr_bf16_conversion_check": (_i, [_i, C.POINTER(C.c_ulonglong)]),
     "rtr_learned_config_default": (_i, [C.POINTER(LearnedConfig)]),
     "rtr_learned_create": (_i, [C.POINTER(_vp), _i, _i, C.POINTER(Params), C.POINTER(LearnedConfig)]),
     "rtr_learned_destroy": (_i, [_vp]),

@@ -208,6 +208,27 @@ struct FusedShared {
 
 __device__ __forceinline__ bf16x4 lds4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
 
+// BF16 of ReLU(v) by the hardware conversion (v_cvt_pk_bf16_f32, round to
+// nearest even): equal to f2bf for every non-NaN float, and the ReLU maps NaN
+// to 0 first
+__device__ __forceinline__ uint16_t bf16_relu(float v) {
+  return __builtin_bit_cast(uint16_t, (__bf16)(v > 0.f ? v : 0.f));
+}
+
+// The hardware conversion against f2bf over every non-NaN float: 16 bit
+// patterns per thread, 2^20 workgroups of 256 cover all 2^32.
+__global__ __launch_bounds__(256) void k_bf16_check(unsigned long long* bad) {
+  const uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16ull;
+  unsigned long long b = 0;
+  for (int i = 0; i < 16; ++i) {
+    const uint32_t u = (uint32_t)(base + (uint64_t)i);
+    if ((u & 0x7fffffffu) > 0x7f800000u) continue;
+    const float f = __uint_as_float(u);
+    b += __builtin_bit_cast(uint16_t, (__bf16)f) != f2bf(f);
+  }
+  if (b) atomicAdd(bad, b);
+}
+
 // acc row of register j for this lane (32x32 MFMA output layout)
 __device__ __forceinline__ int mfma_row(int lane, int j) { return (j >> 2) * 8 + (lane >> 5) * 4 + (j & 3); }
 
@@ -247,7 +268,7 @@ __global__ __launch_bounds__(kFThreads) void k_mlp_pool(const uint16_t* x0, int 
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const float v = acc[j] + sh.b1[col];
-          act[mfma_row(lane, j)][col] = f2bf(v > 0.f ? v : 0.f);
+          act[mfma_row(lane, j)][col] = bf16_relu(v);
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -273,7 +294,7 @@ __global__ __launch_bounds__(kFThreads) void k_mlp_pool(const uint16_t* x0, int 
 #pragma unroll
           for (int j = 0; j < 16; ++j) {
             const float v = acc[nt][j] + sh.b2[col];
-            act[mfma_row(lane, j)][col] = f2bf(v > 0.f ? v : 0.f);
+            act[mfma_row(lane, j)][col] = bf16_relu(v);
           }
         }
       }
@@ -1519,6 +1540,20 @@ int rtr_prop_train_step(rtr_prop* h, int n_pairs, const float* xyz, const double
     for (float v : l) s += v;
     *loss_out = (float)(s / n_pairs);
   }
+  return RTR_OK;
+}
+
+int rtr_bf16_conversion_check(int device, unsigned long long* mismatches) {
+  using namespace rtr;
+  if (!mismatches) return RTR_ERR_INVALID_ARG;
+  RTR_HIP(hipSetDevice(device));
+  DevBuf d;
+  RTR_TRY(d.ensure(sizeof(unsigned long long)));
+  RTR_HIP(hipMemset(d.p, 0, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_bf16_check, dim3(1u << 20), dim3(256), 0, nullptr, d.as<unsigned long long>());
+  RTR_HIP(hipGetLastError());
+  RTR_HIP(hipMemcpy(mismatches, d.p, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  d.release();
   return RTR_OK;
 }
 

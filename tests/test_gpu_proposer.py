@@ -320,3 +320,17 @@ def test_fused_point_mlp_is_bit_identical():
     p2, s2, b2 = prop.propose_multi(Q)
     assert np.array_equal(p1, p2) and np.array_equal(s1, s2) and np.array_equal(b1, b2)
     assert np.array_equal(prop.propose(), p2[:, 0])
+
+
+def test_hardware_bf16_conversion_matches_software_rounding():
+    """The fused point MLP rounds its hidden activations with the hardware
+    conversion (v_cvt_pk_bf16_f32), the layer-by-layer path and the host with
+    software round-to-nearest-even; the fused kernel's features are
+    bit-identical to the layer-by-layer ones only if the two agree on every
+    float that can reach them.  All 2^32 bit patterns but the NaNs (which the
+    ReLU turns into 0 before any rounding) are compared on the device."""
+    import ctypes as C
+    from realtime_robot_amd import _lib as L
+    n = C.c_ulonglong(1)
+    L.check(L.lib.rtr_bf16_conversion_check(0, C.byref(n)))
+    assert n.value == 0
