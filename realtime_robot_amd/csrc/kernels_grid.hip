@@ -335,7 +335,9 @@ __global__ void k_tgt_scatter(DevEngine e) {
 
 // Separable Chebyshev distance transform over each target's cells, capped at
 // dt_cap + 1: pass x from the occupancy (cell_start differences), then y, then
-// z, each a min over |d| <= cap of max(|d|, previous pass at offset d).
+// z, each a min over |d| <= cap of max(|d|, previous pass at offset d).  The
+// offsets go outward by |d| and stop once |d| reaches the best value so far:
+// no farther offset can give less than its |d|.
 template <int AXIS>
 __global__ void k_tgt_dist(DevEngine e, const uint8_t* src, uint8_t* dst) {
   const TargetDesc& t = e.tgts[blockIdx.y];
@@ -344,19 +346,22 @@ __global__ void k_tgt_dist(DevEngine e, const uint8_t* src, uint8_t* dst) {
   const int len = AXIS == 0 ? t.dx : (AXIS == 1 ? t.dy : t.dz);
   const int stride = AXIS == 0 ? 1 : (AXIS == 1 ? t.dx : t.dx * t.dy);
   const uint32_t* cs = e.cell_start + t.cell_off;
+  auto val = [&](uint32_t cc) -> int {
+    if (AXIS == 0) return cs[cc + 1] > cs[cc] ? 0 : K + 1;
+    return src[t.cell_off + cc];
+  };
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
     const int pos = AXIS == 0 ? (int)(c % t.dx) : (AXIS == 1 ? (int)((c / t.dx) % t.dy) : (int)(c / ((uint32_t)t.dx * t.dy)));
-    int best = K + 1;
-    const int dlo = -min(K, pos), dhi = min(K, len - 1 - pos);
-    for (int d = dlo; d <= dhi; ++d) {
-      const uint32_t cc = (uint32_t)((int)c + d * stride);
-      int v;
-      if (AXIS == 0)
-        v = cs[cc + 1] > cs[cc] ? 0 : K + 1;
-      else
-        v = src[t.cell_off + cc];
-      const int ad = d < 0 ? -d : d;
-      v = v > ad ? v : ad;
+    int best = val(c);
+    const int lo = min(K, pos), hi = min(K, len - 1 - pos);
+    for (int a = 1; a < best && (a <= lo || a <= hi); ++a) {
+      int v = K + 1;
+      if (a <= lo) v = val(c - (uint32_t)(a * stride));
+      if (a <= hi) {
+        const int w = val(c + (uint32_t)(a * stride));
+        v = w < v ? w : v;
+      }
+      v = v > a ? v : a;
       best = v < best ? v : best;
     }
     dst[t.cell_off + c] = (uint8_t)best;
