@@ -1091,8 +1091,21 @@ struct KnnRegList {
     w.f = t.w;
     uint64_t kk = nn_key(d2_f32(qx, qy, qz, t.x, t.y, t.z), w.i);
     if (kk >= key[K - 1]) return;
+    // two halves: a candidate not below the first half's last key leaves
+    // that half as it is, and a wave skips the half when none of its lanes
+    // reaches it (four quarters spill)
+    constexpr int H = K / 2;
+    if (kk < key[H - 1]) {
 #pragma unroll
-    for (int u = 0; u < K; ++u) {
+      for (int u = 0; u < H; ++u) {
+        const uint64_t a = key[u];
+        const bool lt = kk < a;
+        key[u] = lt ? kk : a;
+        kk = lt ? a : kk;
+      }
+    }
+#pragma unroll
+    for (int u = H; u < K; ++u) {
       const uint64_t a = key[u];
       const bool lt = kk < a;
       key[u] = lt ? kk : a;
