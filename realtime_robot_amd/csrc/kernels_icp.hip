@@ -1321,17 +1321,18 @@ __global__ __launch_bounds__(kNrmBlock) __attribute__((amdgpu_waves_per_eu(3))) 
       v.slot = nullptr;
     }
     v.reset();
-    // One walk bounded by the correspondence radius visits every target point
-    // within it, so its nearest key is the exact nearest neighbour whenever
-    // one lies within the radius (the inlier test), and when the k-th key is
-    // within the radius too the list is the exact k-NN list.  Only an inlier
-    // with fewer than k target points within the radius walks again, unbounded.
-    ring_walk(g, qx, qy, qz, r2f, v);
-    if (v.size() == 0 || !(key_d2(v.first()) <= r2f)) continue;
-    if (v.size() < kk || !(key_d2(v.last()) <= r2f)) {
-      v.reset();
-      ring_walk(g, qx, qy, qz, FLT_MAX, v);
+    // A query whose cell's distance-field value reaches skip_ring has no
+    // target point within the correspondence radius: an outlier, no walk.
+    // Every other query takes one unbounded walk, whose list is the exact
+    // k-NN list and whose nearest key decides the inlier test.
+    {
+      int cx, cy, cz;
+      if (e.cell_dist && query_cell(g, qx, qy, qz, cx, cy, cz) &&
+          (int)e.cell_dist[td.cell_off + ((uint32_t)cz * g.dy + cy) * g.dx + cx] >= td.skip_ring)
+        continue;
     }
+    ring_walk(g, qx, qy, qz, FLT_MAX, v);
+    if (v.size() == 0 || !(key_d2(v.first()) <= r2f)) continue;
     cnt += 1.0;
     const int nn = v.size();
     if (nn < 3) continue;
