@@ -1265,7 +1265,7 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
 // registration judge (plane-distance RMSE of the inliers)
 // ---------------------------------------------------------------------------
 
-// One workgroup per problem.  A strided sample of up to max_pts source points
+// gridDim.y workgroups per problem.  A strided sample of up to max_pts source points
 // (in the stored order) is moved by the problem's final pose (float, the same
 // rounding as the ICP queries).  A point is an inlier when its nearest target
 // point lies within the correspondence radius (d2 <= r2f).  For an inlier the
