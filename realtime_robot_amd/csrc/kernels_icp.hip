@@ -1277,8 +1277,9 @@ void launch_target_normals(const DevEngine& e, int k, float vpx, float vpy, floa
 // surface.  Unlike the point-to-point MSE, this residual does not grow with
 // the target's sampling spacing, so one threshold serves sparse and dense
 // targets.
-// The sample of problem p is split over gridDim.y workgroups (sample j goes
-// to workgroup j % gridDim.y); workgroup s writes its partial sums to
+// The sample of problem p is split over gridDim.y workgroups in consecutive
+// runs (the stored order is spatially sorted, so the lanes of a wave hold
+// nearby points and walk alike); workgroup s writes its partial sums to
 // out[4 (p * gridDim.y + s) + {0, 1, 2, 3}] = {sum of squared plane
 // distances, planar inliers, inliers, sampled points}, which the host adds
 // in fixed order.
@@ -1306,7 +1307,9 @@ __global__ __launch_bounds__(kNrmBlock) __attribute__((amdgpu_waves_per_eu(3))) 
   const int kk = K ? K : k;
   double ssum = 0.0, cnt = 0.0, pcnt = 0.0;
   const uint32_t split = gridDim.y;
-  for (uint32_t j = blockIdx.y + split * threadIdx.x; j < ms; j += split * kNrmBlock) {
+  const uint32_t per = (ms + split - 1) / split;
+  const uint32_t j0 = blockIdx.y * per, j1 = min(ms, j0 + per);
+  for (uint32_t j = j0 + threadIdx.x; j < j1; j += kNrmBlock) {
     const uint32_t sj = sd.off + (uint32_t)(((uint64_t)j * n) / ms);
     float qx, qy, qz;
     apply_f32(m, e.sx[sj], e.sy[sj], e.sz[sj], qx, qy, qz);
