@@ -691,8 +691,9 @@ int Engine::judge(int k, int max_pts, double max_curv, double* plane_rmse, doubl
   DevEngine e = dev_view();
   launch_judge(e, k, max_pts, e.prm.r2f, max_curv, reinterpret_cast<double*>(out_f32_.p), stream_);
   RTR_HIP(hipGetLastError());
-  std::vector<double> h(words);
-  RTR_HIP(hipMemcpyAsync(h.data(), out_f32_.p, words * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  RTR_TRY(ensure_pinned(words * sizeof(double)));
+  const double* const h = static_cast<const double*>(pin_);
+  RTR_HIP(hipMemcpyAsync(pin_, out_f32_.p, words * sizeof(double), hipMemcpyDeviceToHost, stream_));
   RTR_HIP(hipStreamSynchronize(stream_));
   for (size_t p = 0; p < np; ++p) {
     double s = 0.0, pc = 0.0, c = 0.0, m = 0.0;

@@ -1037,6 +1037,18 @@ struct Proposer {
   bool fused_off = false;           // inference through the layer-by-layer products
   std::vector<uint32_t> h_off, h_cnt;
   float last_ms = 0.f;
+  void* pin = nullptr;  // pinned staging of the proposals read back each call
+  size_t pin_bytes = 0;
+
+  int ensure_pinned(size_t bytes) {
+    if (pin && bytes <= pin_bytes) return RTR_OK;
+    if (pin) (void)hipHostFree(pin);
+    pin = nullptr;
+    pin_bytes = 0;
+    RTR_HIP(hipHostMalloc(&pin, bytes, hipHostMallocDefault));
+    pin_bytes = bytes;
+    return RTR_OK;
+  }
 
   int init(int dev, int maxp) {
     device = dev;
@@ -1114,6 +1126,7 @@ struct Proposer {
     for (DevBuf* b : all) b->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (pin) (void)hipHostFree(pin);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1424,11 +1437,15 @@ int rtr_prop_propose_multi(rtr_prop* h, int k, const double* rots_in, float tau,
   P.enqueue_forward_multi(k, tau * tau);
   RTR_HIP(hipGetLastError());
   RTR_HIP(hipEventRecord(P.ev1, P.stream));
-  std::vector<double> pose((size_t)k * n * 12), sc((size_t)k * n);
-  std::vector<float> cen((size_t)2 * n * 3);
-  RTR_HIP(hipMemcpyAsync(pose.data(), P.pose.p, pose.size() * 8, hipMemcpyDeviceToHost, P.stream));
-  RTR_HIP(hipMemcpyAsync(sc.data(), P.score.p, sc.size() * 8, hipMemcpyDeviceToHost, P.stream));
-  RTR_HIP(hipMemcpyAsync(cen.data(), P.cen.p, cen.size() * 4, hipMemcpyDeviceToHost, P.stream));
+  // read back through pinned staging (poses, scores, centroids)
+  const size_t npose = (size_t)k * n * 12, nsc = (size_t)k * n, ncen = (size_t)2 * n * 3;
+  RTR_TRY(P.ensure_pinned((npose + nsc) * 8 + ncen * 4));
+  double* const pose = static_cast<double*>(P.pin);
+  double* const sc = pose + npose;
+  float* const cen = reinterpret_cast<float*>(sc + nsc);
+  RTR_HIP(hipMemcpyAsync(pose, P.pose.p, npose * 8, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipMemcpyAsync(sc, P.score.p, nsc * 8, hipMemcpyDeviceToHost, P.stream));
+  RTR_HIP(hipMemcpyAsync(cen, P.cen.p, ncen * 4, hipMemcpyDeviceToHost, P.stream));
   RTR_HIP(hipStreamSynchronize(P.stream));
   RTR_HIP(hipEventElapsedTime(&P.last_ms, P.ev0, P.ev1));
   for (int p = 0; p < n; ++p) {
