@@ -256,8 +256,8 @@ def launch_roofline(reg, guesses, pair_of, srcs, tgts, n_sample, coarse_stride=1
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--starts", type=int, default=8)
     ap.add_argument("--hypotheses", type=int, default=8)
