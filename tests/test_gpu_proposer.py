@@ -232,6 +232,54 @@ def test_judge_matches_brute_force_restatement(k):
         np.testing.assert_allclose(rmse[p], np.sqrt(np.sum(dist[planar] ** 2) / n_pl), rtol=1e-9)
 
 
+@pytest.mark.parametrize("k", [20, 12])
+def test_judge_on_targets_smaller_than_k(k):
+    """Targets of 5, 9, 15 and 40 points (some below k): a query's list then
+    holds every target point, and the plane fit runs over all of them.  Counts
+    exactly and the plane RMSE to 1e-9 relative against brute force (the
+    neighbour set is the whole target when it has at most k points) and an
+    FP64 PCA; sources hold near-plane points, some farther than the radius
+    from every target point (outliers), some with a 3D spread."""
+    rng = np.random.default_rng(31)
+    sizes = [5, 9, 15, 40]
+    srcs, tgts = [], []
+    for nt in sizes:
+        t = np.column_stack([rng.uniform(0, 0.08, nt), rng.uniform(0, 0.08, nt), rng.normal(0, 2e-4, nt)])
+        s = np.column_stack([rng.uniform(-0.02, 0.1, 400), rng.uniform(-0.02, 0.1, 400), rng.normal(0, 1e-3, 400)])
+        s[:40, 2] += rng.uniform(0.06, 0.2, 40)           # off the plane: beyond the radius
+        tgts.append(t.astype(np.float32))
+        srcs.append(s.astype(np.float32))
+    n = len(sizes)
+    r, curv = 0.05, 0.05
+    b = R.BatchRegistration()
+    b.set_clouds(srcs, tgts)
+    b.set_problems(range(n), range(n))
+    b.set_params(R.make_params(max_iterations=1, max_correspondence_distance=r))
+    res = b.align([np.eye(4)] * n)
+    rmse, frac, pfrac = b.judge(k, 4096, curv)
+    for p in range(n):
+        q = oracle.transform(res[p]["T"], srcs[p])
+        idx, _ = oracle.nn_brute(q, tgts[p], r)
+        inl = idx >= 0
+        T64 = tgts[p].astype(np.float64)
+        if len(T64) <= k:
+            nb = np.broadcast_to(T64, (int(inl.sum()),) + T64.shape)
+        else:
+            kidx, _ = oracle.KdTree(tgts[p]).knn(q[inl], k)
+            nb = T64[kidx]
+        mean = nb.mean(1)
+        d = nb - mean[:, None, :]
+        C = np.matmul(np.transpose(d, (0, 2, 1)), d)
+        w, V = np.linalg.eigh(C)
+        planar = (w.sum(1) > 0) & (w[:, 0] <= curv * w.sum(1))
+        dist = np.sum(V[:, :, 0] * (q[inl].astype(np.float64) - mean), axis=1)
+        n_inl, n_pl = int(inl.sum()), int(planar.sum())
+        print(f"judge small target {len(T64)} points (k {k}): {len(q)} queries, {n_inl} inliers, {n_pl} planar")
+        assert 0 < n_inl < len(q) and n_pl > 0
+        assert round(frac[p] * len(q)) == n_inl and round(pfrac[p] * n_inl) == n_pl
+        np.testing.assert_allclose(rmse[p], np.sqrt(np.sum(dist[planar] ** 2) / n_pl), rtol=1e-9)
+
+
 WEIGHTS = learned.WEIGHTS_PATH
 
 
