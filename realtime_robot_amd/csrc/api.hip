@@ -9,6 +9,7 @@
 
 #include "../../include/rtr.h"
 #include "engine.hpp"
+#include "internal.h"
 #include "geom.hpp"
 #include "guard.hpp"
 
@@ -345,6 +346,17 @@ int rtr_batch_set_clouds(rtr_batch* b, int n_src, const float* const* src, const
     if (nt[j] && !tgt[j]) return RTR_ERR_INVALID_ARG;
   return b->eng.set_clouds(n_src, src, ns, nullptr, n_tgt, tgt, nt, nullptr);
   RTR_GUARD_END
+}
+
+int rtr_batch_device_clouds(const rtr_batch* b, const float** src_xyz, const float** tgt_xyz) {
+  if (!b || !src_xyz || !tgt_xyz) return RTR_ERR_INVALID_ARG;
+  *src_xyz = b->eng.source_xyz();
+  *tgt_xyz = b->eng.target_xyz();
+  if (!*src_xyz || !*tgt_xyz) {
+    set_error("rtr_batch_device_clouds: no clouds are resident");
+    return RTR_ERR_INVALID_ARG;
+  }
+  return RTR_OK;
 }
 
 int rtr_batch_set_target_normals(rtr_batch* b, const float* const* nrm) {

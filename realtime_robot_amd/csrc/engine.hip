@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <thread>
+#include <vector>
 
 namespace rtr {
 
@@ -84,6 +86,57 @@ static void pack_xyz(const float* src, size_t n, size_t stride, float* dst) {
   for (size_t i = 0; i < n; ++i) memcpy(dst + 3 * i, b + i * stride, 12);
 }
 
+// Host clouds to one packed xyz array on the device (cloud c at point offset
+// off[c], offsets ascending, total points in all): the clouds are packed into
+// the pinned staging buffer a chunk at a time, by several host threads at
+// once, and each chunk's copy is queued on the stream as soon as it is
+// packed, so it runs while the next chunk is packed.  Returns once the last
+// copy has finished.
+int Engine::upload_xyz(int count, const float* const* clouds, const size_t* n, const size_t* stride,
+                       const size_t* off, size_t total, void* dst) {
+  if (total == 0) return RTR_OK;
+  RTR_TRY(ensure_pinned(total * 12));
+  float* const pin = static_cast<float*>(pin_);
+  constexpr size_t kChunkPts = (size_t)2 << 20;   // 24 MB per chunk
+  constexpr size_t kMinPerThread = (size_t)1 << 17;
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned max_threads = hw == 0 ? 1u : std::min(hw, 8u);
+  // points [lo, hi) of the concatenation, from every cloud they cover
+  auto pack_range = [&](size_t lo, size_t hi) {
+    int c = (int)(std::upper_bound(off, off + count, lo) - off) - 1;
+    for (; c < count && lo < hi; ++c) {
+      if (c < 0 || off[c] + n[c] <= lo) continue;
+      const size_t a = lo - off[c], e = std::min(n[c], hi - off[c]);
+      const size_t s = (stride && stride[c]) ? stride[c] : 12;
+      pack_xyz(reinterpret_cast<const float*>(reinterpret_cast<const char*>(clouds[c]) + a * s), e - a, s,
+               pin + 3 * lo);
+      lo = off[c] + e;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (size_t a = 0; a < total; a += kChunkPts) {
+    const size_t b = std::min(total, a + kChunkPts);
+    const unsigned t = (unsigned)std::max<size_t>(1, std::min<size_t>(max_threads, (b - a) / kMinPerThread));
+    const size_t per = (b - a + t - 1) / t;
+    pool.clear();
+    for (unsigned i = 1; i < t; ++i) {
+      const size_t lo = a + i * per, hi = std::min(b, lo + per);
+      if (lo >= hi) continue;
+      try {
+        pool.emplace_back(pack_range, lo, hi);
+      } catch (...) {   // no thread to be had: this one packs the part
+        pack_range(lo, hi);
+      }
+    }
+    pack_range(a, std::min(b, a + per));
+    for (std::thread& th : pool) th.join();
+    RTR_HIP(hipMemcpyAsync(static_cast<char*>(dst) + 12 * a, pin + 3 * a, (b - a) * 12, hipMemcpyHostToDevice,
+                           stream_));
+  }
+  RTR_HIP(hipStreamSynchronize(stream_));
+  return RTR_OK;
+}
+
 static uint32_t source_bins(size_t n) {
   int P = 1;
   double want = std::cbrt((double)n / 4.0);
@@ -122,11 +175,11 @@ int Engine::set_clouds(int n_src, const float* const* src, const size_t* ns, con
     src_total_ = st;
     n_bins_ = boff;
     RTR_TRY(src_raw_.ensure(st * 12));
-    RTR_TRY(ensure_pinned(st * 12));
-    for (int i = 0; i < n_src; ++i)
-      pack_xyz(src[i], ns[i], s_stride ? s_stride[i] : 12, static_cast<float*>(pin_) + 3 * (size_t)srcs_[i].off);
-    RTR_HIP(hipMemcpyAsync(src_raw_.p, pin_, st * 12, hipMemcpyHostToDevice, stream_));
-    RTR_HIP(hipStreamSynchronize(stream_));
+    {
+      std::vector<size_t> off(n_src);
+      for (int i = 0; i < n_src; ++i) off[i] = srcs_[i].off;
+      RTR_TRY(upload_xyz(n_src, src, ns, s_stride, off.data(), st, src_raw_.p));
+    }
     RTR_TRY(sx_.ensure(st * 4));
     RTR_TRY(sy_.ensure(st * 4));
     RTR_TRY(sz_.ensure(st * 4));
@@ -149,11 +202,11 @@ int Engine::set_clouds(int n_src, const float* const* src, const size_t* ns, con
     }
     tgt_total_ = tt;
     RTR_TRY(tgt_raw_.ensure(tt * 12));
-    RTR_TRY(ensure_pinned(tt * 12));
-    for (int j = 0; j < n_tgt; ++j)
-      pack_xyz(tgt[j], nt[j], t_stride ? t_stride[j] : 12, static_cast<float*>(pin_) + 3 * (size_t)tgts_[j].raw_off);
-    RTR_HIP(hipMemcpyAsync(tgt_raw_.p, pin_, tt * 12, hipMemcpyHostToDevice, stream_));
-    RTR_HIP(hipStreamSynchronize(stream_));
+    {
+      std::vector<size_t> off(n_tgt);
+      for (int j = 0; j < n_tgt; ++j) off[j] = tgts_[j].raw_off;
+      RTR_TRY(upload_xyz(n_tgt, tgt, nt, t_stride, off.data(), tt, tgt_raw_.p));
+    }
     RTR_TRY(tgt_sorted_.ensure(tt * 16));
     RTR_TRY(cell_id_.ensure(tt * 4));
     RTR_TRY(cell_rank_.ensure(tt * 4));

@@ -58,6 +58,10 @@ class Engine {
                  int n_tgt, const float* const* tgt, const size_t* nt, const size_t* t_stride);
   int set_target_normals(const float* const* nrm, const size_t* stride);
   int compute_target_normals(int k, const float vp[3]);
+  // the resident clouds as packed xyz (12 B per point): sources in their
+  // order, then targets in theirs (set_clouds)
+  const float* source_xyz() const { return static_cast<const float*>(src_raw_.p); }
+  const float* target_xyz() const { return static_cast<const float*>(tgt_raw_.p); }
   int set_problems(int n, const int* src_id, const int* tgt_id);
   int set_params(const rtr_params& p);
   const rtr_params& params() const { return prm_; }
@@ -156,6 +160,8 @@ class Engine {
   void* pin_ = nullptr;
   size_t pin_bytes_ = 0;
   int ensure_pinned(size_t bytes);
+  int upload_xyz(int count, const float* const* clouds, const size_t* n, const size_t* stride,
+                 const size_t* off, size_t total, void* dst);
 };
 
 }  // namespace rtr

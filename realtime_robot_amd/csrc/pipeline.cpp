@@ -24,6 +24,7 @@
 
 #include "../../include/rtr.h"
 #include "guard.hpp"
+#include "internal.h"
 
 using rtr::set_error;
 
@@ -412,8 +413,14 @@ int rtr_learned_set_clouds(rtr_learned* h, int n, const float* const* src, const
       return RTR_ERR_EMPTY_INPUT;
     }
   h->n = 0;
-  RTR_TRY(rtr_prop_set_clouds(h->prop, n, src, ns, tgt, nt));
+  // one upload from host memory (the refine engine's); the network takes
+  // its copy of the clouds on the device
   RTR_TRY(rtr_batch_set_clouds(h->reg, n, src, ns, n, tgt, nt));
+  {
+    const float *dsrc = nullptr, *dtgt = nullptr;
+    RTR_TRY(rtr_batch_device_clouds(h->reg, &dsrc, &dtgt));
+    RTR_TRY(rtr_prop_set_device_clouds(h->prop, n, dsrc, ns, dtgt, nt));
+  }
   std::vector<int> ids(n);
   for (int i = 0; i < n; ++i) ids[i] = i;
   RTR_TRY(rtr_batch_set_problems(h->reg, n, ids.data(), ids.data()));

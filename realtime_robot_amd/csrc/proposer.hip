@@ -19,6 +19,7 @@
 
 #include "../../include/rtr.h"
 #include "engine.hpp"
+#include "internal.h"
 #include "wave.hpp"
 
 namespace rtr {
@@ -1180,6 +1181,49 @@ struct Proposer {
     return RTR_OK;
   }
 
+  // The same resident clouds from packed xyz already on this device: every
+  // source, then every target, each group in pair order (two device copies).
+  int set_device_clouds(int np, const float* src_xyz, const size_t* ns, const float* tgt_xyz, const size_t* nt) {
+    RTR_HIP(hipSetDevice(device));
+    if (np < 1 || np > max_pairs) {
+      set_error("pairs %d outside [1, %d]", np, max_pairs);
+      return RTR_ERR_INVALID_ARG;
+    }
+    size_t ts = 0, tt = 0;
+    for (int p = 0; p < np; ++p) {
+      ts += ns[p];
+      tt += nt[p];
+    }
+    if (ts + tt >= (1ull << 32)) {
+      set_error("clouds too large (%zu points)", ts + tt);
+      return RTR_ERR_CAPACITY;
+    }
+    n_pairs = np;
+    n_resident = 0;
+    h_off.assign(2 * np, 0);
+    h_cnt.assign(2 * np, 0);
+    size_t a = 0, b = ts;
+    for (int p = 0; p < np; ++p) {
+      h_off[2 * p] = (uint32_t)a;
+      h_cnt[2 * p] = (uint32_t)ns[p];
+      a += ns[p];
+      h_off[2 * p + 1] = (uint32_t)b;
+      h_cnt[2 * p + 1] = (uint32_t)nt[p];
+      b += nt[p];
+    }
+    int rc = raw.ensure((ts + tt) * 12);
+    if (!rc) rc = roff.ensure(2 * np * 4);
+    if (!rc) rc = rcnt.ensure(2 * np * 4);
+    if (rc) return rc;
+    RTR_HIP(hipMemcpyAsync(raw.p, src_xyz, ts * 12, hipMemcpyDeviceToDevice, stream));
+    RTR_HIP(hipMemcpyAsync(raw.as<float>() + 3 * ts, tgt_xyz, tt * 12, hipMemcpyDeviceToDevice, stream));
+    RTR_HIP(hipMemcpyAsync(roff.p, h_off.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
+    RTR_HIP(hipMemcpyAsync(rcnt.p, h_cnt.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
+    RTR_HIP(hipStreamSynchronize(stream));
+    n_resident = np;
+    return RTR_OK;
+  }
+
   // FPS of the resident clouds into the network buffers (n_pairs becomes the
   // resident count).
   void enqueue_fps() {
@@ -1355,6 +1399,12 @@ int rtr_prop_set_clouds(rtr_prop* h, int n_pairs, const float* const* src, const
                         const float* const* tgt, const size_t* nt) {
   PROP_GUARD(if (!h || !src || !ns || !tgt || !nt) return RTR_ERR_INVALID_ARG;
              return h->p.set_clouds(n_pairs, src, ns, tgt, nt);)
+}
+
+int rtr_prop_set_device_clouds(rtr_prop* h, int n_pairs, const float* src_xyz, const size_t* ns,
+                               const float* tgt_xyz, const size_t* nt) {
+  PROP_GUARD(if (!h || !src_xyz || !ns || !tgt_xyz || !nt) return RTR_ERR_INVALID_ARG;
+             return h->p.set_device_clouds(n_pairs, src_xyz, ns, tgt_xyz, nt);)
 }
 
 // FPS + forward on the resident clouds; poses (n*16 column-major) map each
