@@ -88,15 +88,22 @@ static void pack_xyz(const float* src, size_t n, size_t stride, float* dst) {
 
 // Host clouds to one packed xyz array on the device (cloud c at point offset
 // off[c], offsets ascending, total points in all): the clouds are packed into
-// the pinned staging buffer a chunk at a time, by several host threads at
+// the pinned staging buffer (grown to the total when smaller) a chunk at a time, by several host threads at
 // once, and each chunk's copy is queued on the stream as soon as it is
 // packed, so it runs while the next chunk is packed.  Returns once the last
 // copy has finished.
-int Engine::upload_xyz(int count, const float* const* clouds, const size_t* n, const size_t* stride,
-                       const size_t* off, size_t total, void* dst) {
+int upload_xyz_staged(hipStream_t stream, void** staging, size_t* staging_bytes, int count,
+                      const float* const* clouds, const size_t* n, const size_t* stride, const size_t* off,
+                      size_t total, void* dst) {
   if (total == 0) return RTR_OK;
-  RTR_TRY(ensure_pinned(total * 12));
-  float* const pin = static_cast<float*>(pin_);
+  if (!*staging || *staging_bytes < total * 12) {
+    if (*staging) (void)hipHostFree(*staging);
+    *staging = nullptr;
+    *staging_bytes = 0;
+    RTR_HIP(hipHostMalloc(staging, total * 12, hipHostMallocDefault));
+    *staging_bytes = total * 12;
+  }
+  float* const pin = static_cast<float*>(*staging);
   constexpr size_t kChunkPts = (size_t)2 << 20;   // 24 MB per chunk
   constexpr size_t kMinPerThread = (size_t)1 << 17;
   const unsigned hw = std::thread::hardware_concurrency();
@@ -131,10 +138,15 @@ int Engine::upload_xyz(int count, const float* const* clouds, const size_t* n, c
     pack_range(a, std::min(b, a + per));
     for (std::thread& th : pool) th.join();
     RTR_HIP(hipMemcpyAsync(static_cast<char*>(dst) + 12 * a, pin + 3 * a, (b - a) * 12, hipMemcpyHostToDevice,
-                           stream_));
+                           stream));
   }
-  RTR_HIP(hipStreamSynchronize(stream_));
+  RTR_HIP(hipStreamSynchronize(stream));
   return RTR_OK;
+}
+
+int Engine::upload_xyz(int count, const float* const* clouds, const size_t* n, const size_t* stride,
+                       const size_t* off, size_t total, void* dst) {
+  return upload_xyz_staged(stream_, &pin_, &pin_bytes_, count, clouds, n, stride, off, total, dst);
 }
 
 static uint32_t source_bins(size_t n) {

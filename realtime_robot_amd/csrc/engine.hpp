@@ -48,6 +48,15 @@ struct TuneParams {
   int fine_from = 0;              // passes from this one on run 256-point work items (0: never)
 };
 
+// Host clouds (xyz, `stride` bytes apart, 12 when stride is null or 0) to one
+// packed xyz array at dst on the device, cloud c at point offset off[c]
+// (ascending), through the caller's pinned staging buffer (grown as needed):
+// host threads pack it a chunk at a time while the previous chunk's copy
+// runs on `stream`; returns once every copy has finished.
+int upload_xyz_staged(hipStream_t stream, void** staging, size_t* staging_bytes, int count,
+                      const float* const* clouds, const size_t* n, const size_t* stride, const size_t* off,
+                      size_t total, void* dst);
+
 class Engine {
  public:
   explicit Engine(int device);

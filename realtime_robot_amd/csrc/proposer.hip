@@ -1164,15 +1164,28 @@ struct Proposer {
       h_cnt[2 * p + 1] = (uint32_t)nt[p];
       tot += nt[p];
     }
+    if (tot >= (1ull << 32)) {
+      set_error("clouds too large (%zu points)", tot);
+      return RTR_ERR_CAPACITY;
+    }
     int rc = raw.ensure(tot * 12);
     if (!rc) rc = roff.ensure(2 * np * 4);
     if (!rc) rc = rcnt.ensure(2 * np * 4);
     if (rc) return rc;
-    for (int p = 0; p < np; ++p) {
-      RTR_HIP(hipMemcpyAsync(raw.as<float>() + 3 * (size_t)h_off[2 * p], src[p], ns[p] * 12,
-                             hipMemcpyHostToDevice, stream));
-      RTR_HIP(hipMemcpyAsync(raw.as<float>() + 3 * (size_t)h_off[2 * p + 1], tgt[p], nt[p] * 12,
-                             hipMemcpyHostToDevice, stream));
+    {
+      // pairs interleaved: source p, then target p
+      std::vector<const float*> cl(2 * np);
+      std::vector<size_t> cn(2 * np), co(2 * np);
+      for (int p = 0; p < np; ++p) {
+        cl[2 * p] = src[p];
+        cl[2 * p + 1] = tgt[p];
+      }
+      for (int c = 0; c < 2 * np; ++c) {
+        cn[c] = h_cnt[c];
+        co[c] = h_off[c];
+      }
+      RTR_TRY(upload_xyz_staged(stream, &pin, &pin_bytes, 2 * np, cl.data(), cn.data(), nullptr, co.data(), tot,
+                                raw.p));
     }
     RTR_HIP(hipMemcpyAsync(roff.p, h_off.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
     RTR_HIP(hipMemcpyAsync(rcnt.p, h_cnt.data(), 2 * np * 4, hipMemcpyHostToDevice, stream));
