@@ -676,7 +676,10 @@ def config3_leg(args, params, threads, timed):
     reg.set_clouds(srcs, tgts)
     reg.set_problems(range(n), range(n))
     guesses = np.repeat(np.eye(4)[None], n, 0)
-    tuned, tuned_ms, table3 = autotune.tune(reg, guesses, candidates=autotune.CANDIDATES, key=None, use_cache=False)
+    # one problem per pair, as in the learned refine launch: the candidates
+    # for few problems per launch (early retirement, 256-point tail items)
+    tuned, tuned_ms, table3 = autotune.tune(reg, guesses, candidates=autotune.LATENCY_CANDIDATES, key=None,
+                                            use_cache=False)
     log(f"config 3: autotune {tuned} ({tuned_ms:.1f} ms)")
     t3, res = timed(lambda: reg.align(guesses), args.steps, args.warmup)
     roof = launch_roofline(reg, guesses, np.arange(n), srcs, tgts, args.account_sample)
